@@ -1,0 +1,136 @@
+/*
+ * pmx.h -- C-ABI of libpmx, the MI355X implementation of ParMooN's CD3D/TCD3D
+ * hot path: element assembly with SUPG, geometric multigrid and the
+ * FGMRES/Richardson/BiCGStab solvers on one GPU.
+ *
+ * Conventions
+ *  - Every function returns 0 (PMX_OK) or a pmx_status; pmx_last_error()
+ *    returns the message of the last failure on the calling thread.  Nothing
+ *    calls exit() (the reference aborts with exit(4711) / exit(0)).
+ *  - Host arrays are borrowed for the duration of the call.  Vectors passed
+ *    as host arrays are in the reference DOF numbering (GlobalNumbers of
+ *    TFESpace3D); device storage is internal and lattice ordered.
+ *  - Parameters come from an explicit `const double prm[PMX_PRM_LEN]` block
+ *    (pmx_prm.h) instead of the global TDatabase::ParamDB.
+ *  - All work is ordered on the context's single HIP stream.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#include "pmx_prm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pmx_status {
+  PMX_OK = 0,
+  PMX_ERR_ARG = 1,       /* invalid argument                                  */
+  PMX_ERR_HIP = 2,       /* HIP runtime failure                               */
+  PMX_ERR_STATE = 3,     /* call out of order (e.g. solve before assemble)    */
+  PMX_ERR_NOCONV = 4,    /* iteration limit reached (cf. FgmresIte.C:604-612) */
+  PMX_ERR_DIVERGED = 5   /* residual > SC_DIV_FACTOR * initial (FgmresIte.C:535, exit(4711) there) */
+};
+
+typedef struct pmx_ctx pmx_ctx;
+typedef struct pmx_system pmx_system;
+
+/* ---- context ----------------------------------------------------------- */
+int pmx_ctx_create(int device, pmx_ctx **out);
+int pmx_ctx_destroy(pmx_ctx *ctx);
+int pmx_ctx_sync(pmx_ctx *ctx);
+const char *pmx_last_error(void);
+
+/* ---- host-only set-up (no device) ----------------------------------------
+ * The same numbering / structure as pmx_space_numbering / pmx_space_structure
+ * for the unit cube refined nref times, computed on the host only. */
+int pmx_host_space_dims(int nref, int order, long long *dims);
+int pmx_host_space_arrays(int nref, int order, int *begin_index, int *global_numbers, int *rowptr, int *kcol);
+
+/* ---- multilevel system --------------------------------------------------
+ * Replaces the per-level set-up of 3DPrograms/CD3D_ParMooN.C:264-351
+ * (RegRefineAll + new TFESpace3D(coll, ..., BoundCondition, ORDER)) and the
+ * constructor TSystemCD3D(N_levels, fespaces, ...) (src/System/SystemCD3D.C:62-225):
+ * levels are the unit cube (data/Wuerfel.GEO) refined nref0 .. nref0+nlev-1
+ * times, all with Q1 (order 1) or Q2 (order 2) elements, all-Dirichlet. */
+int pmx_system_create(pmx_ctx *ctx, int nref0, int nlev, int order, pmx_system **out);
+int pmx_system_destroy(pmx_system *sys);
+/* dims[0]=N_DegreesOfFreedom [1]=ActiveBound [2]=N_Cells [3]=N_Entries of the
+ * CSR structure [4]=N_BaseFunct [5]=lattice points per direction
+ * [6]=stored stencil values [7]=smoother colours */
+int pmx_system_dims(pmx_system *sys, int level, long long *dims);
+
+/* TFESpace3D::GetBeginIndex / GetGlobalNumbers (include/FE/FESpace3D.h),
+ * identical to ConstructSpace (src/FE/FESpace3D.C:485-2080). */
+int pmx_space_numbering(pmx_system *sys, int level, int *begin_index, int *global_numbers);
+/* TSquareStructure3D(space) + Sort(): RowPtr / KCol
+ * (src/FE/SquareStructure3D.C:59-200, src/FE/SquareStructure.C:91-110). */
+int pmx_space_structure(pmx_system *sys, int level, int *rowptr, int *kcol);
+/* multicolour smoother class of every DOF (no reference counterpart for the
+ * sequential build; the hybrid build colours DOFs in ParFEMapper3D.C:1995) */
+int pmx_space_colours(pmx_system *sys, int level, int *colour);
+
+/* ---- assembly -----------------------------------------------------------
+ * TSystemCD3D::Assemble (SystemCD3D.C:340-365): for every level
+ * TAssembleMat3D::Reset + Assemble3D (src/System/AssembleMat3D.C:235,261-535)
+ * with the local form prm[PMX_PRM_FORM] (the AssembleFct3D callbacks of
+ * src/FE/ConvDiff3D.C and src/FE/TCD3D.C) and the coefficient functor
+ * prm[PMX_PRM_EXAMPLE] (CoeffFct3D, include/General/Constants.h:148).
+ * The result stays on the device. */
+int pmx_system_assemble(pmx_system *sys, const double *prm);
+/* TSquareMatrix3D::GetEntries() of a level in CSR order; which = 0 the main
+ * matrix, 1 the second matrix of two-matrix forms (SUPG K of TCD3D.C:128-266). */
+int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries);
+/* right-hand side of a level (Rhs_array[level]) */
+int pmx_system_rhs(pmx_system *sys, int level, double *rhs);
+
+/* ---- level operations on host vectors (reference numbering) --------------
+ * MatVect_Scalar / Defect_Scalar (src/FE/Blas2.C:37-126); precision 0 = FP64
+ * entries, 1 = FP32 copy of the entries. */
+int pmx_matvec(pmx_system *sys, int level, int precision, const double *x, double *y);
+int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, double *r, double *res);
+/* TMGLevel3D::Jacobi (kind 1) / SOR (2) / SSOR (3) (src/FE/MGLevel3D.C:280-563),
+ * SOR/SSOR in multicolour order; `sweeps` sweeps, sol updated in place. */
+int pmx_smooth(pmx_system *sys, int level, int kind, double omega, int sweeps, double *sol, const double *f);
+/* Prolongate / DefectRestriction between fine_level-1 and fine_level
+ * (src/FE/MGComponents3D.C:208-488, 732-1014). */
+int pmx_prolongate(pmx_system *sys, int fine_level, const double *xc, double *xf);
+int pmx_restrict(pmx_system *sys, int fine_level, double *rc, const double *rf);
+/* MatVect on a caller CSR matrix (any TSquareMatrix3D: RowPtr, KCol, Entries). */
+int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, const double *entries,
+                   const double *x, double *y);
+
+/* ---- solvers -------------------------------------------------------------
+ * TSystemCD3D::Solve -> TItMethod::Iterate (include/FE/ItMethod.h) with the
+ * TMultiGridScaIte preconditioner (src/FE/MultiGridScaIte.C:89-130).
+ * opt (16 doubles, same layout as the oracle):
+ *  [0] solver: 16 FGMRES (FgmresIte.C:245), 11 Richardson (FixedPointIte.C:176),
+ *      13 BiCGStab (Bcgs.C:111), 0 = one MG cycle
+ *  [1] restart  [2] maxit  [3] res_norm_min (absolute)  [4] red_factor
+ *  [5] smoother 1 Jacobi 2 SOR 3 SSOR  [6] pre  [7] post  [8] omega
+ *  [9] coarse: 17 exact (FP64 dense inverse), 1/2/3 = that many-kind sweeps
+ *  [10] coarse sweeps  [11] unused  [12] cycle 1 V 2 W 0 F  [13] GMG damp
+ *  [14] BiCGStab flexible branch (1 = v = A y)
+ *  [15] precision: 0 FP64 throughout, 1 FP32 operator/MG/Krylov vectors
+ *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother
+ * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
+ * residual [2] initial residual [3] outer corrections [4] seconds (device
+ * events) [5] learned-smoother fallbacks [6..7] reserved. */
+int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, double *sol, double *info);
+/* Device-resident variant: solves with the assembled right-hand side already
+ * in HBM, zero initial guess with the Dirichlet values of the rhs
+ * (SystemCD3D.C:356).  The solution stays on the device. */
+int pmx_system_solve_resident(pmx_system *sys, const double *opt, double *info);
+int pmx_system_solution(pmx_system *sys, double *sol);
+
+/* ---- measurement ---------------------------------------------------------
+ * Times `reps` back-to-back launches of one kernel on the context stream with
+ * HIP events: 0 stencil SpMV, 1 defect, 2 Jacobi sweep, 3 SSOR sweep,
+ * 4 assembly (all kernels of one level), 5 prolongation, 6 restriction,
+ * 7 one MG V-cycle.  precision 0 FP64, 1 FP32.  out: ms per launch. */
+int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""parmoon_amd -- MI355X implementation of ParMooN's CD3D/TCD3D hot path.
+
+Python view of the C-ABI in include/pmx.h (libpmx.so, built in-tree by
+``python -m parmoon_amd.build``).  The library is the product: this module
+only marshals arrays.  If libpmx.so is missing, importing the binding raises
+immediately -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import prm as prm  # noqa: F401  (parameter block helpers)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmx.so")
+
+PRM_LEN = 512
+_lib = None
+
+
+class PmxError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpmx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PmxError(f"{LIB_PATH} not built: run `python -m parmoon_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.pmx_last_error.restype = ctypes.c_char_p
+        L.pmx_ctx_create.argtypes = [ip, ctypes.POINTER(vp)]
+        L.pmx_ctx_destroy.argtypes = [vp]
+        L.pmx_ctx_sync.argtypes = [vp]
+        L.pmx_host_space_dims.argtypes = [ip, ip, vp]
+        L.pmx_host_space_arrays.argtypes = [ip, ip, vp, vp, vp, vp]
+        L.pmx_system_create.argtypes = [vp, ip, ip, ip, ctypes.POINTER(vp)]
+        L.pmx_system_destroy.argtypes = [vp]
+        L.pmx_system_dims.argtypes = [vp, ip, vp]
+        L.pmx_space_numbering.argtypes = [vp, ip, vp, vp]
+        L.pmx_space_structure.argtypes = [vp, ip, vp, vp]
+        L.pmx_space_colours.argtypes = [vp, ip, vp]
+        L.pmx_system_assemble.argtypes = [vp, vp]
+        L.pmx_matrix_entries.argtypes = [vp, ip, ip, vp]
+        L.pmx_system_rhs.argtypes = [vp, ip, vp]
+        L.pmx_matvec.argtypes = [vp, ip, ip, vp, vp]
+        L.pmx_defect.argtypes = [vp, ip, vp, vp, vp, vp]
+        L.pmx_smooth.argtypes = [vp, ip, ip, dp, ip, vp, vp]
+        L.pmx_prolongate.argtypes = [vp, ip, vp, vp]
+        L.pmx_restrict.argtypes = [vp, ip, vp, vp]
+        L.pmx_csr_matvec.argtypes = [vp, ip, vp, vp, vp, vp, vp]
+        L.pmx_system_solve.argtypes = [vp, vp, vp, vp, vp]
+        L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
+        L.pmx_system_solution.argtypes = [vp, vp]
+        L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise PmxError(f"{what} failed ({rc}): {lib().pmx_last_error().decode()}")
+
+
+def host_space(nref, order):
+    """(dims, BeginIndex, GlobalNumbers, RowPtr, KCol) computed on the host only."""
+    L = lib()
+    d = np.zeros(8, np.int64)
+    _ok(L.pmx_host_space_dims(nref, order, _p(d)), "pmx_host_space_dims")
+    N, _, nc, nnz, nb = (int(v) for v in d[:5])
+    bi = np.zeros(nc + 1, np.int32)
+    gn = np.zeros(nc * nb, np.int32)
+    rp = np.zeros(N + 1, np.int32)
+    kc = np.zeros(nnz, np.int32)
+    _ok(L.pmx_host_space_arrays(nref, order, _p(bi), _p(gn), _p(rp), _p(kc)), "pmx_host_space_arrays")
+    return d, bi, gn, rp, kc
+
+
+class Context:
+    """One HIP device and stream (pmx_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        _ok(lib().pmx_ctx_create(device, ctypes.byref(self.h)), "pmx_ctx_create")
+
+    def sync(self):
+        _ok(lib().pmx_ctx_sync(self.h), "pmx_ctx_sync")
+
+    def close(self):
+        if self.h:
+            lib().pmx_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def csr_matvec(self, rowptr, kcol, entries, x):
+        n = len(rowptr) - 1
+        y = np.zeros(n)
+        _ok(lib().pmx_csr_matvec(self.h, n, _p(np.ascontiguousarray(rowptr, np.int32)),
+                                 _p(np.ascontiguousarray(kcol, np.int32)),
+                                 _p(np.ascontiguousarray(entries, float)),
+                                 _p(np.ascontiguousarray(x, float)), _p(y)), "pmx_csr_matvec")
+        return y
+
+
+class System:
+    """Multilevel CD3D system (pmx_system): unit cube levels nref0..nref0+nlev-1."""
+
+    def __init__(self, ctx, nref0, nlev, order):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        _ok(lib().pmx_system_create(ctx.h, nref0, nlev, order, ctypes.byref(self.h)), "pmx_system_create")
+        self.nlev, self.order, self.nref0 = nlev, order, nref0
+
+    def close(self):
+        if self.h:
+            lib().pmx_system_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dims(self, level=-1):
+        d = np.zeros(8, np.int64)
+        _ok(lib().pmx_system_dims(self.h, level, _p(d)), "pmx_system_dims")
+        return d
+
+    def numbering(self, level=-1):
+        d = self.dims(level)
+        bi = np.zeros(int(d[2]) + 1, np.int32)
+        gn = np.zeros(int(d[2]) * int(d[4]), np.int32)
+        _ok(lib().pmx_space_numbering(self.h, level, _p(bi), _p(gn)), "pmx_space_numbering")
+        return bi, gn
+
+    def structure(self, level=-1):
+        d = self.dims(level)
+        rp = np.zeros(int(d[0]) + 1, np.int32)
+        kc = np.zeros(int(d[3]), np.int32)
+        _ok(lib().pmx_space_structure(self.h, level, _p(rp), _p(kc)), "pmx_space_structure")
+        return rp, kc
+
+    def colours(self, level=-1):
+        c = np.zeros(int(self.dims(level)[0]), np.int32)
+        _ok(lib().pmx_space_colours(self.h, level, _p(c)), "pmx_space_colours")
+        return c
+
+    def assemble(self, prm_block):
+        p = np.ascontiguousarray(prm_block, float)
+        assert p.size == PRM_LEN
+        _ok(lib().pmx_system_assemble(self.h, _p(p)), "pmx_system_assemble")
+
+    def entries(self, level=-1, which=0):
+        e = np.zeros(int(self.dims(level)[3]))
+        _ok(lib().pmx_matrix_entries(self.h, level, which, _p(e)), "pmx_matrix_entries")
+        return e
+
+    def rhs(self, level=-1):
+        r = np.zeros(int(self.dims(level)[0]))
+        _ok(lib().pmx_system_rhs(self.h, level, _p(r)), "pmx_system_rhs")
+        return r
+
+    def matvec(self, x, level=-1, precision=0):
+        y = np.zeros(int(self.dims(level)[0]))
+        _ok(lib().pmx_matvec(self.h, level, precision, _p(np.ascontiguousarray(x, float)), _p(y)), "pmx_matvec")
+        return y
+
+    def defect(self, x, b, level=-1):
+        r = np.zeros(int(self.dims(level)[0]))
+        res = np.zeros(1)
+        _ok(lib().pmx_defect(self.h, level, _p(np.ascontiguousarray(x, float)),
+                             _p(np.ascontiguousarray(b, float)), _p(r), _p(res)), "pmx_defect")
+        return r, float(res[0])
+
+    def smooth(self, kind, omega, sweeps, sol, f, level=-1):
+        s = np.array(sol, float)
+        _ok(lib().pmx_smooth(self.h, level, kind, omega, sweeps, _p(s), _p(np.ascontiguousarray(f, float))),
+            "pmx_smooth")
+        return s
+
+    def prolongate(self, xc, fine_level=-1):
+        fl = fine_level % self.nlev
+        xf = np.zeros(int(self.dims(fl)[0]))
+        _ok(lib().pmx_prolongate(self.h, fl, _p(np.ascontiguousarray(xc, float)), _p(xf)), "pmx_prolongate")
+        return xf
+
+    def restrict(self, rf, fine_level=-1):
+        fl = fine_level % self.nlev
+        rc = np.zeros(int(self.dims(fl - 1)[0]))
+        _ok(lib().pmx_restrict(self.h, fl, _p(rc), _p(np.ascontiguousarray(rf, float))), "pmx_restrict")
+        return rc
+
+    def solve(self, opt, rhs, sol, check=True):
+        o = np.zeros(16)
+        o[:len(opt)] = opt
+        info = np.zeros(8)
+        s = np.array(sol, float)
+        rc = lib().pmx_system_solve(self.h, _p(o), _p(np.ascontiguousarray(rhs, float)), _p(s), _p(info))
+        if check:
+            _ok(rc, "pmx_system_solve")
+        return s, info, rc
+
+    def solve_resident(self, opt, check=True):
+        o = np.zeros(16)
+        o[:len(opt)] = opt
+        info = np.zeros(8)
+        rc = lib().pmx_system_solve_resident(self.h, _p(o), _p(info))
+        if check:
+            _ok(rc, "pmx_system_solve_resident")
+        return info, rc
+
+    def solution(self):
+        s = np.zeros(int(self.dims(-1)[0]))
+        _ok(lib().pmx_system_solution(self.h, _p(s)), "pmx_system_solution")
+        return s
+
+    def time_kernel(self, kernel, level=-1, precision=0, reps=10):
+        ms = np.zeros(1)
+        _ok(lib().pmx_time_kernel(self.h, kernel, level, precision, reps, _p(ms)), "pmx_time_kernel")
+        return float(ms[0])

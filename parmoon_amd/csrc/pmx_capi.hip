@@ -1,0 +1,1005 @@
+// pmx_capi.hip -- the extern "C" boundary of libpmx and the host orchestration
+// of assembly, multigrid and Krylov solvers on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pmx.h"
+#include "pmx_internal.hpp"
+#include "pmx_ops.hpp"
+
+namespace pmx {
+
+static thread_local std::string g_err;
+// V(2,2) SSOR with the exact coarse solve, used when a single kernel is timed
+static const double PMX_DEFAULT_VCYCLE[16] = {0, 1, 1, 0, 0, 3, 2, 2, 1.0, 17, 1, 0, 1, 1.0, 0, 0};
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t m) {
+    if (m == n && p) return;
+    release();
+    if (m == 0) return;
+    check(hipMalloc(&p, sizeof(T) * m), "hipMalloc");
+    n = m;
+  }
+  void upload(const T *h, size_t m, hipStream_t st) {
+    alloc(m);
+    check(hipMemcpyAsync(p, h, sizeof(T) * m, hipMemcpyHostToDevice, st), "upload");
+  }
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  DevBuf<double> part;  // reduction partials
+  double *pinned = nullptr;
+};
+
+// per-precision work vectors of one level
+template <typename T>
+struct Vecs {
+  DevBuf<T> val;                      // CSS values in this precision
+  DevBuf<T> x, f, d, aux;             // MG level vectors
+};
+
+struct Level {
+  Layout L;
+  Numbering Nb;
+  DevLayout D;
+  DevBuf<int> boundary, segidx;
+  DevBuf<DevSegment> segs;
+  DevBuf<BlockInfo> blocks_all, blocks_colour;
+  DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
+  DevBuf<double> val2;                // second matrix (SUPG K)
+  DevBuf<double> rhs;                 // assembled rhs (lattice order)
+  Vecs<double> v64;
+  Vecs<float> v32;
+  bool have_val32 = false;
+  DevBuf<double> ainv;                // coarsest level: explicit inverse
+  int ainv_n = 0;
+};
+
+struct System {
+  Ctx *ctx = nullptr;
+  int order = 1;
+  std::vector<std::unique_ptr<Level>> lev;
+  bool assembled = false;
+  bool two_matrices = false;
+  DevBuf<double> prm, qref, tab, qpbuf;
+  int nq = 0;
+  // Krylov workspace
+  DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
+  DevBuf<double> krylov64;
+  DevBuf<float> krylov32;
+  DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
+  std::vector<int> rec;
+  double opt[16];
+};
+
+// ------------------------------------------------------------- set-up
+static void setup_level(System &Y, Level &lv, int nref) {
+  hipStream_t st = Y.ctx->st;
+  build_layout(lv.L, nref, Y.order, kBlock);
+  build_numbering(lv.Nb, lv.L);
+  const Layout &L = lv.L;
+  DevLayout &D = lv.D;
+  D.order = L.order; D.n = L.n; D.M = L.M; D.S = L.S;
+  D.N = L.N; D.nvals = L.nvals; D.nactive = L.nactive;
+  D.ncolours = L.ncolours;
+  std::vector<DevSegment> ds(L.segs.size());
+  for (size_t i = 0; i < L.segs.size(); i++) {
+    const Segment &s = L.segs[i];
+    ds[i] = DevSegment{s.x0, s.y0, s.z0, s.cx, s.cy, s.cz, s.ex, s.ey, s.ez, s.ns, s.dslot, L.S, s.count, s.voff};
+  }
+  lv.segs.upload(ds.data(), ds.size(), st);
+  std::vector<int> sidx(L.S * L.S * L.S, -1);
+  for (size_t i = 0; i < L.segs.size(); i++) {
+    const Segment &s = L.segs[i];
+    sidx[s.rx + L.S * (s.ry + L.S * s.rz)] = (int)i;
+  }
+  lv.segidx.upload(sidx.data(), sidx.size(), st);
+  lv.blocks_all.upload(L.blocks_all.data(), L.blocks_all.size(), st);
+  std::vector<BlockInfo> bc;
+  D.colour_off.assign(L.ncolours + 1, 0);
+  for (int c = 0; c < L.ncolours; c++) {
+    D.colour_off[c] = (int)bc.size();
+    bc.insert(bc.end(), L.blocks_colour[c].begin(), L.blocks_colour[c].end());
+  }
+  D.colour_off[L.ncolours] = (int)bc.size();
+  lv.blocks_colour.upload(bc.data(), bc.size(), st);
+  lv.boundary.upload(L.boundary.data(), L.boundary.size(), st);
+  lv.lat2dof.upload(lv.Nb.lat2dof.data(), lv.Nb.lat2dof.size(), st);
+  lv.dof2lat.upload(lv.Nb.dof2lat.data(), lv.Nb.dof2lat.size(), st);
+  D.boundary = lv.boundary.p; D.nboundary = (int)L.boundary.size();
+  D.segs = lv.segs.p; D.segidx = lv.segidx.p;
+  D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
+  D.blocks_colour = lv.blocks_colour.p;
+  lv.rhs.alloc(L.N);
+  lv.v64.val.alloc(L.nvals);
+  for (auto *b : {&lv.v64.x, &lv.v64.f, &lv.v64.d, &lv.v64.aux}) b->alloc(L.N);
+}
+
+// Gauss rules of src/QuadFormulas/QuadFormulaHexa.C:70-146 and Q1/Q2 Lagrange
+// tables in the canonical frame: qref[q] = {xi, eta, zeta, w},
+// tab[q][b] = {phi, dphi/dxi, dphi/deta, dphi/dzeta}
+static void make_tables(int order, bool linear, std::vector<double> &qref, std::vector<double> &tab, int &nq) {
+  const int deg = (linear ? 2 : 3) * order;
+  const int np = deg <= 2 ? 2 : (deg <= 5 ? 3 : 4);
+  double x1[4], w1[4];
+  if (np == 2) {
+    const double a = 0.5773502691896257645091489;
+    x1[0] = -a; x1[1] = a; w1[0] = w1[1] = 1.0;
+  } else if (np == 3) {
+    x1[0] = -0.7745966692414833770358530; x1[1] = 0; x1[2] = 0.7745966692414833770358530;
+    w1[0] = 0.5555555555555555555555558; w1[1] = 0.8888888888888888888888888; w1[2] = 0.5555555555555555555555558;
+  } else {
+    x1[0] = 0.33998104358485626480266576; x1[1] = 0.86113631159405257522394649;
+    x1[2] = -0.86113631159405257522394649; x1[3] = -0.33998104358485626480266576;
+    w1[0] = 0.652145154862546142626936051; w1[1] = 0.347854845137453857373063949;
+    w1[2] = 0.347854845137453857373063949; w1[3] = 0.652145154862546142626936051;
+  }
+  nq = np * np * np;
+  const int K = order + 1, NB = K * K * K;
+  qref.assign(4 * nq, 0.0);
+  tab.assign((size_t)nq * NB * 4, 0.0);
+  auto lag = [&](double t, double *v, double *d) {
+    if (order == 1) { v[0] = 0.5 * (1 - t); v[1] = 0.5 * (1 + t); d[0] = -0.5; d[1] = 0.5; }
+    else { v[0] = 0.5 * t * (t - 1); v[1] = 1 - t * t; v[2] = 0.5 * t * (t + 1); d[0] = t - 0.5; d[1] = -2 * t; d[2] = t + 0.5; }
+  };
+  for (int i = 0; i < np; i++)
+    for (int j = 0; j < np; j++)
+      for (int k = 0; k < np; k++) {
+        const int q = k + np * (j + np * i);
+        const double r[3] = {x1[k], x1[j], x1[i]};
+        qref[4 * q] = r[0]; qref[4 * q + 1] = r[1]; qref[4 * q + 2] = r[2];
+        qref[4 * q + 3] = w1[i] * w1[j] * w1[k];
+        double v[3][3], d[3][3];
+        for (int a = 0; a < 3; a++) lag(r[a], v[a], d[a]);
+        for (int c = 0; c < K; c++)
+          for (int bb = 0; bb < K; bb++)
+            for (int a = 0; a < K; a++) {
+              const int b = a + K * (bb + K * c);
+              double *t = &tab[((size_t)q * NB + b) * 4];
+              t[0] = v[0][a] * v[1][bb] * v[2][c];
+              t[1] = d[0][a] * v[1][bb] * v[2][c];
+              t[2] = v[0][a] * d[1][bb] * v[2][c];
+              t[3] = v[0][a] * v[1][bb] * d[2][c];
+            }
+      }
+}
+
+// dense LU inverse of the coarsest operator (host, FP64)
+static void coarse_inverse(System &Y, Level &lv) {
+  const Layout &L = lv.L;
+  const int n = (int)L.N;
+  if (n > 6000) { lv.ainv_n = 0; return; }
+  hipStream_t st = Y.ctx->st;
+  std::vector<double> val(L.nvals);
+  check(hipMemcpyAsync(val.data(), lv.v64.val.p, sizeof(double) * L.nvals, hipMemcpyDeviceToHost, st), "d2h");
+  check(hipStreamSynchronize(st), "sync");
+  std::vector<double> A((size_t)n * n, 0.0);  // row-major, lattice order
+  const int M = L.M;
+  for (int p : L.boundary) A[(size_t)p * n + p] = 1.0;
+  for (const Segment &s : L.segs) {
+    for (int64_t idx = 0; idx < s.count; idx++) {
+      const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
+      const int x = s.x0 + L.S * i, y = s.y0 + L.S * j, z = s.z0 + L.S * k;
+      const int p = x + M * (y + M * z);
+      int slot = 0;
+      for (int dz = -s.ez; dz <= s.ez; dz++)
+        for (int dy = -s.ey; dy <= s.ey; dy++)
+          for (int dx = -s.ex; dx <= s.ex; dx++, slot++) {
+            const int q = (x + dx) + M * ((y + dy) + M * (z + dz));
+            A[(size_t)p * n + q] = val[s.voff + slot * s.count + idx];
+          }
+    }
+  }
+  // Gauss-Jordan with partial pivoting -> inverse
+  std::vector<double> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)piv * n + c])) piv = r;
+    if (A[(size_t)piv * n + c] == 0) throw std::runtime_error("singular coarse matrix");
+    if (piv != c)
+      for (int j = 0; j < n; j++) { std::swap(A[(size_t)c * n + j], A[(size_t)piv * n + j]); std::swap(inv[(size_t)c * n + j], inv[(size_t)piv * n + j]); }
+    const double d = 1.0 / A[(size_t)c * n + c];
+    for (int j = 0; j < n; j++) { A[(size_t)c * n + j] *= d; inv[(size_t)c * n + j] *= d; }
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = A[(size_t)r * n + c];
+      if (f == 0) continue;
+      for (int j = 0; j < n; j++) { A[(size_t)r * n + j] -= f * A[(size_t)c * n + j]; inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j]; }
+    }
+  }
+  lv.ainv.upload(inv.data(), inv.size(), st);
+  check(hipStreamSynchronize(st), "sync");
+  lv.ainv_n = n;
+}
+
+template <typename T> Vecs<T> &vecs(Level &lv);
+template <typename T> DevBuf<T> &kry_buf(System &Y);
+template <> DevBuf<double> &kry_buf<double>(System &Y) { return Y.krylov64; }
+template <> DevBuf<float> &kry_buf<float>(System &Y) { return Y.krylov32; }
+template <> Vecs<double> &vecs<double>(Level &lv) { return lv.v64; }
+template <> Vecs<float> &vecs<float>(Level &lv) { return lv.v32; }
+
+static void ensure_val32(System &Y) {
+  for (auto &lp : Y.lev) {
+    Level &lv = *lp;
+    if (lv.have_val32) continue;
+    lv.v32.val.alloc(lv.L.nvals);
+    for (auto *b : {&lv.v32.x, &lv.v32.f, &lv.v32.d, &lv.v32.aux}) b->alloc(lv.L.N);
+    convert<float, double>(lv.v32.val.p, lv.v64.val.p, lv.L.nvals, Y.ctx->st);
+    lv.have_val32 = true;
+  }
+}
+
+// ------------------------------------------------------------- multigrid
+template <typename T>
+static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
+  hipStream_t st = Y.ctx->st;
+  Vecs<T> &v = vecs<T>(lv);
+  for (int s = 0; s < sweeps; s++) {
+    if (kind == 1) {
+      check(hipMemcpyAsync(v.aux.p, v.x.p, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+      boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
+      css_apply<T>(lv.D, 2, v.val.p, v.aux.p, v.f.p, v.x.p, omega, -1, st);
+    } else {
+      boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
+      for (int c = 0; c < lv.L.ncolours; c++) css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
+      if (kind == 3)
+        for (int c = lv.L.ncolours - 1; c >= 0; c--)
+          css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
+    }
+  }
+}
+template <typename T>
+static void mg_cycle(System &Y, int i) {
+  hipStream_t st = Y.ctx->st;
+  const double *o = Y.opt;
+  Level &lv = *Y.lev[i];
+  Vecs<T> &v = vecs<T>(lv);
+  if (i == 0) {
+    const int cs = (int)o[9];
+    if (cs == 17 && lv.ainv_n > 0) {
+      gemv<T>(lv.ainv_n, lv.ainv.p, v.f.p, v.x.p, st);
+    } else {
+      level_smooth<T>(Y, lv, cs == 17 ? 3 : cs, o[8], std::max(1, (int)o[10]));
+    }
+    return;
+  }
+  Level &cl = *Y.lev[i - 1];
+  Vecs<T> &cv = vecs<T>(cl);
+  const int kind = (int)o[5];
+  level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
+  css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+  boundary<T>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
+  restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st);
+  boundary<T>(cl.D, 3, cv.f.p, nullptr, nullptr, st);  // CorrectDefect
+  fill<T>(cv.x.p, (T)0, cl.L.N, st);                     // Reset
+  for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1);
+  if ((int)o[12] < 1) Y.rec[i] = 1;
+  prolongate<T>(cl.D, lv.D, cv.x.p, v.aux.p, st);
+  boundary<T>(lv.D, 3, v.aux.p, nullptr, nullptr, st);  // CorrectNodes
+  axpy<T>(v.x.p, v.aux.p, nullptr, 1.0, o[13], lv.L.N, st);
+  level_smooth<T>(Y, lv, kind, o[8], (int)o[7]);
+}
+// z = MG(v): TMultiGridScaIte::Iterate with zero start (MultiGridScaIte.C:89-130)
+template <typename T>
+static void mg_prec(System &Y, T *z, const T *vin) {
+  hipStream_t st = Y.ctx->st;
+  const int L = (int)Y.lev.size() - 1;
+  Level &lv = *Y.lev[L];
+  Vecs<T> &v = vecs<T>(lv);
+  check(hipMemcpyAsync(v.f.p, vin, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+  fill<T>(v.x.p, (T)0, lv.L.N, st);
+  boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);  // SetDirichletNodes
+  const int gam = (int)Y.opt[12];
+  Y.rec.assign(Y.lev.size() + 1, 1);
+  for (int k = 2; k <= L; k++) Y.rec[k] = gam > 0 ? gam : 2;
+  mg_cycle<T>(Y, L);
+  check(hipMemcpyAsync(z, v.x.p, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+}
+
+// y = A x on the finest level (stencil + Dirichlet identity rows)
+template <typename T>
+static void fine_matvec(System &Y, const T *x, T *y) {
+  Level &lv = *Y.lev.back();
+  css_apply<T>(lv.D, 0, vecs<T>(lv).val.p, x, nullptr, y, 0.0, -1, Y.ctx->st);
+  boundary<T>(lv.D, 0, y, x, nullptr, Y.ctx->st);
+}
+template <typename T>
+static void fine_defect(System &Y, const T *x, const T *b, T *r) {
+  Level &lv = *Y.lev.back();
+  css_apply<T>(lv.D, 1, vecs<T>(lv).val.p, x, b, r, 0.0, -1, Y.ctx->st);
+  boundary<T>(lv.D, 1, r, x, b, Y.ctx->st);
+}
+
+static double read_scalar(System &Y, const double *dev) {
+  check(hipMemcpyAsync(Y.ctx->pinned, dev, sizeof(double), hipMemcpyDeviceToHost, Y.ctx->st), "d2h scalar");
+  check(hipStreamSynchronize(Y.ctx->st), "sync");
+  return Y.ctx->pinned[0];
+}
+
+// TFgmresIte::Iterate, flexible branch with modified Gram-Schmidt
+// (src/FE/FgmresIte.C:262-625).  Vectors in T, Hessenberg/Givens in FP64.
+// Returns iterations; sets *res_out to the last residual estimate.
+template <typename T>
+static int fgmres(System &Y, T *x, const T *b, int m, int maxit, double rmin, double red, double *res_out,
+                  double *res0_out, int *status) {
+  hipStream_t st = Y.ctx->st;
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  DevBuf<T> &kry = kry_buf<T>(Y);
+  kry.alloc((size_t)(2 * (m + 1) + 1) * N);
+  T *V = kry.p, *Z = kry.p + (size_t)(m + 1) * N, *w = kry.p + (size_t)2 * (m + 1) * N;
+  const int ld = m + 1;
+  Y.scal.alloc((size_t)ld * ld + 5 * ld + 8);
+  double *H = Y.scal.p, *cs = H + ld * ld, *sn = cs + ld, *s = sn + ld, *yv = s + ld, *beta = yv + ld,
+         *res = beta + 1, *tmp = res + 1;
+  double *part = Y.ctx->part.p;
+  fine_defect<T>(Y, x, b, V);
+  dot<T>(V, V, N, part, beta, 1, st);
+  double bet = read_scalar(Y, beta);
+  double res0 = bet, r = bet;
+  *res0_out = bet;
+  *status = PMX_OK;
+  if (bet <= rmin) { *res_out = bet; return 0; }
+  int j = 1;
+  while (j <= maxit) {
+    scale_inv<T>(V, V, beta, N, st);
+    fill<double>(s, 0.0, ld, st);
+    set_scalar(s, beta, 1.0, st);
+    fill<double>(H, 0.0, (int64_t)ld * ld, st);
+    int i;
+    for (i = 0; i < m && j <= maxit; i++, j++) {
+      T *vi = V + (size_t)i * N, *zi = Z + (size_t)i * N;
+      mg_prec<T>(Y, zi, vi);
+      fine_matvec<T>(Y, zi, w);
+      for (int k = 0; k <= i; k++) {
+        T *vk = V + (size_t)k * N;
+        dot<T>(w, vk, N, part, H + (size_t)i * ld + k, 0, st);
+        axpy<T>(w, vk, H + (size_t)i * ld + k, -1.0, 0.0, N, st);
+      }
+      dot<T>(w, w, N, part, H + (size_t)i * ld + i + 1, 1, st);
+      scale_inv<T>(V + (size_t)(i + 1) * N, w, H + (size_t)i * ld + i + 1, N, st);
+      givens(H, cs, sn, s, i, ld, res, st);
+      r = read_scalar(Y, res);
+      if (!(r == r)) { *status = PMX_ERR_DIVERGED; *res_out = r; return j; }
+      if (r < rmin || r < red * res0) {
+        backsolve(H, s, yv, i, ld, st);
+        multi_axpy<T, T>(x, Z, yv, i + 1, N, N, st);
+        *res_out = r;
+        return j;
+      }
+      if (r > 1e10 * res0) { *status = PMX_ERR_DIVERGED; *res_out = r; return j; }
+    }
+    backsolve(H, s, yv, m - 1, ld, st);
+    multi_axpy<T, T>(x, Z, yv, m, N, N, st);
+    fine_defect<T>(Y, x, b, V);
+    dot<T>(V, V, N, part, beta, 1, st);
+    bet = read_scalar(Y, beta);
+    res0 = r = bet;
+    if (bet <= rmin) break;
+  }
+  *res_out = r;
+  if (j > maxit) *status = PMX_ERR_NOCONV;
+  (void)tmp;
+  return std::min(j, maxit);
+}
+
+// TFixedPointIte::Iterate (src/FE/FixedPointIte.C:176-300)
+template <typename T>
+static int richardson(System &Y, T *x, const T *b, int maxit, double rmin, double red, double *res_out,
+                      double *res0_out, int *status) {
+  hipStream_t st = Y.ctx->st;
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  DevBuf<T> &kry = kry_buf<T>(Y);
+  kry.alloc(2 * N);
+  T *d = kry.p, *z = kry.p + N;
+  Y.scal.alloc(64);
+  double *res = Y.scal.p;
+  fine_defect<T>(Y, x, b, d);
+  dot<T>(d, d, N, Y.ctx->part.p, res, 1, st);
+  double r = read_scalar(Y, res);
+  const double r0 = r;
+  *res0_out = r0;
+  *status = PMX_OK;
+  int it = 0;
+  if (r > rmin)
+    for (it = 1; it <= maxit; it++) {
+      mg_prec<T>(Y, z, d);
+      axpy<T>(x, z, nullptr, 1.0, 1.0, N, st);
+      fine_defect<T>(Y, x, b, d);
+      dot<T>(d, d, N, Y.ctx->part.p, res, 1, st);
+      r = read_scalar(Y, res);
+      if (r <= r0 * red || r <= rmin) break;
+      if (!(r == r) || r > 1e10 * r0) { *status = PMX_ERR_DIVERGED; break; }
+    }
+  if (it > maxit) { it = maxit; *status = PMX_ERR_NOCONV; }
+  *res_out = r;
+  return it;
+}
+
+// TBcgs::Iterate (src/FE/Bcgs.C:111-304); flexible selects v = A y
+template <typename T>
+static int bicgstab(System &Y, T *x, const T *b, int maxit, double rmin, double red, bool flexible,
+                    double *res_out, double *res0_out, int *status) {
+  hipStream_t st = Y.ctx->st;
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  DevBuf<T> &kry = kry_buf<T>(Y);
+  kry.alloc(6 * N);
+  T *r = kry.p, *rt = r + N, *p = rt + N, *yv = p + N, *v = yv + N, *t = v + N;
+  Y.scal.alloc(64);
+  double *sc = Y.scal.p;  // 0 rho, 1 delta, 2 nrm, 3 tt, 4 tr
+  double *part = Y.ctx->part.p;
+  fine_defect<T>(Y, x, b, r);
+  check(hipMemcpyAsync(rt, r, sizeof(T) * N, hipMemcpyDeviceToDevice, st), "copy");
+  dot<T>(r, rt, N, part, sc + 2, 1, st);
+  const double dn0 = read_scalar(Y, sc + 2);
+  *res0_out = dn0;
+  *status = PMX_OK;
+  double dn = dn0, rho_last = 1, alpha = 0, omega = 0;
+  if (dn0 <= rmin) { *res_out = dn0; return 0; }
+  int i;
+  for (i = 0; i < maxit; i++) {
+    dot<T>(r, rt, N, part, sc, 0, st);
+    const double rho = read_scalar(Y, sc);
+    if (std::fabs(rho) < 1e-50) { *status = PMX_ERR_NOCONV; break; }
+    if (i == 0) check(hipMemcpyAsync(p, r, sizeof(T) * N, hipMemcpyDeviceToDevice, st), "copy");
+    else {
+      const double beta = (rho / rho_last) * (alpha / omega);
+      // p = beta p + r - beta omega v
+      fill<T>(t, (T)0, N, st);
+      axpy<T>(t, p, nullptr, 1.0, beta, N, st);
+      axpy<T>(t, r, nullptr, 1.0, 1.0, N, st);
+      axpy<T>(t, v, nullptr, 1.0, -beta * omega, N, st);
+      check(hipMemcpyAsync(p, t, sizeof(T) * N, hipMemcpyDeviceToDevice, st), "copy");
+    }
+    rho_last = rho;
+    mg_prec<T>(Y, yv, p);
+    if (flexible) fine_matvec<T>(Y, yv, v);
+    else check(hipMemcpyAsync(v, yv, sizeof(T) * N, hipMemcpyDeviceToDevice, st), "copy");
+    dot<T>(rt, v, N, part, sc + 1, 0, st);
+    double delta = read_scalar(Y, sc + 1);
+    if (std::fabs(delta) < 1e-50) delta = 1e-50;
+    alpha = rho / delta;
+    axpy<T>(r, v, nullptr, 1.0, -alpha, N, st);
+    axpy<T>(x, flexible ? yv : p, nullptr, 1.0, alpha, N, st);
+    dot<T>(r, r, N, part, sc + 2, 1, st);
+    dn = read_scalar(Y, sc + 2);
+    if (dn < dn0 * red || dn < rmin) break;
+    mg_prec<T>(Y, yv, r);
+    if (flexible) fine_matvec<T>(Y, yv, t);
+    else check(hipMemcpyAsync(t, yv, sizeof(T) * N, hipMemcpyDeviceToDevice, st), "copy");
+    dot<T>(t, t, N, part, sc + 3, 0, st);
+    double tt = read_scalar(Y, sc + 3);
+    if (std::fabs(tt) < 1e-50) tt = 1e-50;
+    dot<T>(t, r, N, part, sc + 4, 0, st);
+    omega = read_scalar(Y, sc + 4) / tt;
+    axpy<T>(x, flexible ? yv : r, nullptr, 1.0, omega, N, st);
+    axpy<T>(r, t, nullptr, 1.0, -omega, N, st);
+    dot<T>(r, r, N, part, sc + 2, 1, st);
+    dn = read_scalar(Y, sc + 2);
+    if (dn < dn0 * red || dn < rmin) break;
+    if (!(dn == dn) || dn > 1e10 * dn0) { *status = PMX_ERR_DIVERGED; break; }
+  }
+  if (i >= maxit) *status = PMX_ERR_NOCONV;
+  *res_out = dn;
+  return std::min(i + 1, maxit);
+}
+
+template <typename T>
+static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double red, double *res, double *res0,
+                  int *status) {
+  const int solver = (int)Y.opt[0];
+  if (solver == 16) return fgmres<T>(Y, x, b, std::max(1, (int)Y.opt[1]), maxit, rmin, red, res, res0, status);
+  if (solver == 11) return richardson<T>(Y, x, b, maxit, rmin, red, res, res0, status);
+  if (solver == 13) return bicgstab<T>(Y, x, b, maxit, rmin, red, Y.opt[14] != 0, res, res0, status);
+  mg_prec<T>(Y, x, b);
+  *res = *res0 = 0;
+  *status = PMX_OK;
+  return 1;
+}
+
+// Solve A sol = b on the finest level; sol/b are FP64 lattice vectors on the device.
+static int solve_device(System &Y, double *info) {
+  hipStream_t st = Y.ctx->st;
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  const int prec = (int)Y.opt[15];
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "event");
+  check(hipEventCreate(&e1), "event");
+  check(hipEventRecord(e0, st), "event");
+  int status = PMX_OK, its = 0, outer = 0;
+  double res = 0, res0 = 0;
+  const int maxit = (int)Y.opt[2];
+  if (prec == 0) {
+    its = krylov<double>(Y, Y.sol.p, Y.b.p, maxit, Y.opt[3], Y.opt[4], &res, &res0, &status);
+    outer = 1;
+  } else {
+    // FP64 defect correction around an FP32 Krylov/MG solve
+    ensure_val32(Y);
+    Y.r64.alloc(N);
+    DevBuf<float> r32, e32;
+    r32.alloc(N);
+    e32.alloc(N);
+    const double rmin = Y.opt[3], red = Y.opt[4];
+    const double inner_red = 1e-4;
+    fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
+    double *nr = Y.ctx->part.p + kRedBlocks;
+    dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
+    res0 = res = read_scalar(Y, nr);
+    const double target = std::max(rmin, red * res0);
+    while (res > target && its < maxit) {
+      convert<float, double>(r32.p, Y.r64.p, N, st);
+      fill<float>(e32.p, 0.0f, N, st);
+      double ri, ri0;
+      int st_in;
+      const double want = std::max(inner_red, 0.5 * target / res);
+      its += krylov<float>(Y, e32.p, r32.p, maxit - its, 0.0, want, &ri, &ri0, &st_in);
+      add_convert<double, float>(Y.sol.p, e32.p, N, st);
+      fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
+      dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
+      const double rn = read_scalar(Y, nr);
+      outer++;
+      if (!(rn == rn) || rn > 1e10 * res0) { status = PMX_ERR_DIVERGED; res = rn; break; }
+      res = rn;
+      if (st_in == PMX_ERR_DIVERGED) { status = PMX_ERR_DIVERGED; break; }
+    }
+    if (res > target && status == PMX_OK) status = PMX_ERR_NOCONV;
+  }
+  check(hipEventRecord(e1, st), "event");
+  check(hipEventSynchronize(e1), "event");
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, e0, e1), "event");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (info) {
+    info[0] = its; info[1] = res; info[2] = res0; info[3] = outer; info[4] = ms * 1e-3; info[5] = 0;
+  }
+  return status;
+}
+
+static Level &level_of(System &Y, int level) {
+  if (level < 0) level += (int)Y.lev.size();
+  if (level < 0 || level >= (int)Y.lev.size()) throw std::invalid_argument("level out of range");
+  return *Y.lev[level];
+}
+static void ensure_csr_dev(System &Y, Level &lv) {
+  if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
+  if (!lv.rowptr.p) {
+    lv.rowptr.upload(lv.Nb.rowptr.data(), lv.Nb.rowptr.size(), Y.ctx->st);
+    lv.kcol.upload(lv.Nb.kcol.data(), lv.Nb.kcol.size(), Y.ctx->st);
+  }
+}
+// host (reference numbering) <-> device lattice vector
+template <typename T>
+static void to_lattice(System &Y, Level &lv, const double *host, T *dev) {
+  DevBuf<double> tmp;
+  tmp.upload(host, lv.L.N, Y.ctx->st);
+  DevBuf<double> lat;
+  lat.alloc(lv.L.N);
+  gather<double, double>(lat.p, tmp.p, lv.lat2dof.p, lv.L.N, Y.ctx->st);
+  if (std::is_same<T, double>::value)
+    check(hipMemcpyAsync(dev, lat.p, sizeof(double) * lv.L.N, hipMemcpyDeviceToDevice, Y.ctx->st), "copy");
+  else
+    convert<float, double>((float *)dev, lat.p, lv.L.N, Y.ctx->st);
+  check(hipStreamSynchronize(Y.ctx->st), "sync");
+}
+template <typename T>
+static void from_lattice(System &Y, Level &lv, const T *dev, double *host) {
+  DevBuf<double> lat, tmp;
+  lat.alloc(lv.L.N);
+  tmp.alloc(lv.L.N);
+  if (std::is_same<T, double>::value)
+    check(hipMemcpyAsync(lat.p, dev, sizeof(double) * lv.L.N, hipMemcpyDeviceToDevice, Y.ctx->st), "copy");
+  else
+    convert<double, float>(lat.p, (const float *)dev, lv.L.N, Y.ctx->st);
+  gather<double, double>(tmp.p, lat.p, lv.dof2lat.p, lv.L.N, Y.ctx->st);
+  check(hipMemcpyAsync(host, tmp.p, sizeof(double) * lv.L.N, hipMemcpyDeviceToHost, Y.ctx->st), "d2h");
+  check(hipStreamSynchronize(Y.ctx->st), "sync");
+}
+
+}  // namespace pmx
+
+using namespace pmx;
+
+struct pmx_ctx { Ctx c; };
+struct pmx_system { System s; };
+
+#define PMX_TRY(...)                                         \
+  try {                                                      \
+    __VA_ARGS__;                                                    \
+    return PMX_OK;                                           \
+  } catch (const std::invalid_argument &e) {                 \
+    g_err = e.what();                                        \
+    return PMX_ERR_ARG;                                      \
+  } catch (const std::logic_error &e) {                      \
+    g_err = e.what();                                        \
+    return PMX_ERR_STATE;                                    \
+  } catch (const std::exception &e) {                        \
+    g_err = e.what();                                        \
+    return PMX_ERR_HIP;                                      \
+  }
+
+extern "C" {
+
+const char *pmx_last_error(void) { return g_err.c_str(); }
+
+int pmx_ctx_create(int device, pmx_ctx **out) {
+  PMX_TRY({
+    if (!out) throw std::invalid_argument("out is NULL");
+    int n = 0;
+    check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) throw std::invalid_argument("no such device");
+    check(hipSetDevice(device), "hipSetDevice");
+    auto *c = new pmx_ctx();
+    c->c.device = device;
+    check(hipStreamCreateWithFlags(&c->c.st, hipStreamNonBlocking), "stream");
+    c->c.part.alloc(kRedBlocks + 64);
+    check(hipHostMalloc((void **)&c->c.pinned, 64 * sizeof(double), hipHostMallocDefault), "pinned");
+    *out = c;
+  })
+}
+int pmx_ctx_destroy(pmx_ctx *ctx) {
+  PMX_TRY({
+    if (!ctx) return PMX_OK;
+    (void)hipStreamSynchronize(ctx->c.st);
+    ctx->c.part.release();
+    if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+    (void)hipStreamDestroy(ctx->c.st);
+    delete ctx;
+  })
+}
+int pmx_ctx_sync(pmx_ctx *ctx) { PMX_TRY({ check(hipStreamSynchronize(ctx->c.st), "sync"); }) }
+
+// host-only numbering / structure (no device needed)
+int pmx_host_space_dims(int nref, int order, long long *d) {
+  PMX_TRY({
+    Layout L;
+    Numbering Nb;
+    build_layout(L, nref, order, kBlock);
+    build_numbering(Nb, L);
+    build_csr(Nb, L);
+    d[0] = L.N; d[1] = L.nactive; d[2] = Nb.ncells; d[3] = Nb.rowptr[L.N]; d[4] = Nb.nb; d[5] = L.M;
+    d[6] = L.nvals; d[7] = L.ncolours;
+  })
+}
+int pmx_host_space_arrays(int nref, int order, int *bi, int *gn, int *rp, int *kc) {
+  PMX_TRY({
+    Layout L;
+    Numbering Nb;
+    build_layout(L, nref, order, kBlock);
+    build_numbering(Nb, L);
+    if (bi) std::memcpy(bi, Nb.begin_index.data(), sizeof(int) * Nb.begin_index.size());
+    if (gn) std::memcpy(gn, Nb.global_numbers.data(), sizeof(int) * Nb.global_numbers.size());
+    if (rp || kc) {
+      build_csr(Nb, L);
+      if (rp) std::memcpy(rp, Nb.rowptr.data(), sizeof(int) * Nb.rowptr.size());
+      if (kc) std::memcpy(kc, Nb.kcol.data(), sizeof(int) * Nb.kcol.size());
+    }
+  })
+}
+
+int pmx_system_create(pmx_ctx *ctx, int nref0, int nlev, int order, pmx_system **out) {
+  PMX_TRY({
+    if (!ctx || !out) throw std::invalid_argument("NULL argument");
+    if (order != 1 && order != 2) throw std::invalid_argument("order must be 1 or 2");
+    if (nref0 < 1 || nlev < 1 || nref0 + nlev - 1 > (order == 2 ? 7 : 8))
+      throw std::invalid_argument("refinement levels out of range");
+    auto *h = new pmx_system();
+    System &Y = h->s;
+    Y.ctx = &ctx->c;
+    Y.order = order;
+    try {
+      for (int l = 0; l < nlev; l++) {
+        Y.lev.emplace_back(new Level());
+        setup_level(Y, *Y.lev.back(), nref0 + l);
+      }
+      const Level &f = *Y.lev.back();
+      Y.sol.alloc(f.L.N);
+      Y.b.alloc(f.L.N);
+      check(hipStreamSynchronize(ctx->c.st), "sync");
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  })
+}
+int pmx_system_destroy(pmx_system *sys) {
+  PMX_TRY({
+    if (!sys) return PMX_OK;
+    (void)hipStreamSynchronize(sys->s.ctx->st);
+    delete sys;
+  })
+}
+int pmx_system_dims(pmx_system *sys, int level, long long *d) {
+  PMX_TRY({
+    Level &lv = level_of(sys->s, level);
+    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
+    d[0] = lv.L.N; d[1] = lv.L.nactive; d[2] = lv.Nb.ncells; d[3] = lv.Nb.rowptr[lv.L.N];
+    d[4] = lv.Nb.nb; d[5] = lv.L.M; d[6] = lv.L.nvals; d[7] = lv.L.ncolours;
+  })
+}
+int pmx_space_numbering(pmx_system *sys, int level, int *bi, int *gn) {
+  PMX_TRY({
+    Level &lv = level_of(sys->s, level);
+    if (bi) std::memcpy(bi, lv.Nb.begin_index.data(), sizeof(int) * lv.Nb.begin_index.size());
+    if (gn) std::memcpy(gn, lv.Nb.global_numbers.data(), sizeof(int) * lv.Nb.global_numbers.size());
+  })
+}
+int pmx_space_structure(pmx_system *sys, int level, int *rp, int *kc) {
+  PMX_TRY({
+    Level &lv = level_of(sys->s, level);
+    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
+    if (rp) std::memcpy(rp, lv.Nb.rowptr.data(), sizeof(int) * lv.Nb.rowptr.size());
+    if (kc) std::memcpy(kc, lv.Nb.kcol.data(), sizeof(int) * lv.Nb.kcol.size());
+  })
+}
+int pmx_space_colours(pmx_system *sys, int level, int *colour) {
+  PMX_TRY({
+    Level &lv = level_of(sys->s, level);
+    const int M = lv.L.M;
+    for (int64_t r = 0; r < lv.L.N; r++) {
+      const int l = lv.Nb.dof2lat[r];
+      colour[r] = r < lv.L.nactive ? node_colour(lv.L, l % M, (l / M) % M, l / (M * M)) : -1;
+    }
+  })
+}
+
+int pmx_system_assemble(pmx_system *sys, const double *prm) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!prm) throw std::invalid_argument("prm is NULL");
+    hipStream_t st = Y.ctx->st;
+    const int form = (int)prm[PMX_PRM_FORM];
+    if (form < PMX_FORM_GALERKIN || form > PMX_FORM_TCD_SUPG_AK) throw std::invalid_argument("unknown form");
+    Y.prm.upload(prm, PMX_PRM_LEN, st);
+    std::vector<double> qref, tab;
+    int nq;
+    make_tables(Y.order, prm[PMX_PRM_LINEAR] != 0, qref, tab, nq);
+    Y.qref.upload(qref.data(), qref.size(), st);
+    Y.tab.upload(tab.data(), tab.size(), st);
+    Y.nq = nq;
+    const int64_t nc = Y.lev.back()->Nb.ncells;
+    Y.qpbuf.alloc((size_t)nc * nq * 8);
+    Y.two_matrices = form == PMX_FORM_TCD_SUPG_AK;
+    for (auto &lp : Y.lev) {
+      Level &lv = *lp;
+      if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
+      assemble(lv.D, Y.prm.p, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr,
+               lv.rhs.p, st);
+      lv.have_val32 = false;
+    }
+    check(hipStreamSynchronize(st), "assemble");
+    coarse_inverse(Y, *Y.lev.front());
+    Y.assembled = true;
+  })
+}
+int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    Level &lv = level_of(Y, level);
+    if (which == 1 && !lv.val2.p) throw std::logic_error("no second matrix assembled");
+    ensure_csr_dev(Y, lv);
+    DevBuf<double> out;
+    out.alloc(lv.Nb.kcol.size());
+    css_to_csr<double>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, which ? lv.val2.p : lv.v64.val.p,
+                       out.p, Y.ctx->st);
+    check(hipMemcpyAsync(entries, out.p, sizeof(double) * out.n, hipMemcpyDeviceToHost, Y.ctx->st), "d2h");
+    check(hipStreamSynchronize(Y.ctx->st), "sync");
+  })
+}
+int pmx_system_rhs(pmx_system *sys, int level, double *rhs) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    Level &lv = level_of(Y, level);
+    from_lattice<double>(Y, lv, lv.rhs.p, rhs);
+  })
+}
+int pmx_matvec(pmx_system *sys, int level, int precision, const double *x, double *y) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    Level &lv = level_of(Y, level);
+    hipStream_t st = Y.ctx->st;
+    if (precision == 0) {
+      to_lattice<double>(Y, lv, x, lv.v64.aux.p);
+      css_apply<double>(lv.D, 0, lv.v64.val.p, lv.v64.aux.p, nullptr, lv.v64.d.p, 0.0, -1, st);
+      boundary<double>(lv.D, 0, lv.v64.d.p, lv.v64.aux.p, nullptr, st);
+      from_lattice<double>(Y, lv, lv.v64.d.p, y);
+    } else {
+      ensure_val32(Y);
+      to_lattice<float>(Y, lv, x, lv.v32.aux.p);
+      css_apply<float>(lv.D, 0, lv.v32.val.p, lv.v32.aux.p, nullptr, lv.v32.d.p, 0.0, -1, st);
+      boundary<float>(lv.D, 0, lv.v32.d.p, lv.v32.aux.p, nullptr, st);
+      from_lattice<float>(Y, lv, lv.v32.d.p, y);
+    }
+  })
+}
+int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, double *r, double *res) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    Level &lv = level_of(Y, level);
+    hipStream_t st = Y.ctx->st;
+    to_lattice<double>(Y, lv, x, lv.v64.aux.p);
+    to_lattice<double>(Y, lv, b, lv.v64.f.p);
+    css_apply<double>(lv.D, 1, lv.v64.val.p, lv.v64.aux.p, lv.v64.f.p, lv.v64.d.p, 0.0, -1, st);
+    boundary<double>(lv.D, 1, lv.v64.d.p, lv.v64.aux.p, lv.v64.f.p, st);
+    double *nr = Y.ctx->part.p + kRedBlocks;
+    dot<double>(lv.v64.d.p, lv.v64.d.p, lv.L.N, Y.ctx->part.p, nr, 1, st);
+    if (res) *res = read_scalar(Y, nr);
+    from_lattice<double>(Y, lv, lv.v64.d.p, r);
+  })
+}
+int pmx_smooth(pmx_system *sys, int level, int kind, double omega, int sweeps, double *sol, const double *f) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (kind < 1 || kind > 3) throw std::invalid_argument("kind must be 1, 2 or 3");
+    Level &lv = level_of(Y, level);
+    to_lattice<double>(Y, lv, sol, lv.v64.x.p);
+    to_lattice<double>(Y, lv, f, lv.v64.f.p);
+    level_smooth<double>(Y, lv, kind, omega, sweeps);
+    from_lattice<double>(Y, lv, lv.v64.x.p, sol);
+  })
+}
+int pmx_prolongate(pmx_system *sys, int fine_level, const double *xc, double *xf) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (fine_level < 0) fine_level += (int)Y.lev.size();
+    if (fine_level < 1) throw std::invalid_argument("fine_level must be >= 1");
+    Level &f = level_of(Y, fine_level), &c = level_of(Y, fine_level - 1);
+    to_lattice<double>(Y, c, xc, c.v64.aux.p);
+    prolongate<double>(c.D, f.D, c.v64.aux.p, f.v64.aux.p, Y.ctx->st);
+    from_lattice<double>(Y, f, f.v64.aux.p, xf);
+  })
+}
+int pmx_restrict(pmx_system *sys, int fine_level, double *rc, const double *rf) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (fine_level < 0) fine_level += (int)Y.lev.size();
+    if (fine_level < 1) throw std::invalid_argument("fine_level must be >= 1");
+    Level &f = level_of(Y, fine_level), &c = level_of(Y, fine_level - 1);
+    to_lattice<double>(Y, f, rf, f.v64.aux.p);
+    restrict_<double>(c.D, f.D, c.v64.aux.p, f.v64.aux.p, Y.ctx->st);
+    from_lattice<double>(Y, c, c.v64.aux.p, rc);
+  })
+}
+int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, const double *entries, const double *x,
+                   double *y) {
+  PMX_TRY({
+    if (!ctx || n < 0) throw std::invalid_argument("bad argument");
+    hipStream_t st = ctx->c.st;
+    const int nnz = rowptr[n];
+    DevBuf<int> rp, kc;
+    DevBuf<double> v, dx, dy;
+    rp.upload(rowptr, n + 1, st);
+    kc.upload(kcol, nnz, st);
+    v.upload(entries, nnz, st);
+    dx.upload(x, n, st);
+    dy.alloc(n);
+    csr_spmv<double>(n, rp.p, kc.p, v.p, dx.p, dy.p, st);
+    check(hipMemcpyAsync(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipStreamSynchronize(st), "sync");
+  })
+}
+
+int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, double *sol, double *info) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    Level &lv = *Y.lev.back();
+    to_lattice<double>(Y, lv, rhs, Y.b.p);
+    to_lattice<double>(Y, lv, sol, Y.sol.p);
+    const int rc = solve_device(Y, info);
+    from_lattice<double>(Y, lv, Y.sol.p, sol);
+    if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+int pmx_system_solve_resident(pmx_system *sys, const double *opt, double *info) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    Level &lv = *Y.lev.back();
+    hipStream_t st = Y.ctx->st;
+    check(hipMemcpyAsync(Y.b.p, lv.rhs.p, sizeof(double) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+    fill<double>(Y.sol.p, 0.0, lv.L.N, st);
+    boundary<double>(lv.D, 2, Y.sol.p, nullptr, Y.b.p, st);
+    const int rc = solve_device(Y, info);
+    if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+int pmx_system_solution(pmx_system *sys, double *sol) {
+  PMX_TRY({
+    System &Y = sys->s;
+    from_lattice<double>(Y, *Y.lev.back(), Y.sol.p, sol);
+  })
+}
+
+int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (reps < 1) throw std::invalid_argument("reps < 1");
+    Level &lv = level_of(Y, level);
+    const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    if (kernel == 7 && li != (int)Y.lev.size() - 1) throw std::invalid_argument("V-cycle timing is on the finest level");
+    std::memcpy(Y.opt, PMX_DEFAULT_VCYCLE, sizeof(Y.opt));
+    hipStream_t st = Y.ctx->st;
+    if (precision) ensure_val32(Y);
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    auto one = [&]() {
+      if (precision == 0) {
+        Vecs<double> &v = lv.v64;
+        switch (kernel) {
+          case 0: css_apply<double>(lv.D, 0, v.val.p, v.x.p, nullptr, v.d.p, 0.0, -1, st); break;
+          case 1: css_apply<double>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
+          case 2: level_smooth<double>(Y, lv, 1, 0.8, 1); break;
+          case 3: level_smooth<double>(Y, lv, 3, 1.0, 1); break;
+          case 4: assemble(lv.D, Y.prm.p, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
+          case 5: if (li > 0) prolongate<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.x.p, v.aux.p, st); break;
+          case 6: if (li > 0) restrict_<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.f.p, v.d.p, st); break;
+          case 7: mg_prec<double>(Y, v.aux.p, v.d.p); break;
+          default: break;
+        }
+      } else {
+        Vecs<float> &v = lv.v32;
+        switch (kernel) {
+          case 0: css_apply<float>(lv.D, 0, v.val.p, v.x.p, nullptr, v.d.p, 0.0, -1, st); break;
+          case 1: css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
+          case 2: level_smooth<float>(Y, lv, 1, 0.8, 1); break;
+          case 3: level_smooth<float>(Y, lv, 3, 1.0, 1); break;
+          case 5: if (li > 0) prolongate<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.x.p, v.aux.p, st); break;
+          case 6: if (li > 0) restrict_<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.f.p, v.d.p, st); break;
+          case 7: mg_prec<float>(Y, v.aux.p, v.d.p); break;
+          default: break;
+        }
+      }
+    };
+    one();
+    check(hipEventRecord(e0, st), "event");
+    for (int r = 0; r < reps; r++) one();
+    check(hipEventRecord(e1, st), "event");
+    check(hipEventSynchronize(e1), "event");
+    float t = 0;
+    check(hipEventElapsedTime(&t, e0, e1), "event");
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  })
+}
+
+}  // extern "C"

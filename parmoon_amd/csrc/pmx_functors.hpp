@@ -1,0 +1,135 @@
+// pmx_functors.hpp -- device restatements of the example coefficient callbacks
+// (CoeffFct3D / BoundValueFunct3D, include/General/Constants.h:128-148) and
+// of Compute_SDFEM_delta (src/FE/ConvDiff.C:520-611).  The callbacks of the
+// reference are host function pointers called per cell; here they are
+// selected by the PMX_PRM_EXAMPLE entry of the parameter block and evaluated
+// per quadrature point on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pmx_prm.h"
+
+namespace pmx {
+
+struct QpCoeffs {
+  double eps, b0, b1, b2, c, f;
+};
+
+__host__ __device__ inline void fourier_b(const double *prm, double x, double y, double z, double b[3]) {
+  const double twopi = 6.283185307179586476925286766559;
+  double dA[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double *F = prm + PMX_PRM_FOURIER;
+  for (int c = 0; c < 3; c++)
+    for (int m = 0; m < 8; m++) {
+      const double *t = F + (c * 8 + m) * 5;
+      const double arg = twopi * (t[1] * x + t[2] * y + t[3] * z) + t[4];
+      const double cs = t[0] * cos(arg) * twopi;
+      dA[c][0] += cs * t[1];
+      dA[c][1] += cs * t[2];
+      dA[c][2] += cs * t[3];
+    }
+  const double s = prm[PMX_PRM_BSCALE];
+  b[0] = s * (dA[2][1] - dA[1][2]);
+  b[1] = s * (dA[0][2] - dA[2][0]);
+  b[2] = s * (dA[1][0] - dA[0][1]);
+}
+
+__host__ __device__ inline double grf_field(const double *prm, double x, double y, double z) {
+  const double *G = prm + PMX_PRM_GRF;
+  double s = 0;
+  for (int k = 0; k < 64; k++) s += cos(G[4 * k] * x + G[4 * k + 1] * y + G[4 * k + 2] * z + G[4 * k + 3]);
+  return s * 0.17677669529663688110021109052621;  // sqrt(2/64)
+}
+
+__host__ __device__ inline QpCoeffs example_coeffs(const double *prm, double x, double y, double z) {
+  const double pi = 3.14159265358979323846;
+  QpCoeffs k;
+  const int ex = (int)prm[PMX_PRM_EXAMPLE];
+  const double eps = prm[PMX_PRM_EPS];
+  k.eps = eps;
+  k.c = prm[PMX_PRM_C];
+  k.b0 = prm[PMX_PRM_B0];
+  k.b1 = prm[PMX_PRM_B1];
+  k.b2 = prm[PMX_PRM_B2];
+  k.f = 0;
+  const double t = prm[PMX_PRM_TIME];
+  if (ex == PMX_EX_LAPLACE) {
+    k.b0 = k.b1 = k.b2 = 0;
+    k.c = 0;
+    k.f = 3 * pi * pi * eps * sin(pi * x) * sin(pi * y) * sin(pi * z);
+  } else if (ex == PMX_EX_SINE_CD) {
+    const double sx = sin(pi * x), sy = sin(pi * y), sz = sin(pi * z);
+    const double cx = cos(pi * x), cy = cos(pi * y), cz = cos(pi * z);
+    const double u = sx * sy * sz;
+    k.f = 3 * pi * pi * eps * u + pi * (k.b0 * cx * sy * sz + k.b1 * sx * cy * sz + k.b2 * sx * sy * cz) + k.c * u;
+  } else if (ex == PMX_EX_CURL_FOURIER) {
+    double b[3];
+    fourier_b(prm, x, y, z, b);
+    k.b0 = b[0];
+    k.b1 = b[1];
+    k.b2 = b[2];
+    k.c = 0;
+    k.f = 1.0;
+  } else if (ex == PMX_EX_HEAT_AMC) {
+    k.eps = 1.0;
+    k.b0 = k.b1 = k.b2 = 0;
+    k.c = 0;
+    k.f = (3. * pi * pi - 0.1) * exp(-0.1 * t) * sin(pi * x) * cos(pi * y) * cos(pi * z);
+  } else if (ex == PMX_EX_LOGNORMAL) {
+    k.eps = eps * exp(prm[PMX_PRM_GRF_SIGMA] * grf_field(prm, x, y, z));
+    k.f = 1.0;
+  } else if (ex == PMX_EX_ROTATING) {
+    k.b0 = -(y - 0.5);
+    k.b1 = x - 0.5;
+    k.b2 = 0.2;
+    k.c = 0;
+    k.f = 0.0;
+  }
+  return k;
+}
+
+__host__ __device__ inline double example_bound_value(const double *prm, double x, double y, double z) {
+  const double pi = 3.14159265358979323846;
+  const int ex = (int)prm[PMX_PRM_EXAMPLE];
+  const double t = prm[PMX_PRM_TIME];
+  if (ex == PMX_EX_LAPLACE || ex == PMX_EX_SINE_CD) return sin(pi * x) * sin(pi * y) * sin(pi * z);
+  if (ex == PMX_EX_HEAT_AMC) return exp(-0.1 * t) * sin(pi * x) * cos(pi * y) * cos(pi * z);
+  return 0.0;
+}
+
+__host__ __device__ inline double example_exact(const double *prm, double x, double y, double z) {
+  const double pi = 3.14159265358979323846;
+  const int ex = (int)prm[PMX_PRM_EXAMPLE];
+  const double t = prm[PMX_PRM_TIME];
+  if (ex == PMX_EX_HEAT_AMC) return exp(-0.1 * t) * sin(pi * x) * cos(pi * y) * cos(pi * z);
+  if (ex == PMX_EX_ROTATING) {
+    const double *B = prm + PMX_PRM_BLOB;
+    const double r2 = (x - B[0]) * (x - B[0]) + (y - B[1]) * (y - B[1]) + (z - B[2]) * (z - B[2]);
+    return exp(-r2 / (2 * B[3] * B[3]));
+  }
+  return sin(pi * x) * sin(pi * y) * sin(pi * z);
+}
+
+// Compute_SDFEM_delta for SDFEM_TYPE 0/1/2 with CELL_MEASURE 0
+__host__ __device__ inline double sdfem_delta(const double *prm, double hK, double eps, double b1, double b2,
+                                              double b3) {
+  const double delta0 = prm[PMX_PRM_DELTA0], delta1 = prm[PMX_PRM_DELTA1];
+  if (fabs(eps) < 1e-20) eps = 1e-20;
+  const double nb = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+  const int type = (int)prm[PMX_PRM_SDFEM_TYPE];
+  double delta = 0;
+  if (type == 0) {
+    delta = (eps < hK * nb) ? delta0 * hK / nb : delta1 * hK * hK / eps;
+  } else {
+    const double pd = prm[PMX_PRM_POLYDEG] > 0 ? prm[PMX_PRM_POLYDEG] : 1.0;
+    const double nu = 1.0 / pd;
+    if (nb > 0) {
+      const double alpha = nu * nb * hK / (2 * eps);
+      delta = nu * hK * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+    }
+    delta *= delta0;
+  }
+  return delta;
+}
+
+}  // namespace pmx

@@ -1,0 +1,579 @@
+// pmx_ops.hip -- operator, transfer, assembly and Krylov kernels (gfx950) and
+// their stream-ordered launchers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <stdexcept>
+
+#include "pmx_functors.hpp"
+#include "pmx_kernels.hpp"
+#include "pmx_ops.hpp"
+
+namespace pmx {
+
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restrict__ part, int np,
+                                                         double *__restrict__ out, int do_sqrt) {
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) out[0] = do_sqrt ? sqrt(s) : s;
+}
+
+// ============================================================ stencil rows
+// MODE 0: y = A x                          (MatVect, Blas2.C:86-118)
+// MODE 1: y = f - A x                      (ScalarDefect, Blas2.C:37-83)
+// MODE 2: y = (1-w) x + w (f - A_off x)/d  (TMGLevel3D::Jacobi, MGLevel3D.C:469-563), x = old iterate
+// MODE 3: x = w (s/d - x) + x in place     (TMGLevel3D::SOR, MGLevel3D.C:280-361) for one colour
+template <typename T> struct Acc { using type = T; };
+
+template <typename T, int EX, int EY, int EZ, int MODE>
+__device__ __forceinline__ void css_row(const DevSegment &s, int idx, int M, const T *__restrict__ val,
+                                        const T *x, const T *__restrict__ f, T *y, double omega) {
+  using A = typename Acc<T>::type;
+  const int i = idx % s.cx;
+  const int t = idx / s.cx;
+  const int j = t % s.cy, k = t / s.cy;
+  const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
+  const T *v = val + s.voff + idx;
+  const int64_t cnt = s.count;
+  A acc = 0, diag = 1;
+  int slot = 0;
+#pragma unroll
+  for (int dz = -EZ; dz <= EZ; dz++)
+#pragma unroll
+    for (int dy = -EY; dy <= EY; dy++)
+#pragma unroll
+      for (int dx = -EX; dx <= EX; dx++, slot++) {
+        const A a = (A)v[slot * cnt];
+        if (MODE >= 2 && dx == 0 && dy == 0 && dz == 0) diag = a;
+        else acc += a * (A)x[p + dx + M * (dy + M * dz)];
+      }
+  if (MODE == 0) y[p] = (T)acc;
+  else if (MODE == 1) y[p] = (T)((A)f[p] - acc);
+  else if (MODE == 2) {
+    const A told = (A)x[p];
+    y[p] = (T)((1 - (A)omega) * told + (A)omega * ((A)f[p] - acc) / diag);
+  } else {
+    const A told = (A)y[p];
+    y[p] = (T)((A)omega * (((A)f[p] - acc) / diag - told) + told);
+  }
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_css(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs,
+                                                int M, const T *__restrict__ val, const T *x, const T *__restrict__ f,
+                                                T *y, double omega) {
+  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int code = s.ex * 100 + s.ey * 10 + s.ez;
+  switch (code) {
+    case 111: css_row<T, 1, 1, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 211: css_row<T, 2, 1, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 121: css_row<T, 1, 2, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 221: css_row<T, 2, 2, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 112: css_row<T, 1, 1, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 212: css_row<T, 2, 1, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
+    case 122: css_row<T, 1, 2, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
+    default: css_row<T, 2, 2, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
+  }
+}
+
+// ============================================================ transfers
+// 1D taps of the nodal interpolation from the coarse to the fine lattice
+// (the prolongation matrices of FEDatabase3D.C:1656 for Q1/Q2 Lagrange,
+// averaged over the cells sharing a fine DOF as MGComponents3D.C:208-488
+// does, which for a continuous interpolant is the interpolant itself).
+__device__ __forceinline__ int prolong_taps(int x, int order, int nc, int *ci, double *w) {
+  if (order == 1) {
+    if ((x & 1) == 0) { ci[0] = x >> 1; w[0] = 1; return 1; }
+    ci[0] = x >> 1; ci[1] = (x >> 1) + 1; w[0] = w[1] = 0.5; return 2;
+  }
+  int e = x >> 2;
+  if (e > nc - 1) e = nc - 1;
+  const int r = x - 4 * e;
+  if (r == 0) { ci[0] = 2 * e; w[0] = 1; return 1; }
+  if (r == 2) { ci[0] = 2 * e + 1; w[0] = 1; return 1; }
+  if (r == 4) { ci[0] = 2 * e + 2; w[0] = 1; return 1; }
+  ci[0] = 2 * e; ci[1] = 2 * e + 1; ci[2] = 2 * e + 2;
+  if (r == 1) { w[0] = 0.375; w[1] = 0.75; w[2] = -0.125; }
+  else { w[0] = -0.125; w[1] = 0.75; w[2] = 0.375; }
+  return 3;
+}
+// transposed taps: fine nodes feeding coarse node y (DefectRestriction = P^T)
+__device__ __forceinline__ int restrict_taps(int y, int order, int Mf, int *fi, double *w) {
+  int n = 0;
+  auto add = [&](int x, double ww) {
+    if (x >= 0 && x < Mf) { fi[n] = x; w[n] = ww; n++; }
+  };
+  if (order == 1) {
+    add(2 * y - 1, 0.5); add(2 * y, 1.0); add(2 * y + 1, 0.5);
+  } else if ((y & 1) == 0) {
+    const int c = 2 * y;
+    add(c - 3, -0.125); add(c - 1, 0.375); add(c, 1.0); add(c + 1, 0.375); add(c + 3, -0.125);
+  } else {
+    const int c = 2 * y;  // = 4e+2
+    add(c - 1, 0.75); add(c, 1.0); add(c + 1, 0.75);
+  }
+  return n;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf, int Mc, int Mf,
+                                                    int order) {
+  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t Nf = (int64_t)Mf * Mf * Mf;
+  if (id >= Nf) return;
+  const int x = id % Mf, y = (id / Mf) % Mf, z = id / ((int64_t)Mf * Mf);
+  const int nc = (Mc - 1) / order;
+  int ix[3], iy[3], iz[3];
+  double wx[3], wy[3], wz[3];
+  const int nx = prolong_taps(x, order, nc, ix, wx), ny = prolong_taps(y, order, nc, iy, wy),
+            nz = prolong_taps(z, order, nc, iz, wz);
+  double s = 0;
+  for (int c = 0; c < nz; c++)
+    for (int b = 0; b < ny; b++) {
+      const double wzy = wz[c] * wy[b];
+      const T *row = xc + (int64_t)Mc * (iy[b] + (int64_t)Mc * iz[c]);
+      for (int a = 0; a < nx; a++) s += wzy * wx[a] * (double)row[ix[a]];
+    }
+  xf[id] = (T)s;
+}
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T *__restrict__ rf, int Mc, int Mf,
+                                                     int order) {
+  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t Nc = (int64_t)Mc * Mc * Mc;
+  if (id >= Nc) return;
+  const int x = id % Mc, y = (id / Mc) % Mc, z = id / ((int64_t)Mc * Mc);
+  int ix[5], iy[5], iz[5];
+  double wx[5], wy[5], wz[5];
+  const int nx = restrict_taps(x, order, Mf, ix, wx), ny = restrict_taps(y, order, Mf, iy, wy),
+            nz = restrict_taps(z, order, Mf, iz, wz);
+  double s = 0;
+  for (int c = 0; c < nz; c++)
+    for (int b = 0; b < ny; b++) {
+      const double wzy = wz[c] * wy[b];
+      const T *row = rf + (int64_t)Mf * (iy[b] + (int64_t)Mf * iz[c]);
+      for (int a = 0; a < nx; a++) s += wzy * wx[a] * (double)row[ix[a]];
+    }
+  rc[id] = (T)s;
+}
+
+// ============================================================ assembly
+// Stage 1: coefficients at every quadrature point of every cell, in the
+// canonical (axis-aligned) frame of the lattice: {w|detJ|, eps, b0, b1, b2, c, f, delta}.
+__global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ prm, const double *__restrict__ qref,
+                                                    int nq, int n, double *__restrict__ out) {
+  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t ncell = (int64_t)n * n * n;
+  if (id >= ncell * nq) return;
+  const int q = id % nq;
+  const int64_t cell = id / nq;
+  const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
+  const double h = 1.0 / n;
+  const double xi = qref[4 * q], eta = qref[4 * q + 1], zeta = qref[4 * q + 2], w = qref[4 * q + 3];
+  // THexaAffin::GetOrigFromRef: X = xc0 + xc1 xi + ... with xc0 the cell centre
+  const double X = (cx * h + 0.5 * h) + 0.5 * h * xi;
+  const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
+  const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
+  const QpCoeffs c = example_coeffs(prm, X, Y, Z);
+  const int form = (int)prm[PMX_PRM_FORM];
+  const double hK = sqrt(3.0) * h;  // THexahedron::GetDiameter of a cube
+  double delta = 0;
+  if (form == PMX_FORM_SDFEM) {
+    delta = sdfem_delta(prm, hK, c.eps, c.b0, c.b1, c.b2);
+  } else if (form == PMX_FORM_TCD_SUPG_AK) {
+    const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
+    delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
+  }
+  double *o = out + id * 8;
+  o[0] = w * (h * h * h / 8.0);
+  o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = delta;
+}
+
+// Stage 2: row gather.  The thread of active node p visits the cells that
+// contain p, forms row a of each local matrix
+//   K_ab = sum_q w [ u0 phi_b + u . grad phi_b ]
+// with s_a = phi_a + delta b.grad phi_a, u0 = c s_a, u = eps grad phi_a + b s_a
+// (BilinearAssemble / _SD, ConvDiff3D.C:118-145,182-213; mass and TCD forms,
+// TCD3D.C:37-322), and adds it into its own CSS row.  No two threads write
+// the same row, so no atomics and no colouring of cells are needed, and the
+// writes of a wavefront are coalesced.
+template <int ORDER, int EX, int EY, int EZ>
+__device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M, int n, int nq, int form,
+                                             const double *__restrict__ qp, const double *__restrict__ tab,
+                                             double *__restrict__ val, double *__restrict__ val2,
+                                             double *__restrict__ rhs) {
+  constexpr int K = ORDER + 1, NB = K * K * K;
+  const int i = idx % s.cx;
+  const int t = idx / s.cx;
+  const int j = t % s.cy, k = t / s.cy;
+  const int px = s.x0 + s.stride * i, py = s.y0 + s.stride * j, pz = s.z0 + s.stride * k;
+  const int p = px + M * (py + M * pz);
+  const double g = 2.0 * n;  // reference -> physical derivative factor 2/h
+  double *v = val + s.voff + idx;
+  double *v2 = val2 ? val2 + s.voff + idx : nullptr;
+  const int64_t cnt = s.count;
+  double frhs = 0;
+  // incident cells per direction: lattice coordinate x lies in cells
+  // floor((x-1)/ORDER) .. floor(x/ORDER) (two cells at a vertex plane)
+  const int cx0 = (px - 1) / ORDER, cx1 = px / ORDER;
+  const int cy0 = (py - 1) / ORDER, cy1 = py / ORDER;
+  const int cz0 = (pz - 1) / ORDER, cz1 = pz / ORDER;
+  for (int cz = cz0; cz <= cz1; cz++)
+    for (int cy = cy0; cy <= cy1; cy++)
+      for (int cx = cx0; cx <= cx1; cx++) {
+        const int ax = px - ORDER * cx, ay = py - ORDER * cy, az = pz - ORDER * cz;
+        const int a = ax + K * (ay + K * az);
+        const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        const double *Q = qp + cell * nq * 8;
+        double acc[NB], acc2[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[b] = acc2[b] = 0;
+        for (int q = 0; q < nq; q++) {
+          const double w = Q[8 * q], eps = Q[8 * q + 1], b0 = Q[8 * q + 2], b1 = Q[8 * q + 3], b2 = Q[8 * q + 4];
+          const double cc = Q[8 * q + 5], ff = Q[8 * q + 6], dl = Q[8 * q + 7];
+          const double *Ta = tab + (q * NB + a) * 4;
+          const double fa = Ta[0], gax = g * Ta[1], gay = g * Ta[2], gaz = g * Ta[3];
+          const double bga = b0 * gax + b1 * gay + b2 * gaz;
+          double u0, ux, uy, uz, k0 = 0;
+          if (form == PMX_FORM_MASS) {
+            u0 = fa; ux = uy = uz = 0;
+            frhs += w * ff * fa;
+          } else {
+            const double sa = fa + dl * bga;
+            u0 = cc * sa;
+            ux = eps * gax + b0 * sa;
+            uy = eps * gay + b1 * sa;
+            uz = eps * gaz + b2 * sa;
+            frhs += w * ff * sa;
+            k0 = dl * bga;
+          }
+          u0 *= w; ux *= w * g; uy *= w * g; uz *= w * g; k0 *= w;
+          const double *Tq = tab + q * NB * 4;
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            const double *Tb = Tq + 4 * b;
+            acc[b] += u0 * Tb[0] + ux * Tb[1] + uy * Tb[2] + uz * Tb[3];
+            acc2[b] += k0 * Tb[0];
+          }
+        }
+        // add the row into the CSS slots of node p
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          const int bx = b % K, by = (b / K) % K, bz = b / (K * K);
+          const int dx = ORDER * cx + bx - px, dy = ORDER * cy + by - py, dz = ORDER * cz + bz - pz;
+          const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
+          v[slot * cnt] += acc[b];
+          if (v2) v2[slot * cnt] += acc2[b];
+        }
+      }
+  rhs[p] = frhs;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
+                                                     const DevSegment *__restrict__ segs, int M, int n, int nq,
+                                                     const double *__restrict__ prm, const double *__restrict__ qp,
+                                                     const double *__restrict__ tab, double *__restrict__ val,
+                                                     double *__restrict__ val2, double *__restrict__ rhs) {
+  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int form = (int)prm[PMX_PRM_FORM];
+  if (ORDER == 1) {
+    assemble_row<1, 1, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs);
+    return;
+  }
+  const int code = s.ex * 100 + s.ey * 10 + s.ez;
+  switch (code) {
+    case 111: assemble_row<2, 1, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 211: assemble_row<2, 2, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 121: assemble_row<2, 1, 2, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 221: assemble_row<2, 2, 2, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 112: assemble_row<2, 1, 1, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 212: assemble_row<2, 2, 1, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 122: assemble_row<2, 1, 2, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    default: assemble_row<2, 2, 2, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+  }
+}
+// Dirichlet rhs: RHS[dof] = g(node) (AssembleMat3D.C:1031-1096)
+__global__ void k_boundary_rhs(const int *__restrict__ bnd, int nbd, int M, const double *__restrict__ prm,
+                               double *__restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbd) return;
+  const int p = bnd[i];
+  const double h = 1.0 / (M - 1);
+  const int x = p % M, y = (p / M) % M, z = p / (M * M);
+  rhs[p] = example_bound_value(prm, x * h, y * h, z * h);
+}
+
+// ============================================================ CSR export
+// entries in the reference CSR (sorted rows, Dirichlet rows = 1) from CSS values
+template <typename T>
+__global__ void k_css_to_csr(const int *__restrict__ rowptr, const int *__restrict__ kcol,
+                             const int *__restrict__ dof2lat, int64_t N, int64_t active, int M, int S,
+                             const int *__restrict__ segidx, const DevSegment *__restrict__ segs,
+                             const T *__restrict__ val, double *__restrict__ out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= N) return;
+  const int b = rowptr[r], e = rowptr[r + 1];
+  if (r >= active) { out[b] = 1.0; return; }
+  const int p = dof2lat[r];
+  const int x = p % M, y = (p / M) % M, z = p / (M * M);
+  const DevSegment s = segs[segidx[(x % S) + S * ((y % S) + S * (z % S))]];
+  const int idx = ((x - s.x0) / s.stride) + s.cx * (((y - s.y0) / s.stride) + s.cy * ((z - s.z0) / s.stride));
+  for (int n = b; n < e; n++) {
+    const int q = dof2lat[kcol[n]];
+    const int dx = q % M - x, dy = (q / M) % M - y, dz = q / (M * M) - z;
+    const int slot = (dx + s.ex) + (2 * s.ex + 1) * ((dy + s.ey) + (2 * s.ey + 1) * (dz + s.ez));
+    out[n] = (double)val[s.voff + (int64_t)slot * s.count + idx];
+  }
+}
+
+// ============================================================ permutations
+template <typename T, typename U>
+__global__ void k_gather(T *__restrict__ out, const U *__restrict__ in, const int *__restrict__ map, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[map[i]];
+}
+
+// ============================================================ generic CSR
+// y = A x on a caller CSR matrix (MatVect on any TSquareMatrix3D): 16 lanes per row
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int *__restrict__ rowptr,
+                                                     const int *__restrict__ kcol, const T *__restrict__ val,
+                                                     const T *__restrict__ x, T *__restrict__ y) {
+  const int lane = threadIdx.x & 15;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 4;
+  if (row >= n) return;
+  double s = 0;
+  for (int j = rowptr[row] + lane; j < rowptr[row + 1]; j += 16) s += (double)val[j] * (double)x[kcol[j]];
+  for (int o = 8; o > 0; o >>= 1) s += __shfl_down(s, o, 16);
+  if (lane == 0) y[row] = (T)s;
+}
+
+// ============================================================ dense coarse solve
+// y = Ainv x with the explicit inverse of the coarsest matrix (FP64), one wavefront per row
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gemv(int n, const double *__restrict__ Ainv, const T *__restrict__ x,
+                                                 T *__restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
+  if (row >= n) return;
+  double s = 0;
+  const double *a = Ainv + (int64_t)row * n;
+  for (int j = lane; j < n; j += 64) s += a[j] * (double)x[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) y[row] = (T)s;
+}
+
+// ============================================================ FGMRES scalars
+// Givens update of column i of H and of s (FgmresIte.C:194-222, 447-459),
+// H stored column-major with leading dimension m+1; writes |s[i+1]| to res.
+__global__ void k_givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double *h = H + (int64_t)i * ld;
+  for (int k = 0; k < i; k++) {
+    const double t = cs[k] * h[k] + sn[k] * h[k + 1];
+    h[k + 1] = -sn[k] * h[k] + cs[k] * h[k + 1];
+    h[k] = t;
+  }
+  const double dx = h[i], dy = h[i + 1];
+  double c, sg;
+  if (dy == 0.0) { c = 1.0; sg = 0.0; }
+  else if (fabs(dy) > fabs(dx)) { const double t = dx / dy; sg = 1.0 / sqrt(1.0 + t * t); c = t * sg; }
+  else { const double t = dy / dx; c = 1.0 / sqrt(1.0 + t * t); sg = t * c; }
+  cs[i] = c; sn[i] = sg;
+  {
+    const double t = c * h[i] + sg * h[i + 1];
+    h[i + 1] = -sg * h[i] + c * h[i + 1];
+    h[i] = t;
+  }
+  {
+    const double t = c * s[i] + sg * s[i + 1];
+    s[i + 1] = -sg * s[i] + c * s[i + 1];
+    s[i] = t;
+  }
+  res[0] = fabs(s[i + 1]);
+}
+// back substitution of UpdateGmresIterate (FgmresIte.C:224-243): y = H^{-1} s
+__global__ void k_backsolve(const double *H, const double *s, double *y, int k, int ld) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i <= k; i++) y[i] = s[i];
+  for (int i = k; i >= 0; i--) {
+    y[i] /= H[(int64_t)i * ld + i];
+    for (int j = i - 1; j >= 0; j--) y[j] -= H[(int64_t)i * ld + j] * y[i];
+  }
+}
+// x += sum_j y[j] z_j over k+1 vectors (z stored contiguously, stride n)
+template <typename T, typename X>
+__global__ __launch_bounds__(kBlock) void k_multi_axpy(X *__restrict__ x, const T *__restrict__ z,
+                                                       const double *__restrict__ y, int k1, int64_t n,
+                                                       int64_t stride) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    double s = 0;
+    for (int j = 0; j < k1; j++) s += y[j] * (double)z[j * stride + i];
+    x[i] += (X)s;
+  }
+}
+// write a scalar into H(i,j) from a reduction result
+__global__ void k_set_scalar(double *dst, const double *src, double scale) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = scale * src[0];
+}
+
+// ================================================================ launchers
+static inline int grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 65535 * 8) g = 65535 * 8;
+  return (int)(g < 1 ? 1 : g);
+}
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+void fill(T *x, T v, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, x, v, n);
+}
+template <typename T>
+void axpy(T *y, const T *x, const double *alpha_dev, double sgn, double ac, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, alpha_dev, sgn, ac, n);
+}
+template <typename T, typename U>
+void convert(T *y, const U *x, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert<T, U>), dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, n);
+}
+template <typename T, typename U>
+void add_convert(T *y, const U *x, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_add_convert<T, U>), dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, n);
+}
+template <typename T>
+void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_inv<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, nrm, n);
+}
+template <typename T>
+void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st) {
+  int g = (int)((n + kBlock - 1) / kBlock);
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(kBlock), 0, st, x, y, part, n);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, part, g, out, do_sqrt);
+}
+
+template <typename T>
+void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
+               hipStream_t st) {
+  const BlockInfo *blk = colour < 0 ? D.blocks_all : D.blocks_colour + D.colour_off[colour];
+  const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
+  if (nblk == 0) return;
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_css<T, 0>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    case 1: hipLaunchKernelGGL((k_css<T, 1>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    case 2: hipLaunchKernelGGL((k_css<T, 2>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    default: hipLaunchKernelGGL((k_css<T, 3>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+  }
+}
+template <typename T>
+void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st) {
+  const int g = (D.nboundary + kBlock - 1) / kBlock;
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_boundary<T, 0>), dim3(g), dim3(kBlock), 0, st, D.boundary, D.nboundary, y, x, f); break;
+    case 1: hipLaunchKernelGGL((k_boundary<T, 1>), dim3(g), dim3(kBlock), 0, st, D.boundary, D.nboundary, y, x, f); break;
+    case 2: hipLaunchKernelGGL((k_boundary<T, 2>), dim3(g), dim3(kBlock), 0, st, D.boundary, D.nboundary, y, x, f); break;
+    default: hipLaunchKernelGGL((k_boundary<T, 3>), dim3(g), dim3(kBlock), 0, st, D.boundary, D.nboundary, y, x, f); break;
+  }
+}
+template <typename T>
+void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st) {
+  hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, F.order);
+}
+template <typename T>
+void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st) {
+  hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M, F.order);
+}
+template <typename T>
+void gemv(int n, const double *Ainv, const T *x, T *y, hipStream_t st) {
+  const int g = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_gemv<T>, dim3(g), dim3(kBlock), 0, st, n, Ainv, x, y);
+}
+template <typename T, typename U>
+void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_gather<T, U>), dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, out, in,
+                     map, n);
+}
+template <typename T>
+void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st) {
+  const int g = (int)(((int64_t)n * 16 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_csr_spmv<T>, dim3(g), dim3(kBlock), 0, st, n, rp, kc, val, x, y);
+}
+template <typename T>
+void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
+                const T *val, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_css_to_csr<T>, dim3((unsigned)((D.N + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, rowptr,
+                     kcol, dof2lat, D.N, active, D.M, D.S, D.segidx, D.segs, val, out);
+}
+
+void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int nq, const double *tab,
+              double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st) {
+  const int64_t ncq = (int64_t)D.n * D.n * D.n * nq;
+  hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
+                     nq, D.n, qpbuf);
+  check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
+  if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
+  if (D.order == 1)
+    hipLaunchKernelGGL(k_assemble<1>, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, D.n, nq,
+                       prm_dev, qpbuf, tab, val, val2, rhs);
+  else
+    hipLaunchKernelGGL(k_assemble<2>, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, D.n, nq,
+                       prm_dev, qpbuf, tab, val, val2, rhs);
+  hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
+                     D.nboundary, D.M, prm_dev, rhs);
+}
+
+void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st) {
+  hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, st, H, cs, sn, s, i, ld, res);
+}
+void backsolve(const double *H, const double *s, double *y, int k, int ld, hipStream_t st) {
+  hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, st, H, s, y, k, ld);
+}
+template <typename T, typename X>
+void multi_axpy(X *x, const T *z, const double *y, int k1, int64_t n, int64_t stride, hipStream_t st) {
+  hipLaunchKernelGGL((k_multi_axpy<T, X>), dim3(grid_for(n)), dim3(kBlock), 0, st, x, z, y, k1, n, stride);
+}
+void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(64), 0, st, dst, src, scale);
+}
+
+// explicit instantiations
+#define PMX_INST(T)                                                                                         \
+  template void fill<T>(T *, T, int64_t, hipStream_t);                                                      \
+  template void axpy<T>(T *, const T *, const double *, double, double, int64_t, hipStream_t);              \
+  template void scale_inv<T>(T *, const T *, const double *, int64_t, hipStream_t);                         \
+  template void dot<T>(const T *, const T *, int64_t, double *, double *, int, hipStream_t);                \
+  template void css_apply<T>(const DevLayout &, int, const T *, const T *, const T *, T *, double, int,     \
+                             hipStream_t);                                                                  \
+  template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
+  template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
+  template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t);            \
+  template void gemv<T>(int, const double *, const T *, T *, hipStream_t);                                  \
+  template void csr_spmv<T>(int, const int *, const int *, const T *, const T *, T *, hipStream_t);         \
+  template void css_to_csr<T>(const DevLayout &, const int *, const int *, const int *, int64_t, const T *, \
+                              double *, hipStream_t);
+PMX_INST(double)
+PMX_INST(float)
+template void convert<float, double>(float *, const double *, int64_t, hipStream_t);
+template void convert<double, float>(double *, const float *, int64_t, hipStream_t);
+template void add_convert<double, float>(double *, const float *, int64_t, hipStream_t);
+template void add_convert<double, double>(double *, const double *, int64_t, hipStream_t);
+template void gather<double, double>(double *, const double *, const int *, int64_t, hipStream_t);
+template void gather<float, float>(float *, const float *, const int *, int64_t, hipStream_t);
+template void multi_axpy<double, double>(double *, const double *, const double *, int, int64_t, int64_t, hipStream_t);
+template void multi_axpy<float, float>(float *, const float *, const double *, int, int64_t, int64_t, hipStream_t);
+
+}  // namespace pmx

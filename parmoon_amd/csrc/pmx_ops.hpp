@@ -1,0 +1,63 @@
+// pmx_ops.hpp -- stream-ordered launchers of the libpmx kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pmx_internal.hpp"
+
+namespace pmx {
+
+// device mirror of a Layout
+struct DevLayout {
+  int order = 1, n = 0, M = 0, S = 2;
+  int64_t N = 0, nvals = 0, nactive = 0;
+  int nboundary = 0;
+  int *boundary = nullptr;
+  DevSegment *segs = nullptr;
+  int *segidx = nullptr;  // residue class (rx + S(ry + S rz)) -> segment, -1 if empty
+  BlockInfo *blocks_all = nullptr;
+  int nblocks_all = 0;
+  BlockInfo *blocks_colour = nullptr;
+  std::vector<int> colour_off;  // host: block offsets per colour
+  int ncolours = 0;
+};
+
+void check(hipError_t e, const char *what);
+
+template <typename T> void fill(T *x, T v, int64_t n, hipStream_t st);
+template <typename T>
+void axpy(T *y, const T *x, const double *alpha_dev, double sgn, double ac, int64_t n, hipStream_t st);
+template <typename T, typename U> void convert(T *y, const U *x, int64_t n, hipStream_t st);
+template <typename T, typename U> void add_convert(T *y, const U *x, int64_t n, hipStream_t st);
+template <typename T> void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st);
+template <typename T>
+void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st);
+
+// mode 0 spmv, 1 defect, 2 jacobi (x = old iterate, y = new), 3 sor (in place on y); colour -1 = all classes
+template <typename T>
+void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
+               hipStream_t st);
+// Dirichlet rows: mode 0 y=x, 1 y=f-x, 2 y=f, 3 y=0
+template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st);
+template <typename T>
+void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st);
+template <typename T> void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st);
+template <typename T> void gemv(int n, const double *Ainv, const T *x, T *y, hipStream_t st);
+template <typename T, typename U> void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st);
+template <typename T>
+void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st);
+template <typename T>
+void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
+                const T *val, double *out, hipStream_t st);
+void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int nq, const double *tab,
+              double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st);
+void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st);
+void backsolve(const double *H, const double *s, double *y, int k, int ld, hipStream_t st);
+template <typename T, typename X>
+void multi_axpy(X *x, const T *z, const double *y, int k1, int64_t n, int64_t stride, hipStream_t st);
+void set_scalar(double *dst, const double *src, double scale, hipStream_t st);
+
+}  // namespace pmx

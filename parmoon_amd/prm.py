@@ -1,0 +1,128 @@
+"""Parameter blocks (include/pmx_prm.h) for the CD3D/TCD3D workloads.
+
+ParMooN reads these values from TDatabase::ParamDB / TimeDB, filled from a
+`.dat` file (src/General/ReadParam.C:66) over the defaults of
+src/General/Database.C:350.  Here a workload is a plain float64 array of
+PRM_LEN entries that is handed to every entry point.
+"""
+import numpy as np
+
+PRM_LEN = 512
+EXAMPLE, EPS, B0, B1, B2, C, TIME, FORM = 0, 1, 2, 3, 4, 5, 6, 7
+SDFEM_TYPE, DELTA0, DELTA1, TAU, THETA1, LINEAR, BSCALE, POLYDEG = 8, 9, 10, 11, 12, 13, 14, 15
+FOURIER, GRF, GRF_SIGMA, BLOB = 16, 136, 392, 393
+
+EX_LAPLACE, EX_SINE_CD, EX_CURL_FOURIER, EX_HEAT_AMC, EX_LOGNORMAL, EX_ROTATING = range(6)
+FORM_GALERKIN, FORM_SDFEM, FORM_MASS, FORM_TCD_A, FORM_TCD_SUPG_AK = 1, 2, 3, 4, 5
+
+
+def base(example, eps=1.0, form=FORM_GALERKIN, b=(0.0, 0.0, 0.0), c=0.0, time=0.0,
+         sdfem_type=2, delta0=1.0, delta1=1.0, linear=False):
+    """A parameter block with ParMooN's defaults (Database.C:471-483, 895, 914)."""
+    p = np.zeros(PRM_LEN)
+    p[EXAMPLE] = example
+    p[EPS] = eps
+    p[B0:B2 + 1] = b
+    p[C] = c
+    p[TIME] = time
+    p[FORM] = form
+    p[SDFEM_TYPE] = sdfem_type
+    p[DELTA0] = delta0
+    p[DELTA1] = delta1
+    p[LINEAR] = 1.0 if linear else 0.0
+    p[POLYDEG] = 1.0
+    p[THETA1] = 1.0
+    p[TAU] = 1.0
+    p[BSCALE] = 1.0
+    return p
+
+
+def config1(form=FORM_SDFEM):
+    """32^3 Q1 manufactured CD3D: eps = 1e-3, b = (1,2,3)/sqrt(14), SUPG delta0 = 0.25."""
+    b = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
+    return base(EX_SINE_CD, eps=1e-3, form=form, b=b, delta0=0.25)
+
+
+def fourier_modes(seed=2024, nmodes=8, kmax=2):
+    """Seeded vector potential: per component 8 modes {a ~ N(0,1), k in [-kmax,kmax]^3 \\ 0, phi ~ U[0,2pi)}."""
+    rng = np.random.default_rng(seed)
+    F = np.zeros((3, nmodes, 5))
+    for c in range(3):
+        for m in range(nmodes):
+            while True:
+                k = rng.integers(-kmax, kmax + 1, size=3)
+                if np.any(k != 0):
+                    break
+            F[c, m, 0] = rng.standard_normal()
+            F[c, m, 1:4] = k
+            F[c, m, 4] = rng.uniform(0, 2 * np.pi)
+    return F
+
+
+def curl_b(F, x, y, z):
+    """b = curl(A), A_c = sum_m a sin(2 pi k.x + phi); x, y, z arrays."""
+    dA = np.zeros((3, 3) + np.shape(x))
+    for c in range(3):
+        for m in range(F.shape[1]):
+            a, kx, ky, kz, ph = F[c, m]
+            cs = a * np.cos(2 * np.pi * (kx * x + ky * y + kz * z) + ph) * 2 * np.pi
+            dA[c, 0] += cs * kx
+            dA[c, 1] += cs * ky
+            dA[c, 2] += cs * kz
+    return np.stack([dA[2, 1] - dA[1, 2], dA[0, 2] - dA[2, 0], dA[1, 0] - dA[0, 1]])
+
+
+def config2(nref=6, seed=2024, form=FORM_SDFEM, linear=True):
+    """64^3 Q2: eps = 1e-4, b = curl of a seeded Fourier potential with max|b| = 1
+    on the Q2 nodes, f = 1, homogeneous Dirichlet, SUPG (27-point rule when linear)."""
+    F = fourier_modes(seed)
+    M = 2 * (1 << nref) + 1
+    g = np.linspace(0.0, 1.0, M)
+    bmax = 0.0
+    for iz in range(M):  # slab by slab to bound memory
+        X, Y = np.meshgrid(g, g, indexing="ij")
+        b = curl_b(F, X, Y, np.full_like(X, g[iz]))
+        bmax = max(bmax, float(np.sqrt((b ** 2).sum(0)).max()))
+    p = base(EX_CURL_FOURIER, eps=1e-4, form=form, linear=linear)
+    p[FOURIER:FOURIER + 120] = F.reshape(-1)
+    p[BSCALE] = 1.0 / bmax
+    return p
+
+
+def heat_amc(time=0.0, form=FORM_TCD_A):
+    """Examples/TCD_3D/amc.h (eps = 1, b = 0)."""
+    return base(EX_HEAT_AMC, eps=1.0, form=form, time=time)
+
+
+def lognormal(seed=0, nref=6, corr=0.1, sigma=1.0, peclet=100.0):
+    """Config-4 coefficient field: eps(x) = exp(sigma G(x)) with a Gaussian random
+    field G of correlation length `corr` (random Fourier features), convection
+    (1,2,3)/sqrt(14) scaled to the cell Peclet number `peclet`."""
+    rng = np.random.default_rng(seed)
+    W = rng.standard_normal((64, 3)) / corr
+    ph = rng.uniform(0, 2 * np.pi, 64)
+    h = 1.0 / (1 << nref)
+    b = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0) * peclet / h
+    p = base(EX_LOGNORMAL, eps=1.0, form=FORM_SDFEM, b=b)
+    p[GRF:GRF + 256] = np.concatenate([W, ph[:, None]], 1).reshape(-1)
+    p[GRF_SIGMA] = sigma
+    return p
+
+
+def rotating(seed=0, width=0.05, form=FORM_TCD_A):
+    """Config-5 transient: eps = 1e-4, b = (-(y-1/2), x-1/2, 0.2), Gaussian blob."""
+    rng = np.random.default_rng(seed)
+    p = base(EX_ROTATING, eps=1e-4, form=form)
+    p[BLOB:BLOB + 3] = rng.uniform(0.3, 0.7, 3)
+    p[BLOB + 3] = width
+    return p
+
+
+def solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-10, red_factor=0.0, smoother=1,
+               pre=2, post=2, omega=0.8, coarse=17, coarse_sweeps=10, cycle=1, gmg_damp=1.0,
+               bicg_flexible=1, precision=0):
+    """The 16-entry solver option vector shared by libpmx and the oracle."""
+    o = np.zeros(16)
+    o[:] = [solver, restart, maxit, res_norm_min, red_factor, smoother, pre, post, omega, coarse,
+            coarse_sweeps, 0.1, cycle, gmg_damp, bicg_flexible, precision]
+    return o

@@ -1,0 +1,198 @@
+"""ctypes binding of the CPU oracle (oracle/liborc.so).
+
+Test infrastructure only: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the parity checker / timed CPU path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "..", "oracle", "liborc.so")
+PRM_LEN = 512
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        L.orc_space_new.restype = vp
+        L.orc_space_new.argtypes = [ip, ip]
+        L.orc_space_free.argtypes = [vp]
+        L.orc_space_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+        for n in ("orc_space_arrays",):
+            getattr(L, n).argtypes = [vp] + [ctypes.c_void_p] * 4
+        L.orc_dof_coords.argtypes = [vp, vp]
+        L.orc_cell_vertices.argtypes = [vp, vp]
+        L.orc_assemble.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_assemble.restype = ip
+        L.orc_matvec.argtypes = [vp, vp, vp, vp]
+        L.orc_defect.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_defect.restype = ctypes.c_double
+        L.orc_smooth.argtypes = [vp, vp, ip, ctypes.c_double, vp, vp, vp]
+        L.orc_prolongate.argtypes = [vp, vp, vp, vp]
+        L.orc_restrict.argtypes = [vp, vp, vp, vp]
+        L.orc_system_new.restype = vp
+        L.orc_system_new.argtypes = [ip, ip, ip]
+        L.orc_system_free.argtypes = [vp]
+        L.orc_system_space.restype = vp
+        L.orc_system_space.argtypes = [vp, ip]
+        L.orc_system_assemble.argtypes = [vp, vp]
+        L.orc_system_assemble.restype = ip
+        L.orc_system_entries.restype = dp
+        L.orc_system_entries.argtypes = [vp, ip]
+        L.orc_system_rhs.restype = dp
+        L.orc_system_rhs.argtypes = [vp, ip]
+        L.orc_system_solve.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_system_solve.restype = ip
+        L.orc_errors.argtypes = [vp, vp, vp, vp]
+        L.orc_interpolate.argtypes = [vp, vp, ip, vp]
+        L.orc_tcd3d_run.argtypes = [vp, vp, vp, vp, vp, vp, ip]
+        L.orc_tcd3d_run.restype = ip
+        L.orc_dense_lu_solve.argtypes = [ip, vp, vp]
+        L.orc_dense_lu_solve.restype = ip
+        _lib = L
+    return _lib
+
+
+def P(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Space:
+    """A standalone oracle FE space (unit cube refined nref times)."""
+
+    def __init__(self, nref, order, handle=None, owner=None):
+        self.L = lib()
+        self.owned = handle is None
+        self.h = handle if handle is not None else self.L.orc_space_new(nref, order)
+        self.owner = owner
+        d = (ctypes.c_longlong * 5)()
+        self.L.orc_space_dims(self.h, d)
+        self.N, self.active, self.ncells, self.nnz, self.nb = [int(v) for v in d]
+        self.nref, self.order = nref, order
+
+    def __del__(self):
+        if getattr(self, "owned", False) and self.h:
+            self.L.orc_space_free(self.h)
+            self.h = None
+
+    def arrays(self):
+        bi = np.zeros(self.ncells + 1, np.int32)
+        gn = np.zeros(self.ncells * self.nb, np.int32)
+        rp = np.zeros(self.N + 1, np.int32)
+        kc = np.zeros(self.nnz, np.int32)
+        self.L.orc_space_arrays(self.h, P(bi), P(gn), P(rp), P(kc))
+        return bi, gn, rp, kc
+
+    def dof_coords(self):
+        x = np.zeros((self.N, 3))
+        self.L.orc_dof_coords(self.h, P(x))
+        return x
+
+    def cell_vertices(self):
+        x = np.zeros((self.ncells, 8, 3))
+        self.L.orc_cell_vertices(self.h, P(x))
+        return x
+
+    def assemble(self, prm, two=False):
+        E = np.zeros(self.nnz)
+        r = np.zeros(self.N)
+        E2 = np.zeros(self.nnz) if two else None
+        rc = self.L.orc_assemble(self.h, P(prm), P(E), P(r), P(E2) if two else None)
+        assert rc == 0, rc
+        return (E, r, E2) if two else (E, r)
+
+    def matvec(self, E, x):
+        y = np.zeros(self.N)
+        self.L.orc_matvec(self.h, P(E), P(np.ascontiguousarray(x, float)), P(y))
+        return y
+
+    def defect(self, E, x, b):
+        r = np.zeros(self.N)
+        res = self.L.orc_defect(self.h, P(E), P(np.ascontiguousarray(x, float)),
+                                P(np.ascontiguousarray(b, float)), P(r))
+        return r, res
+
+    def smooth(self, E, kind, omega, sol, f):
+        aux = np.zeros(self.N)
+        s = np.array(sol, float)
+        self.L.orc_smooth(self.h, P(E), kind, omega, P(s), P(np.ascontiguousarray(f, float)), P(aux))
+        return s
+
+    def errors(self, prm, sol):
+        e = np.zeros(2)
+        self.L.orc_errors(self.h, P(prm), P(np.ascontiguousarray(sol, float)), P(e))
+        return e
+
+    def interpolate(self, prm, which=0):
+        s = np.zeros(self.N)
+        self.L.orc_interpolate(self.h, P(prm), which, P(s))
+        return s
+
+
+def prolongate(c, f, xc):
+    xf = np.zeros(f.N)
+    lib().orc_prolongate(c.h, f.h, P(np.ascontiguousarray(xc, float)), P(xf))
+    return xf
+
+
+def restrict(c, f, rf):
+    rc = np.zeros(c.N)
+    lib().orc_restrict(c.h, f.h, P(rc), P(np.ascontiguousarray(rf, float)))
+    return rc
+
+
+class System:
+    """Oracle multilevel system: levels nref0..nref0+nlev-1."""
+
+    def __init__(self, nref0, nlev, order):
+        self.L = lib()
+        self.h = self.L.orc_system_new(nref0, nlev, order)
+        self.levels = [Space(nref0 + l, order, handle=self.L.orc_system_space(self.h, l), owner=self)
+                       for l in range(nlev)]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_system_free(self.h)
+            self.h = None
+
+    def assemble(self, prm):
+        rc = self.L.orc_system_assemble(self.h, P(prm))
+        assert rc == 0, rc
+
+    def entries(self, level=-1):
+        l = level % len(self.levels)
+        return np.ctypeslib.as_array(self.L.orc_system_entries(self.h, l), (self.levels[l].nnz,)).copy()
+
+    def rhs(self, level=-1):
+        l = level % len(self.levels)
+        return np.ctypeslib.as_array(self.L.orc_system_rhs(self.h, l), (self.levels[l].N,)).copy()
+
+    def solve(self, opt, rhs, sol):
+        o = np.zeros(16)
+        o[:len(opt)] = opt
+        info = np.zeros(3)
+        s = np.array(sol, float)
+        self.L.orc_system_solve(self.h, P(o), P(np.ascontiguousarray(rhs, float)), P(s), P(info))
+        return s, info
+
+    def tcd3d(self, prm, tdisc, opt, sol, max_steps=100000):
+        o = np.zeros(16)
+        o[:len(opt)] = opt
+        err = np.zeros(5 * max_steps)
+        s = np.array(sol, float)
+        m = self.L.orc_tcd3d_run(self.h, P(prm), P(np.asarray(tdisc, float)), P(o), P(s), P(err), max_steps)
+        return s, err[:5 * m].reshape(m, 5)
+
+
+def dense_lu_solve(a, b):
+    a = np.asfortranarray(a, float).copy(order="F")
+    x = np.array(b, float)
+    rc = lib().orc_dense_lu_solve(a.shape[0], P(a), P(x))
+    assert rc == 0
+    return x
