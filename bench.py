@@ -1,0 +1,183 @@
+"""Benchmark of the CD3D hot path on one MI355X.
+
+Workload (one "step"): 64^3 Q2 SUPG convection-diffusion (2,146,689 DOFs,
+eps = 1e-4, b = curl of a seeded Fourier potential with max|b| = 1, f = 1,
+homogeneous Dirichlet, 27-point rule), 5 GMG levels 4^3..64^3.  A step
+assembles every level on the device and solves A x = b to FP64 relative
+residual 1e-10 with V(2,2) multicolour-SSOR GMG preconditioning FGMRES(30):
+  * plain form   : FP64 operator, MG and Krylov vectors;
+  * reduced form : FP32 operator/MG/Krylov vectors inside an FP64 defect
+                   correction (FP64 residual, Hessenberg/Givens, coarse solve).
+The metric is the reduced form's time to 1e-10 (seconds, lower is better),
+with the plain form, assembly DOF/s and the SpMV roofline alongside.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nref 6] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def cpu_baseline(nref_cpu=4, seed=2024):
+    """The oracle (single-threaded FP64 C++ restatement of the reference
+    algorithms: linear-search CSR scatter, lexicographic SSOR, cellwise
+    transfers, FGMRES with MGS) on the same workload at 16^3 Q2
+    (35,937 DOFs): assembly and solve to 1e-10, timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    from parmoon_amd import prm as P
+    prm = P.config2(nref=nref_cpu, seed=seed)
+    S = orc.System(nref_cpu - 4, 5, 2)
+    fine = S.levels[-1]
+    t0 = time.perf_counter()
+    S.assemble(prm)
+    t_asm_all = time.perf_counter() - t0
+    # assembly of the finest level alone
+    t0 = time.perf_counter()
+    fine.assemble(prm)
+    t_asm = time.perf_counter() - t0
+    rhs = S.rhs()
+    sol0 = np.zeros(fine.N)
+    sol0[fine.active:] = rhs[fine.active:]
+    E = S.entries()
+    r0 = np.linalg.norm(fine.defect(E, sol0, rhs)[0])
+    opt = P.solver_opt(solver=16, restart=30, maxit=300, res_norm_min=1e-10 * r0, smoother=3, omega=1.0)
+    t0 = time.perf_counter()
+    s, info = S.solve(opt, rhs, sol0)
+    t_solve = time.perf_counter() - t0
+    return {"N": fine.N, "t_assembly_fine": t_asm, "t_assembly_all": t_asm_all, "t_solve": t_solve,
+            "iters": int(info[0]), "final_rel_res": float(info[1] / r0)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nref", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp32,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import parmoon_amd as pa
+    from parmoon_amd import prm as P
+
+    nref = args.nref
+    prm = P.config2(nref=nref)
+    ctx = pa.Context(0)
+    S = pa.System(ctx, nref - 4, 5, 2)
+    d = S.dims(-1)
+    N = int(d[0])
+    print(f"[bench] 2^{nref} Q2: N={N} stencil values={int(d[6])}", file=sys.stderr, flush=True)
+
+    def opt(precision):
+        # res_norm_min is set per solve from the initial residual (relative 1e-10)
+        return P.solver_opt(solver=16, restart=30, maxit=400, res_norm_min=0.0, red_factor=0.0,
+                            smoother=3, omega=1.0, pre=2, post=2, precision=precision, rel_target=1e-10)
+
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    prec_of = {"fp64": 0, "fp32": 1}
+
+    def step(form):
+        t0 = time.perf_counter()
+        S.assemble(prm)
+        t1 = time.perf_counter()
+        info, rc = S.solve_resident(opt(prec_of[form]), check=False)
+        t2 = time.perf_counter()
+        if rc != 0:
+            print(f"[bench] {form} solve rc={rc} info={info}", file=sys.stderr, flush=True)
+        return t1 - t0, t2 - t1, info, rc
+
+    res = {}
+    for form in forms:
+        for w in range(args.warmup):
+            step(form)
+            print(f"[bench] warmup {form} {w} done", file=sys.stderr, flush=True)
+        ts_asm, ts_sol, infos = [], [], []
+        for k in range(args.steps):
+            ta, tsol, info, rc = step(form)
+            ts_asm.append(ta)
+            ts_sol.append(info[4])
+            infos.append(info)
+            print(f"[bench] {form} step {k}: assembly {ta:.4f}s solve {info[4]:.4f}s (wall {tsol:.4f}s) "
+                  f"iters {int(info[0])} outer {int(info[3])} rel.res {info[1] / info[2]:.2e}",
+                  file=sys.stderr, flush=True)
+        res[form] = {"t_asm": float(np.median(ts_asm)), "t_solve": float(np.median(ts_sol)),
+                     "iters": int(infos[-1][0]), "outer": int(infos[-1][3]),
+                     "rel_res": float(infos[-1][1] / infos[-1][2]), "rc": int(rc)}
+
+    # SpMV roofline on the finest level (FP32 and FP64 stencil values)
+    nvals = int(d[6])
+    roof = {}
+    for prec, s in ((1, 4), (0, 8)):
+        ms = S.time_kernel(0, -1, prec, reps=50)
+        nact = (2 ** (nref + 1) - 1) ** 3
+        bytes_alg = nvals * s + N * s + nact * s  # values + x (read once) + y (active rows)
+        roof[prec] = (bytes_alg / (ms * 1e-3) / 1e9, ms, bytes_alg)
+    kern = {name: S.time_kernel(k, -1, 1, reps=10) for name, k in
+            (("ssor_sweep_fp32_ms", 3), ("vcycle_fp32_ms", 7), ("restrict_fp32_ms", 6), ("prolong_fp32_ms", 5))}
+
+    main_form = "fp32" if "fp32" in res else forms[0]
+    t_main = res[main_form]["t_solve"]
+    t_asm = res[main_form]["t_asm"]
+    steps_ms = (res[main_form]["t_asm"] + res[main_form]["t_solve"]) * 1e3
+    achieved, ms_spmv, traffic_alg = roof[1]
+    out = {
+        "metric": "time-to-1e-10 FP64 residual (s), learned+FP32 vs plain FP64 GMG; assembly DOF/s",
+        "value": round(t_main, 5),
+        "unit": "s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(steps_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-in-f64" if main_form == "fp32" else "f64",
+        "data": "synthetic (seeded Fourier convection field)",
+        "config": {"workload": f"CD3D SUPG Q2 {2 ** nref}^3 (config 2), 5-level GMG V(2,2) multicolour SSOR, "
+                               f"FGMRES(30), FP64 rel. residual 1e-10", "dofs": N, "parallelism": "1 GPU"},
+        "plain_fp64_time_s": res.get("fp64", {}).get("t_solve"),
+        "fp32_time_s": res.get("fp32", {}).get("t_solve"),
+        "speedup_fp32_vs_fp64": (res["fp64"]["t_solve"] / res["fp32"]["t_solve"]) if "fp64" in res and "fp32" in res else None,
+        "learned_smoother": "not in this measurement",
+        "assembly_dofs_per_s": round(N / t_asm, 1),
+        "assembly_s": round(t_asm, 5),
+        "iterations": {f: res[f]["iters"] for f in res},
+        "final_rel_residual": {f: res[f]["rel_res"] for f in res},
+        "spmv_fp64_GBs": round(roof[0][0], 1),
+        "kernels_ms": {"spmv_fp32": round(ms_spmv, 4), "spmv_fp64": round(roof[0][1], 4),
+                       **{k: round(v, 4) for k, v in kern.items()}},
+        "roofline": {"bound": "hbm", "kernel": "k_css<float,0> (stencil SpMV, finest level)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": int(traffic_alg)},
+    }
+    if not args.no_cpu:
+        try:
+            c = cpu_baseline()
+            scale = N / c["N"]
+            out["cpu_baseline"] = {"value": round(c["t_solve"] * scale, 3), "unit": "s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"oracle solve of the same workload at 16^3 Q2 ({c['N']} DOFs, "
+                                             f"{c['iters']} FGMRES its, {c['t_solve']:.2f}s) scaled by N ratio "
+                                             f"{scale:.1f}; assembly {c['N'] / c['t_assembly_fine']:.0f} DOF/s",
+                                   "assembly_dofs_per_s": round(c["N"] / c["t_assembly_fine"], 1)}
+        except Exception as e:  # the CPU leg never hides the GPU line
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

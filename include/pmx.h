@@ -103,7 +103,7 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
 /* ---- solvers -------------------------------------------------------------
  * TSystemCD3D::Solve -> TItMethod::Iterate (include/FE/ItMethod.h) with the
  * TMultiGridScaIte preconditioner (src/FE/MultiGridScaIte.C:89-130).
- * opt (16 doubles, same layout as the oracle):
+ * opt (24 doubles; entries 0-15 have the oracle's layout):
  *  [0] solver: 16 FGMRES (FgmresIte.C:245), 11 Richardson (FixedPointIte.C:176),
  *      13 BiCGStab (Bcgs.C:111), 0 = one MG cycle
  *  [1] restart  [2] maxit  [3] res_norm_min (absolute)  [4] red_factor
@@ -113,6 +113,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [14] BiCGStab flexible branch (1 = v = A y)
  *  [15] precision: 0 FP64 throughout, 1 FP32 operator/MG/Krylov vectors
  *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother
+ *  [16] stop when ||r|| <= [16] * ||r0|| (r0 = initial FP64 defect; 0 = off)
+ *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

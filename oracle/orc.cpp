@@ -750,6 +750,17 @@ static void smooth(const Space &S, const double *E, int kind, double omega, doub
     for (int i = NA - 1; i >= 0; i--) row_sweep(S, E, i, omega, sol, f, sol);
 }
 
+// SOR / SSOR visiting the active rows in a caller-given order (the same
+// update as TMGLevel3D::SOR/SSOR, MGLevel3D.C:280-468, with the row loop
+// permuted; used to check multicolour sweeps row for row)
+static void smooth_ordered(const Space &S, const double *E, int kind, double omega, double *sol,
+                           const double *f, const int *order, int n) {
+  std::memcpy(sol + S.active, f + S.active, sizeof(double) * (S.N - S.active));
+  for (int k = 0; k < n; k++) row_sweep(S, E, order[k], omega, sol, f, sol);
+  if (kind == 3)
+    for (int k = n - 1; k >= 0; k--) row_sweep(S, E, order[k], omega, sol, f, sol);
+}
+
 // ===========================================================================
 // Grid transfers: cellwise Prolongate / DefectRestriction with fine-DOF
 // multiplicity (src/FE/MGComponents3D.C:208-488, 732-1014).  QQ of child j is
@@ -784,8 +795,8 @@ static Transfer make_transfer(const Space &C, const Space &F, const Tree &T) {
       node_ref(F.order, k, r);
       Af.map(r, X);
       const double dx[3] = {X[0] - Ap.xc[0], X[1] - Ap.yc[0], X[2] - Ap.zc[0]};
-      // ref = J^{-1} (X - c0); rows of I are d(ref)/dX
-      for (int a = 0; a < 3; a++) rp[a] = I[3 * a] * dx[0] + I[3 * a + 1] * dx[1] + I[3 * a + 2] * dx[2];
+      // ref = J^{-1} (X - c0); I[3a+k] = d ref_k / d X_a (HexaAffin.C:612-620)
+      for (int k = 0; k < 3; k++) rp[k] = I[k] * dx[0] + I[3 + k] * dx[1] + I[6 + k] * dx[2];
       std::vector<double> v(C.nb), d1(C.nb), d2(C.nb), d3(C.nb);
       basis(C.order, rp, v.data(), d1.data(), d2.data(), d3.data());
       for (int l = 0; l < C.nb; l++) Q[(size_t)k * C.nb + l] = v[l];
@@ -1182,6 +1193,10 @@ double orc_defect(void *sp, const double *E, const double *x, const double *b, d
 void orc_smooth(void *sp, const double *E, int kind, double omega, double *sol, const double *f,
                 double *aux) {
   smooth(SP(sp), E, kind, omega, sol, f, aux);
+}
+void orc_smooth_ordered(void *sp, const double *E, int kind, double omega, double *sol, const double *f,
+                        const int *order, int n) {
+  smooth_ordered(SP(sp), E, kind, omega, sol, f, order, n);
 }
 
 // Transfers between two spaces.  Spaces of one system share a tree; two

@@ -42,6 +42,10 @@ double orc_defect(void *sp, const double *entries, const double *x,
 void orc_smooth(void *sp, const double *entries, int kind, double omega,
                 double *sol, const double *f, double *aux);
 
+/* SOR (2) / SSOR (3) with the active rows visited in `order` (n rows) */
+void orc_smooth_ordered(void *sp, const double *entries, int kind, double omega, double *sol,
+                        const double *f, const int *order, int n);
+
 /* ---- grid transfers between space `coarse` (nref) and `fine` (nref+1) ---- */
 void orc_prolongate(void *coarse, void *fine, const double *xc, double *xf);
 void orc_restrict(void *coarse, void *fine, double *rc, const double *rf);

@@ -204,7 +204,7 @@ class System:
         return rc
 
     def solve(self, opt, rhs, sol, check=True):
-        o = np.zeros(16)
+        o = np.zeros(24)
         o[:len(opt)] = opt
         info = np.zeros(8)
         s = np.array(sol, float)
@@ -214,7 +214,7 @@ class System:
         return s, info, rc
 
     def solve_resident(self, opt, check=True):
-        o = np.zeros(16)
+        o = np.zeros(24)
         o[:len(opt)] = opt
         info = np.zeros(8)
         rc = lib().pmx_system_solve_resident(self.h, _p(o), _p(info))

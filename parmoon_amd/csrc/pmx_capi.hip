@@ -18,7 +18,7 @@ namespace pmx {
 
 static thread_local std::string g_err;
 // V(2,2) SSOR with the exact coarse solve, used when a single kernel is timed
-static const double PMX_DEFAULT_VCYCLE[16] = {0, 1, 1, 0, 0, 3, 2, 2, 1.0, 17, 1, 0, 1, 1.0, 0, 0};
+static const double PMX_DEFAULT_VCYCLE[24] = {0, 1, 1, 0, 0, 3, 2, 2, 1.0, 17, 1, 0, 1, 1.0, 0, 0};
 
 template <typename T>
 struct DevBuf {
@@ -91,7 +91,7 @@ struct System {
   DevBuf<float> krylov32;
   DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
   std::vector<int> rec;
-  double opt[16];
+  double opt[24];
 };
 
 // ------------------------------------------------------------- set-up
@@ -537,8 +537,18 @@ static int solve_device(System &Y, double *info) {
   int status = PMX_OK, its = 0, outer = 0;
   double res = 0, res0 = 0;
   const int maxit = (int)Y.opt[2];
+  const double rel_target = Y.opt[16];
+  double *nr = Y.ctx->part.p + kRedBlocks;
   if (prec == 0) {
-    its = krylov<double>(Y, Y.sol.p, Y.b.p, maxit, Y.opt[3], Y.opt[4], &res, &res0, &status);
+    double rmin = Y.opt[3];
+    if (rel_target > 0) {
+      // absolute SC_LIN_RES_NORM_MIN = rel_target * ||b - A x0|| over all rows
+      Y.r64.alloc(N);
+      fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
+      dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
+      rmin = std::max(rmin, rel_target * read_scalar(Y, nr));
+    }
+    its = krylov<double>(Y, Y.sol.p, Y.b.p, maxit, rmin, Y.opt[4], &res, &res0, &status);
     outer = 1;
   } else {
     // FP64 defect correction around an FP32 Krylov/MG solve
@@ -548,12 +558,11 @@ static int solve_device(System &Y, double *info) {
     r32.alloc(N);
     e32.alloc(N);
     const double rmin = Y.opt[3], red = Y.opt[4];
-    const double inner_red = 1e-4;
+    const double inner_red = Y.opt[17] > 0 ? Y.opt[17] : 1e-4;
     fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
-    double *nr = Y.ctx->part.p + kRedBlocks;
     dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
     res0 = res = read_scalar(Y, nr);
-    const double target = std::max(rmin, red * res0);
+    const double target = std::max(std::max(rmin, red * res0), rel_target * res0);
     while (res > target && its < maxit) {
       convert<float, double>(r32.p, Y.r64.p, N, st);
       fill<float>(e32.p, 0.0f, N, st);

@@ -37,10 +37,20 @@ def base(example, eps=1.0, form=FORM_GALERKIN, b=(0.0, 0.0, 0.0), c=0.0, time=0.
     return p
 
 
-def config1(form=FORM_SDFEM):
-    """32^3 Q1 manufactured CD3D: eps = 1e-3, b = (1,2,3)/sqrt(14), SUPG delta0 = 0.25."""
+def config1(form=FORM_SDFEM, delta0=0.25):
+    """32^3 Q1 manufactured CD3D: eps = 1e-3, b = (1,2,3)/sqrt(14), SUPG delta0 = 0.25.
+
+    With delta0 = 0.25 the damped-Jacobi and Gauss-Seidel iteration matrices
+    of this operator have spectral radius > 1 (3.3 and 5.9-7.0 at 8^3/16^3),
+    so no V-cycle built on them contracts; solves use delta0 = 1 (ParMooN's
+    default DELTA0), see config1b()."""
     b = np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0)
-    return base(EX_SINE_CD, eps=1e-3, form=form, b=b, delta0=0.25)
+    return base(EX_SINE_CD, eps=1e-3, form=form, b=b, delta0=delta0)
+
+
+def config1b(form=FORM_SDFEM):
+    """config1 with ParMooN's default DELTA0 = 1 (Database.C:480): the solvable variant."""
+    return config1(form=form, delta0=1.0)
 
 
 def fourier_modes(seed=2024, nmodes=8, kmax=2):
@@ -120,9 +130,9 @@ def rotating(seed=0, width=0.05, form=FORM_TCD_A):
 
 def solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-10, red_factor=0.0, smoother=1,
                pre=2, post=2, omega=0.8, coarse=17, coarse_sweeps=10, cycle=1, gmg_damp=1.0,
-               bicg_flexible=1, precision=0):
-    """The 16-entry solver option vector shared by libpmx and the oracle."""
-    o = np.zeros(16)
-    o[:] = [solver, restart, maxit, res_norm_min, red_factor, smoother, pre, post, omega, coarse,
-            coarse_sweeps, 0.1, cycle, gmg_damp, bicg_flexible, precision]
+               bicg_flexible=1, precision=0, rel_target=0.0, inner_red=1e-4):
+    """The 24-entry solver option vector; entries 0-15 are shared with the oracle."""
+    o = np.zeros(24)
+    o[:18] = [solver, restart, maxit, res_norm_min, red_factor, smoother, pre, post, omega, coarse,
+              coarse_sweeps, 0.1, cycle, gmg_damp, bicg_flexible, precision, rel_target, inner_red]
     return o

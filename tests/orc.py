@@ -34,6 +34,7 @@ def lib():
         L.orc_defect.argtypes = [vp, vp, vp, vp, vp]
         L.orc_defect.restype = ctypes.c_double
         L.orc_smooth.argtypes = [vp, vp, ip, ctypes.c_double, vp, vp, vp]
+        L.orc_smooth_ordered.argtypes = [vp, vp, ip, ctypes.c_double, vp, vp, vp, ip]
         L.orc_prolongate.argtypes = [vp, vp, vp, vp]
         L.orc_restrict.argtypes = [vp, vp, vp, vp]
         L.orc_system_new.restype = vp
@@ -124,6 +125,12 @@ class Space:
         self.L.orc_smooth(self.h, P(E), kind, omega, P(s), P(np.ascontiguousarray(f, float)), P(aux))
         return s
 
+    def smooth_ordered(self, E, kind, omega, sol, f, order):
+        s = np.array(sol, float)
+        o = np.ascontiguousarray(order, np.int32)
+        self.L.orc_smooth_ordered(self.h, P(E), kind, omega, P(s), P(np.ascontiguousarray(f, float)), P(o), len(o))
+        return s
+
     def errors(self, prm, sol):
         e = np.zeros(2)
         self.L.orc_errors(self.h, P(prm), P(np.ascontiguousarray(sol, float)), P(e))
@@ -174,7 +181,7 @@ class System:
         return np.ctypeslib.as_array(self.L.orc_system_rhs(self.h, l), (self.levels[l].N,)).copy()
 
     def solve(self, opt, rhs, sol):
-        o = np.zeros(16)
+        o = np.zeros(24)
         o[:len(opt)] = opt
         info = np.zeros(3)
         s = np.array(sol, float)
@@ -182,7 +189,7 @@ class System:
         return s, info
 
     def tcd3d(self, prm, tdisc, opt, sol, max_steps=100000):
-        o = np.zeros(16)
+        o = np.zeros(24)
         o[:len(opt)] = opt
         err = np.zeros(5 * max_steps)
         s = np.array(sol, float)

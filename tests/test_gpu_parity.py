@@ -1,0 +1,206 @@
+"""HIP path (libpmx through the C-ABI) against the CPU oracle on identical inputs.
+
+Tolerances: integer structure bit-exact; assembled entries and rhs within
+1e-12 of the row's largest oracle entry (entries that cancel to zero make a
+pure per-entry relative bound meaningless); SpMV / defect / smoothers /
+transfers within 1e-13 relative (norm-wise); converged solutions within 1e-10
+relative in the max norm.
+"""
+import numpy as np
+import pytest
+
+import orc
+import parmoon_amd as pa
+from parmoon_amd import prm as P
+
+pytestmark = pytest.mark.gpu
+
+
+def row_rel_err(E_ref, E, rowptr):
+    rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
+    scale = np.maximum.reduceat(np.abs(E_ref), rowptr[:-1])
+    return float(np.max(np.abs(E - E_ref) / scale[rows]))
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+CASES = [
+    ("laplace-Q1-galerkin", 1, 3, lambda: P.base(P.EX_LAPLACE, eps=1e-2)),
+    ("config1-Q1-sdfem", 1, 3, lambda: P.config1()),
+    ("config1-Q1-galerkin", 1, 3, lambda: P.config1(form=P.FORM_GALERKIN)),
+    ("heat-Q1-mass", 1, 3, lambda: P.heat_amc(0.3, form=P.FORM_MASS)),
+    ("heat-Q1-tcdA", 1, 3, lambda: P.heat_amc(0.3)),
+    ("curl-Q2-sdfem-27pt", 2, 2, lambda: P.config2(nref=2)),
+    ("curl-Q2-sdfem-64pt", 2, 2, lambda: P.config2(nref=2, linear=False)),
+    ("config1-Q2-galerkin", 2, 2, lambda: P.config1(form=P.FORM_GALERKIN)),
+    ("lognormal-Q1-sdfem", 1, 3, lambda: P.lognormal(nref=3, peclet=10.0)),
+    ("rotating-Q1-supgAK", 1, 3, lambda: dict_supg(P.rotating())),
+]
+
+
+def dict_supg(p):
+    p = p.copy()
+    p[P.FORM] = P.FORM_TCD_SUPG_AK
+    p[P.TAU] = 1e-2
+    p[P.THETA1] = 0.5
+    return p
+
+
+@pytest.mark.parametrize("name,order,nref,mk", CASES, ids=[c[0] for c in CASES])
+def test_assembly_matches_oracle(ctx, name, order, nref, mk):
+    prm = mk()
+    print(f"[case] assembly {name} order={order} nref={nref}", flush=True)
+    sp = orc.Space(nref, order)
+    two = prm[P.FORM] == P.FORM_TCD_SUPG_AK
+    out = sp.assemble(prm, two=two)
+    S = pa.System(ctx, nref, 1, order)
+    S.assemble(prm)
+    rp, kc = S.structure()
+    bi, gn, rp0, kc0 = sp.arrays()
+    assert np.array_equal(rp, rp0) and np.array_equal(kc, kc0)
+    E = S.entries()
+    assert row_rel_err(out[0], E, rp) < 1e-12
+    rhs = S.rhs()
+    assert rel(rhs, out[1]) < 1e-12
+    if two:
+        assert row_rel_err(out[2], S.entries(which=1), rp) < 1e-12
+    S.close()
+
+
+def test_jacobi_mg_laplace(ctx):
+    """damped-Jacobi V(2,2) GMG-FGMRES on the Laplace example (Laplace.h)."""
+    prm = P.base(P.EX_LAPLACE, eps=1.0)
+    S = pa.System(ctx, 1, 3, 1)
+    S.assemble(prm)
+    O = orc.System(1, 3, 1)
+    O.assemble(prm)
+    rhs = O.rhs()
+    ref = reference_solution(O, rhs, 1)
+    fine = O.levels[-1]
+    sol0 = np.zeros_like(rhs)
+    sol0[fine.active:] = rhs[fine.active:]
+    opt = P.solver_opt(solver=16, restart=20, maxit=100, res_norm_min=0.0, red_factor=1e-13, smoother=1, omega=0.8)
+    s, info, rc = S.solve(opt, rhs, sol0)
+    assert rc == 0 and rel(s, ref) < 1e-10
+    S.close()
+
+
+def test_numbering_through_device(ctx):
+    for order, nref in [(1, 3), (2, 2)]:
+        sp = orc.Space(nref, order)
+        bi0, gn0, _, _ = sp.arrays()
+        S = pa.System(ctx, nref, 1, order)
+        bi, gn = S.numbering()
+        assert np.array_equal(bi, bi0) and np.array_equal(gn, gn0)
+        S.close()
+
+
+@pytest.fixture(scope="module")
+def q1sys(ctx):
+    prm = P.config1b()
+    S = pa.System(ctx, 1, 3, 1)     # 2^3 .. 8^3 Q1
+    S.assemble(prm)
+    O = orc.System(1, 3, 1)
+    O.assemble(prm)
+    yield S, O, prm
+    S.close()
+
+
+@pytest.fixture(scope="module")
+def q2sys(ctx):
+    prm = P.config2(nref=3)
+    S = pa.System(ctx, 1, 3, 2)     # 2^3 .. 8^3 Q2
+    S.assemble(prm)
+    O = orc.System(1, 3, 2)
+    O.assemble(prm)
+    yield S, O, prm
+    S.close()
+
+
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+def test_blas2_and_smoothers(request, which):
+    S, O, prm = request.getfixturevalue(which)
+    fine = O.levels[-1]
+    E = O.entries()
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(fine.N)
+    b = rng.standard_normal(fine.N)
+    print(f"[case] blas2 {which}", flush=True)
+    assert rel(S.matvec(x), fine.matvec(E, x)) < 1e-13
+    assert rel(S.matvec(x, precision=1), fine.matvec(E, x)) < 1e-6
+    r, res = S.defect(x, b)
+    r0, res0 = fine.defect(E, x, b)
+    assert rel(r, r0) < 1e-13 and abs(res - res0) < 1e-12 * res0
+    # damped Jacobi sweep: identical update
+    assert rel(S.smooth(1, 0.8, 1, x, b), fine.smooth(E, 1, 0.8, x, b)) < 1e-13
+    # multicolour SSOR equals SSOR with the rows in colour order
+    col = S.colours()
+    order = np.argsort(col[:fine.active], kind="stable").astype(np.int32)
+    for kind in (2, 3):
+        g = S.smooth(kind, 1.0, 1, x, b)
+        o = fine.smooth_ordered(E, kind, 1.0, x, b, order)
+        assert rel(g, o) < 1e-12, kind
+    # generic CSR SpMV on the reference arrays
+    rp, kc = S.structure()
+    assert rel(S.ctx.csr_matvec(rp, kc, E, x), fine.matvec(E, x)) < 1e-13
+
+
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+def test_transfers(request, which):
+    S, O, prm = request.getfixturevalue(which)
+    rng = np.random.default_rng(2)
+    for fl in (1, 2):
+        c, f = O.levels[fl - 1], O.levels[fl]
+        xc = rng.standard_normal(c.N)
+        rf = rng.standard_normal(f.N)
+        print(f"[case] transfers {which} fine_level={fl}", flush=True)
+        assert rel(S.prolongate(xc, fl), orc.prolongate(c, f, xc)) < 1e-13
+        assert rel(S.restrict(rf, fl), orc.restrict(c, f, rf)) < 1e-13
+
+
+def reference_solution(O, rhs, smoother):
+    opt = P.solver_opt(solver=16, restart=30, maxit=300, res_norm_min=0.0, red_factor=1e-13,
+                       smoother=smoother, omega=0.8 if smoother == 1 else 1.0)
+    sol0 = np.zeros_like(rhs)
+    fine = O.levels[-1]
+    sol0[fine.active:] = rhs[fine.active:]
+    s, info = O.solve(opt, rhs, sol0)
+    return s
+
+
+@pytest.mark.parametrize("which,smoother", [("q1sys", 3), ("q1sys", 2), ("q2sys", 3)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fgmres_mg_solution(request, which, smoother, precision):
+    S, O, prm = request.getfixturevalue(which)
+    rhs = O.rhs()
+    ref = reference_solution(O, rhs, smoother)
+    fine = O.levels[-1]
+    r0 = np.linalg.norm(fine.defect(O.entries(), np.where(np.arange(fine.N) < fine.active, 0.0, rhs), rhs)[0])
+    opt = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-12 * r0, smoother=smoother,
+                       omega=0.8 if smoother == 1 else 1.0, precision=precision)
+    sol0 = np.zeros_like(rhs)
+    sol0[fine.active:] = rhs[fine.active:]
+    print(f"[case] fgmres {which} smoother={smoother} precision={precision}", flush=True)
+    s, info, rc = S.solve(opt, rhs, sol0)
+    assert rc == 0, info
+    assert info[1] <= 1e-12 * r0 * 1.0001
+    assert rel(s, ref) < 1e-10
+
+
+@pytest.mark.parametrize("solver", [11, 13])
+def test_other_krylov(q1sys, solver):
+    S, O, prm = q1sys
+    rhs = O.rhs()
+    ref = reference_solution(O, rhs, 3)
+    fine = O.levels[-1]
+    sol0 = np.zeros_like(rhs)
+    sol0[fine.active:] = rhs[fine.active:]
+    opt = P.solver_opt(solver=solver, maxit=400, res_norm_min=0.0, red_factor=1e-13, smoother=3, omega=1.0)
+    print(f"[case] krylov solver={solver}", flush=True)
+    s, info, rc = S.solve(opt, rhs, sol0)
+    assert rc == 0, info
+    assert rel(s, ref) < 1e-9
+    o, oinfo = O.solve(opt, rhs, sol0)
+    assert rel(o, ref) < 1e-9
