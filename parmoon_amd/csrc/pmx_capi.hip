@@ -186,53 +186,17 @@ static void make_tables(int order, bool linear, std::vector<double> &qref, std::
       }
 }
 
-// dense LU inverse of the coarsest operator (host, FP64)
+// explicit FP64 inverse of the coarsest operator (device Gauss-Jordan); used
+// as the exact coarse solve (SC_COARSE_SMOOTHER_SCALAR 17, MGLevel3D.C:865-905,
+// which refactors a dense copy on every call; here it is formed once per assembly)
 static void coarse_inverse(System &Y, Level &lv) {
-  const Layout &L = lv.L;
-  const int n = (int)L.N;
+  const int n = (int)lv.L.N;
   if (n > 6000) { lv.ainv_n = 0; return; }
-  hipStream_t st = Y.ctx->st;
-  std::vector<double> val(L.nvals);
-  check(hipMemcpyAsync(val.data(), lv.v64.val.p, sizeof(double) * L.nvals, hipMemcpyDeviceToHost, st), "d2h");
-  check(hipStreamSynchronize(st), "sync");
-  std::vector<double> A((size_t)n * n, 0.0);  // row-major, lattice order
-  const int M = L.M;
-  for (int p : L.boundary) A[(size_t)p * n + p] = 1.0;
-  for (const Segment &s : L.segs) {
-    for (int64_t idx = 0; idx < s.count; idx++) {
-      const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
-      const int x = s.x0 + L.S * i, y = s.y0 + L.S * j, z = s.z0 + L.S * k;
-      const int p = x + M * (y + M * z);
-      int slot = 0;
-      for (int dz = -s.ez; dz <= s.ez; dz++)
-        for (int dy = -s.ey; dy <= s.ey; dy++)
-          for (int dx = -s.ex; dx <= s.ex; dx++, slot++) {
-            const int q = (x + dx) + M * ((y + dy) + M * (z + dz));
-            A[(size_t)p * n + q] = val[s.voff + slot * s.count + idx];
-          }
-    }
-  }
-  // Gauss-Jordan with partial pivoting -> inverse
-  std::vector<double> inv((size_t)n * n, 0.0);
-  for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
-  for (int c = 0; c < n; c++) {
-    int piv = c;
-    for (int r = c + 1; r < n; r++)
-      if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)piv * n + c])) piv = r;
-    if (A[(size_t)piv * n + c] == 0) throw std::runtime_error("singular coarse matrix");
-    if (piv != c)
-      for (int j = 0; j < n; j++) { std::swap(A[(size_t)c * n + j], A[(size_t)piv * n + j]); std::swap(inv[(size_t)c * n + j], inv[(size_t)piv * n + j]); }
-    const double d = 1.0 / A[(size_t)c * n + c];
-    for (int j = 0; j < n; j++) { A[(size_t)c * n + j] *= d; inv[(size_t)c * n + j] *= d; }
-    for (int r = 0; r < n; r++) {
-      if (r == c) continue;
-      const double f = A[(size_t)r * n + c];
-      if (f == 0) continue;
-      for (int j = 0; j < n; j++) { A[(size_t)r * n + j] -= f * A[(size_t)c * n + j]; inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j]; }
-    }
-  }
-  lv.ainv.upload(inv.data(), inv.size(), st);
-  check(hipStreamSynchronize(st), "sync");
+  DevBuf<double> fac;
+  fac.alloc(n);
+  lv.ainv.alloc((size_t)n * 2 * n);
+  dense_inverse(lv.D, lv.v64.val.p, lv.ainv.p, fac.p, Y.ctx->st);
+  check(hipStreamSynchronize(Y.ctx->st), "coarse inverse");
   lv.ainv_n = n;
 }
 
@@ -282,7 +246,7 @@ static void mg_cycle(System &Y, int i) {
   if (i == 0) {
     const int cs = (int)o[9];
     if (cs == 17 && lv.ainv_n > 0) {
-      gemv<T>(lv.ainv_n, lv.ainv.p, v.f.p, v.x.p, st);
+      gemv<T>(lv.ainv_n, 2 * (int64_t)lv.ainv_n, lv.ainv.p + lv.ainv_n, v.f.p, v.x.p, st);
     } else {
       level_smooth<T>(Y, lv, cs == 17 ? 3 : cs, o[8], std::max(1, (int)o[10]));
     }

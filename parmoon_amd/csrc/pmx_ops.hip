@@ -360,16 +360,87 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int *__restric
 // ============================================================ dense coarse solve
 // y = Ainv x with the explicit inverse of the coarsest matrix (FP64), one wavefront per row
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gemv(int n, const double *__restrict__ Ainv, const T *__restrict__ x,
-                                                 T *__restrict__ y) {
+__global__ __launch_bounds__(kBlock) void k_gemv(int n, int64_t ld, const double *__restrict__ Ainv,
+                                                 const T *__restrict__ x, T *__restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
   if (row >= n) return;
   double s = 0;
-  const double *a = Ainv + (int64_t)row * n;
+  const double *a = Ainv + (int64_t)row * ld;
   for (int j = lane; j < n; j += 64) s += a[j] * (double)x[j];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   if (lane == 0) y[row] = (T)s;
+}
+
+// ============================================================ coarse inverse
+// Dense operator of a level in lattice order, row-major into the left half of
+// the augmented [A | I] (leading dimension 2n).
+__global__ void k_css_to_dense(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
+                               const double *__restrict__ val, double *__restrict__ aug, int64_t ld) {
+  const BlockInfo bi = blocks[blockIdx.x];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
+  const int x = s.x0 + s.stride * i, y = s.y0 + s.stride * j, z = s.z0 + s.stride * k;
+  const int p = x + M * (y + M * z);
+  int slot = 0;
+  for (int dz = -s.ez; dz <= s.ez; dz++)
+    for (int dy = -s.ey; dy <= s.ey; dy++)
+      for (int dx = -s.ex; dx <= s.ex; dx++, slot++)
+        aug[(int64_t)p * ld + (x + dx) + M * ((y + dy) + M * (z + dz))] = val[s.voff + (int64_t)slot * s.count + idx];
+}
+__global__ void k_dense_identity(const int *__restrict__ bnd, int nbd, int n, double *__restrict__ aug) {
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id < nbd) aug[(int64_t)bnd[id] * 2 * n + bnd[id]] = 1.0;   // Dirichlet row of A
+  if (id < n) aug[id * 2 * n + n + id] = 1.0;                    // right half: I
+}
+// Gauss-Jordan step c with partial pivoting on [A | I] (n x 2n, row-major):
+// (1) one block finds the pivot row and swaps it into row c
+__global__ __launch_bounds__(1024) void k_gj_pivot(double *__restrict__ aug, int n, int c, double *__restrict__ fac) {
+  __shared__ double bv[1024];
+  __shared__ int bi[1024];
+  const int64_t ld = 2 * (int64_t)n;
+  double best = -1;
+  int arg = c;
+  for (int r = c + threadIdx.x; r < n; r += 1024) {
+    const double a = fabs(aug[r * ld + c]);
+    if (a > best) { best = a; arg = r; }
+  }
+  bv[threadIdx.x] = best;
+  bi[threadIdx.x] = arg;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double b2 = bv[threadIdx.x + o];
+      const int i2 = bi[threadIdx.x + o];
+      if (b2 > bv[threadIdx.x] || (b2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) { bv[threadIdx.x] = b2; bi[threadIdx.x] = i2; }
+    }
+    __syncthreads();
+  }
+  const int p = bi[0];
+  if (p != c)
+    for (int j = threadIdx.x; j < ld; j += 1024) {
+      const double t = aug[(int64_t)c * ld + j];
+      aug[(int64_t)c * ld + j] = aug[(int64_t)p * ld + j];
+      aug[(int64_t)p * ld + j] = t;
+    }
+  __syncthreads();
+  // scale the pivot row, then save the column of multipliers
+  const double inv = 1.0 / aug[(int64_t)c * ld + c];
+  __syncthreads();
+  for (int j = threadIdx.x; j < ld; j += 1024) aug[(int64_t)c * ld + j] *= inv;
+  for (int r = threadIdx.x; r < n; r += 1024) fac[r] = (r == c) ? 0.0 : aug[(int64_t)r * ld + c];
+}
+// (2) eliminate column c from every other row; the pivot row is read-only here
+__global__ __launch_bounds__(256) void k_gj_eliminate(double *__restrict__ aug, int n, int c,
+                                                       const double *__restrict__ fac) {
+  const int64_t ld = 2 * (int64_t)n;
+  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (id >= (int64_t)n * ld) return;
+  const int r = id / ld, j = id % ld;
+  const double f = fac[r];
+  if (r != c && f != 0.0) aug[id] -= f * aug[(int64_t)c * ld + j];
 }
 
 // ============================================================ FGMRES scalars
@@ -498,9 +569,23 @@ void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipSt
   hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M, F.order);
 }
 template <typename T>
-void gemv(int n, const double *Ainv, const T *x, T *y, hipStream_t st) {
+void gemv(int n, int64_t ld, const double *Ainv, const T *x, T *y, hipStream_t st) {
   const int g = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_gemv<T>, dim3(g), dim3(kBlock), 0, st, n, Ainv, x, y);
+  hipLaunchKernelGGL(k_gemv<T>, dim3(g), dim3(kBlock), 0, st, n, ld, Ainv, x, y);
+}
+// explicit inverse of a level's operator by Gauss-Jordan on the device;
+// aug: n x 2n workspace, the inverse is its right half (leading dimension 2n)
+void dense_inverse(const DevLayout &D, const double *val, double *aug, double *fac, hipStream_t st) {
+  const int n = (int)D.N;
+  const int64_t ld = 2 * (int64_t)n;
+  check(hipMemsetAsync(aug, 0, sizeof(double) * n * ld, st), "memset aug");
+  hipLaunchKernelGGL(k_css_to_dense, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, val, aug, ld);
+  hipLaunchKernelGGL(k_dense_identity, dim3((n + 255) / 256), dim3(256), 0, st, D.boundary, D.nboundary, n, aug);
+  const unsigned g = (unsigned)((n * ld + 255) / 256);
+  for (int c = 0; c < n; c++) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, st, aug, n, c, fac);
+    hipLaunchKernelGGL(k_gj_eliminate, dim3(g), dim3(256), 0, st, aug, n, c, fac);
+  }
 }
 template <typename T, typename U>
 void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st) {
@@ -561,7 +646,7 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
   template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
   template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t);            \
-  template void gemv<T>(int, const double *, const T *, T *, hipStream_t);                                  \
+  template void gemv<T>(int, int64_t, const double *, const T *, T *, hipStream_t);                         \
   template void csr_spmv<T>(int, const int *, const int *, const T *, const T *, T *, hipStream_t);         \
   template void css_to_csr<T>(const DevLayout &, const int *, const int *, const int *, int64_t, const T *, \
                               double *, hipStream_t);

@@ -45,7 +45,8 @@ template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T 
 template <typename T>
 void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st);
 template <typename T> void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st);
-template <typename T> void gemv(int n, const double *Ainv, const T *x, T *y, hipStream_t st);
+template <typename T> void gemv(int n, int64_t ld, const double *Ainv, const T *x, T *y, hipStream_t st);
+void dense_inverse(const DevLayout &D, const double *val, double *aug, double *fac, hipStream_t st);
 template <typename T, typename U> void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st);
 template <typename T>
 void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st);
