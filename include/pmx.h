@@ -115,6 +115,7 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother
  *  [16] stop when ||r|| <= [16] * ||r0|| (r0 = initial FP64 defect; 0 = off)
  *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
+ *  [18] 1 = launch the MG cycle kernel by kernel instead of replaying its HIP graph
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -92,12 +93,32 @@ struct System {
   DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
   std::vector<int> rec;
   double opt[24];
+  // captured MG cycles (finest f -> finest x), keyed by precision and cycle options
+  struct Graph {
+    bool valid = false;
+    int prec = 0;
+    double key[10] = {0};
+    hipGraphExec_t exec = nullptr;
+  };
+  Graph graphs[2];
+  bool use_graphs = true;
+  ~System() {
+    for (auto &g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  }
+  void invalidate_graphs() {
+    for (auto &g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+      g.valid = false;
+    }
+  }
 };
 
 // ------------------------------------------------------------- set-up
 static void setup_level(System &Y, Level &lv, int nref) {
   hipStream_t st = Y.ctx->st;
-  build_layout(lv.L, nref, Y.order, kBlock);
+  build_layout(lv.L, nref, Y.order, kRowsPerBlock);
   build_numbering(lv.Nb, lv.L);
   const Layout &L = lv.L;
   DevLayout &D = lv.D;
@@ -132,6 +153,7 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
   D.blocks_colour = lv.blocks_colour.p;
+
   lv.rhs.alloc(L.N);
   lv.v64.val.alloc(L.nvals);
   for (auto *b : {&lv.v64.x, &lv.v64.f, &lv.v64.d, &lv.v64.aux}) b->alloc(L.N);
@@ -211,6 +233,7 @@ static void ensure_val32(System &Y) {
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (lv.have_val32) continue;
+    Y.invalidate_graphs();
     lv.v32.val.alloc(lv.L.nvals);
     for (auto *b : {&lv.v32.x, &lv.v32.f, &lv.v32.d, &lv.v32.aux}) b->alloc(lv.L.N);
     convert<float, double>(lv.v32.val.p, lv.v64.val.p, lv.L.nvals, Y.ctx->st);
@@ -225,7 +248,7 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
   Vecs<T> &v = vecs<T>(lv);
   for (int s = 0; s < sweeps; s++) {
     if (kind == 1) {
-      check(hipMemcpyAsync(v.aux.p, v.x.p, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+      copy<T>(v.aux.p, v.x.p, lv.L.N, st);
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
       css_apply<T>(lv.D, 2, v.val.p, v.aux.p, v.f.p, v.x.p, omega, -1, st);
     } else {
@@ -268,21 +291,52 @@ static void mg_cycle(System &Y, int i) {
   axpy<T>(v.x.p, v.aux.p, nullptr, 1.0, o[13], lv.L.N, st);
   level_smooth<T>(Y, lv, kind, o[8], (int)o[7]);
 }
-// z = MG(v): TMultiGridScaIte::Iterate with zero start (MultiGridScaIte.C:89-130)
+// one MG cycle on the finest level's f -> x (zero start, SetDirichletNodes)
 template <typename T>
-static void mg_prec(System &Y, T *z, const T *vin) {
+static void mg_cycle_from_zero(System &Y) {
   hipStream_t st = Y.ctx->st;
   const int L = (int)Y.lev.size() - 1;
   Level &lv = *Y.lev[L];
   Vecs<T> &v = vecs<T>(lv);
-  check(hipMemcpyAsync(v.f.p, vin, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
   fill<T>(v.x.p, (T)0, lv.L.N, st);
   boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);  // SetDirichletNodes
   const int gam = (int)Y.opt[12];
   Y.rec.assign(Y.lev.size() + 1, 1);
   for (int k = 2; k <= L; k++) Y.rec[k] = gam > 0 ? gam : 2;
   mg_cycle<T>(Y, L);
-  check(hipMemcpyAsync(z, v.x.p, sizeof(T) * lv.L.N, hipMemcpyDeviceToDevice, st), "copy");
+}
+// z = MG(v): TMultiGridScaIte::Iterate with zero start (MultiGridScaIte.C:89-130).
+// The cycle (about a thousand kernels for 5 Q2 levels) is captured once into a
+// HIP graph and replayed, so the GPU is not left waiting on launches.
+template <typename T>
+static void mg_prec(System &Y, T *z, const T *vin) {
+  hipStream_t st = Y.ctx->st;
+  Level &lv = *Y.lev.back();
+  Vecs<T> &v = vecs<T>(lv);
+  copy<T>(v.f.p, vin, lv.L.N, st);
+  const int pi = std::is_same<T, double>::value ? 0 : 1;
+  System::Graph &g = Y.graphs[pi];
+  double key[10];
+  for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
+  key[9] = 0;
+  if (Y.use_graphs) {
+    if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+      hipGraph_t graph;
+      check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
+      mg_cycle_from_zero<T>(Y);
+      check(hipStreamEndCapture(st, &graph), "end capture");
+      check(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(graph);
+      std::memcpy(g.key, key, sizeof(key));
+      g.valid = true;
+    }
+    check(hipGraphLaunch(g.exec, st), "graph launch");
+  } else {
+    mg_cycle_from_zero<T>(Y);
+  }
+  copy<T>(z, v.x.p, lv.L.N, st);
 }
 
 // y = A x on the finest level (stencil + Dirichlet identity rows)
@@ -491,6 +545,9 @@ static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double re
 // Solve A sol = b on the finest level; sol/b are FP64 lattice vectors on the device.
 static int solve_device(System &Y, double *info) {
   hipStream_t st = Y.ctx->st;
+  // PMX_EAGER=1 launches kernel by kernel (profilers that cannot follow graph capture)
+  const char *eager = std::getenv("PMX_EAGER");
+  Y.use_graphs = Y.opt[18] == 0 && !(eager && eager[0] == '1');
   Level &lv = *Y.lev.back();
   const int64_t N = lv.L.N;
   const int prec = (int)Y.opt[15];
@@ -771,6 +828,7 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
     }
     check(hipStreamSynchronize(st), "assemble");
     coarse_inverse(Y, *Y.lev.front());
+    Y.invalidate_graphs();
     Y.assembled = true;
   })
 }

@@ -12,9 +12,11 @@
 // Q2).  A class has one smoother colour (coordinates mod 2 resp. mod 3) and
 // one stencil shape (Q2: extent 2 along an even coordinate, 1 along an odd
 // one), so a class is a strided sub-lattice whose rows all have the same
-// column offsets.  Values are slot-major inside a class:
-//   val[seg.voff + slot*seg.count + idx]
-// so consecutive threads (consecutive idx) read consecutive addresses and no
+// column offsets.  Inside a class the rows are cut into slices of 16 and a
+// slice stores its values slot-major:
+//   val[seg.voff + (idx/16)*(16*ns) + slot*16 + idx%16]
+// so the wavefront that owns 16 rows with 4 lanes per row reads 4
+// consecutive slots of 16 rows = 256 contiguous bytes (FP32) per load, and no
 // column indices are stored.  Dirichlet rows (lattice boundary) are identity
 // and not stored.
 #pragma once
@@ -25,6 +27,9 @@
 namespace pmx {
 
 constexpr int kBlock = 256;       // threads per workgroup of the operator kernels
+constexpr int kRowLanes = 4;      // lanes per matrix row in the row kernels
+constexpr int kRowsPerBlock = kBlock / kRowLanes;
+constexpr int kSlice = 16;        // rows per value slice (see Layout)
 constexpr int kRedBlocks = 1024;  // partial sums of the deterministic reductions
 
 struct Segment {

@@ -24,6 +24,11 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
   return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + k;
 }
 
+// first value of row idx of a class (slot stride kSlice)
+__device__ __forceinline__ int64_t vbase(const DevSegment &s, int idx) {
+  return s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
+}
+
 // ---------------------------------------------------------------- BLAS-1
 template <typename T>
 __global__ void k_fill(T *__restrict__ x, T v, int64_t n) {
@@ -43,6 +48,11 @@ __global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, const double 
   const T at = (T)a;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     y[i] += at * x[i];
+}
+template <typename T>
+__global__ void k_copy(T *__restrict__ y, const T *__restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
 }
 // y += (T) x  (precision-changing update)
 template <typename T, typename U>

@@ -80,7 +80,7 @@ void build_layout(Layout &L, int nref, int order, int block) {
           s.ns = (2 * s.ex + 1) * (2 * s.ey + 1) * (2 * s.ez + 1);
           s.dslot = s.ex + (2 * s.ex + 1) * (s.ey + (2 * s.ey + 1) * s.ez);
           s.voff = L.nvals;
-          L.nvals += s.count * s.ns;
+          L.nvals += ((s.count + kSlice - 1) / kSlice) * kSlice * s.ns;
           L.segs.push_back(s);
         }
   L.blocks_all.clear();

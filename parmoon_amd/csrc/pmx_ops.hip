@@ -21,64 +21,71 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
 }
 
 // ============================================================ stencil rows
-// MODE 0: y = A x                          (MatVect, Blas2.C:86-118)
-// MODE 1: y = f - A x                      (ScalarDefect, Blas2.C:37-83)
-// MODE 2: y = (1-w) x + w (f - A_off x)/d  (TMGLevel3D::Jacobi, MGLevel3D.C:469-563), x = old iterate
-// MODE 3: x = w (s/d - x) + x in place     (TMGLevel3D::SOR, MGLevel3D.C:280-361) for one colour
+// One matrix row is handled by R = kRowLanes lanes: lane `part` accumulates
+// slots part, part+R, ... and a shuffle reduction combines them.  Modes:
+// 0: y = A x                          (MatVect, Blas2.C:86-118)
+// 1: y = f - A x                      (ScalarDefect, Blas2.C:37-83)
+// 2: y = (1-w) x + w (f - A_off x)/d  (TMGLevel3D::Jacobi, MGLevel3D.C:469-563), x = old iterate
+// 3: x = w ((f - A_off x)/d - x) + x  (TMGLevel3D::SOR, MGLevel3D.C:280-361) in place, one colour
 template <typename T> struct Acc { using type = T; };
 
-template <typename T, int EX, int EY, int EZ, int MODE>
-__device__ __forceinline__ void css_row(const DevSegment &s, int idx, int M, const T *__restrict__ val,
-                                        const T *x, const T *__restrict__ f, T *y, double omega) {
+template <typename T, int EX, int EY, int EZ, int R, int MODE>
+__device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part, int M,
+                                          const T *__restrict__ val, const T *x, const T *__restrict__ f, T *y,
+                                          double omega) {
   using A = typename Acc<T>::type;
+  constexpr int WX = 2 * EX + 1, WY = 2 * EY + 1, WZ = 2 * EZ + 1, NS = WX * WY * WZ;
+  constexpr int DS = EX + WX * (EY + WY * EZ);
   const int i = idx % s.cx;
   const int t = idx / s.cx;
   const int j = t % s.cy, k = t / s.cy;
   const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
-  const T *v = val + s.voff + idx;
-  const int64_t cnt = s.count;
-  A acc = 0, diag = 1;
-  int slot = 0;
+  const T *v = val + vbase(s, idx);
+  A acc = 0;
 #pragma unroll
-  for (int dz = -EZ; dz <= EZ; dz++)
+  for (int m = 0; m < (NS + R - 1) / R; m++) {
+    const int sl = part + R * m;
+    if (sl < NS && (MODE < 2 || sl != DS)) {
+      const int dx = sl % WX - EX, dy = (sl / WX) % WY - EY, dz = sl / (WX * WY) - EZ;
+      acc += (A)v[sl * kSlice] * (A)x[p + dx + M * (dy + M * dz)];
+    }
+  }
 #pragma unroll
-    for (int dy = -EY; dy <= EY; dy++)
-#pragma unroll
-      for (int dx = -EX; dx <= EX; dx++, slot++) {
-        const A a = (A)v[slot * cnt];
-        if (MODE >= 2 && dx == 0 && dy == 0 && dz == 0) diag = a;
-        else acc += a * (A)x[p + dx + M * (dy + M * dz)];
-      }
+  for (int o = R / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, R);
+  if (part != 0) return;
   if (MODE == 0) y[p] = (T)acc;
   else if (MODE == 1) y[p] = (T)((A)f[p] - acc);
-  else if (MODE == 2) {
+  else {
+    const A diag = (A)v[DS * kSlice];
     const A told = (A)x[p];
-    y[p] = (T)((1 - (A)omega) * told + (A)omega * ((A)f[p] - acc) / diag);
-  } else {
-    const A told = (A)y[p];
-    y[p] = (T)((A)omega * (((A)f[p] - acc) / diag - told) + told);
+    if (MODE == 2) y[p] = (T)((1 - (A)omega) * told + (A)omega * ((A)f[p] - acc) / diag);
+    else y[p] = (T)((A)omega * (((A)f[p] - acc) / diag - told) + told);
   }
 }
 
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_css(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs,
-                                                int M, const T *__restrict__ val, const T *x, const T *__restrict__ f,
-                                                T *y, double omega) {
+__global__ __launch_bounds__(kBlock) void k_rows(const BlockInfo *__restrict__ blocks,
+                                                 const DevSegment *__restrict__ segs, int M,
+                                                 const T *__restrict__ val, const T *x, const T *__restrict__ f,
+                                                 T *y, double omega) {
+  constexpr int R = kRowLanes;
   const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
   const DevSegment s = segs[bi.seg];
-  const int idx = bi.base + threadIdx.x;
-  if (idx >= s.count) return;
-  const int code = s.ex * 100 + s.ey * 10 + s.ez;
-  switch (code) {
-    case 111: css_row<T, 1, 1, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 211: css_row<T, 2, 1, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 121: css_row<T, 1, 2, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 221: css_row<T, 2, 2, 1, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 112: css_row<T, 1, 1, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 212: css_row<T, 2, 1, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
-    case 122: css_row<T, 1, 2, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
-    default: css_row<T, 2, 2, 2, MODE>(s, idx, M, val, x, f, y, omega); break;
+  const int idx = bi.base + threadIdx.x / R;
+  const int part = threadIdx.x % R;
+  if (idx >= s.count) return;  // the R lanes of a row leave together
+#define PMX_ROW(EX, EY, EZ) row_split<T, EX, EY, EZ, R, MODE>(s, idx, part, M, val, x, f, y, omega)
+  switch (s.ex * 100 + s.ey * 10 + s.ez) {
+    case 111: PMX_ROW(1, 1, 1); break;
+    case 211: PMX_ROW(2, 1, 1); break;
+    case 121: PMX_ROW(1, 2, 1); break;
+    case 221: PMX_ROW(2, 2, 1); break;
+    case 112: PMX_ROW(1, 1, 2); break;
+    case 212: PMX_ROW(2, 1, 2); break;
+    case 122: PMX_ROW(1, 2, 2); break;
+    default: PMX_ROW(2, 2, 2); break;
   }
+#undef PMX_ROW
 }
 
 // ============================================================ transfers
@@ -214,9 +221,9 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   const int px = s.x0 + s.stride * i, py = s.y0 + s.stride * j, pz = s.z0 + s.stride * k;
   const int p = px + M * (py + M * pz);
   const double g = 2.0 * n;  // reference -> physical derivative factor 2/h
-  double *v = val + s.voff + idx;
-  double *v2 = val2 ? val2 + s.voff + idx : nullptr;
-  const int64_t cnt = s.count;
+  double *v = val + vbase(s, idx);
+  double *v2 = val2 ? val2 + vbase(s, idx) : nullptr;
+  constexpr int cnt = kSlice;
   double frhs = 0;
   // incident cells per direction: lattice coordinate x lies in cells
   // floor((x-1)/ORDER) .. floor(x/ORDER) (two cells at a vertex plane)
@@ -275,7 +282,7 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
 }
 
 template <int ORDER>
-__global__ __launch_bounds__(kBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
+__global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
                                                      const DevSegment *__restrict__ segs, int M, int n, int nq,
                                                      const double *__restrict__ prm, const double *__restrict__ qp,
                                                      const double *__restrict__ tab, double *__restrict__ val,
@@ -331,7 +338,7 @@ __global__ void k_css_to_csr(const int *__restrict__ rowptr, const int *__restri
     const int q = dof2lat[kcol[n]];
     const int dx = q % M - x, dy = (q / M) % M - y, dz = q / (M * M) - z;
     const int slot = (dx + s.ex) + (2 * s.ex + 1) * ((dy + s.ey) + (2 * s.ey + 1) * (dz + s.ez));
-    out[n] = (double)val[s.voff + (int64_t)slot * s.count + idx];
+    out[n] = (double)val[vbase(s, idx) + (int64_t)slot * kSlice];
   }
 }
 
@@ -388,7 +395,7 @@ __global__ void k_css_to_dense(const BlockInfo *__restrict__ blocks, const DevSe
   for (int dz = -s.ez; dz <= s.ez; dz++)
     for (int dy = -s.ey; dy <= s.ey; dy++)
       for (int dx = -s.ex; dx <= s.ex; dx++, slot++)
-        aug[(int64_t)p * ld + (x + dx) + M * ((y + dy) + M * (z + dz))] = val[s.voff + (int64_t)slot * s.count + idx];
+        aug[(int64_t)p * ld + (x + dx) + M * ((y + dy) + M * (z + dz))] = val[vbase(s, idx) + (int64_t)slot * kSlice];
 }
 __global__ void k_dense_identity(const int *__restrict__ bnd, int nbd, int n, double *__restrict__ aug) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -513,6 +520,10 @@ void fill(T *x, T v, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, x, v, n);
 }
 template <typename T>
+void copy(T *y, const T *x, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, n);
+}
+template <typename T>
 void axpy(T *y, const T *x, const double *alpha_dev, double sgn, double ac, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_axpy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, alpha_dev, sgn, ac, n);
 }
@@ -544,10 +555,10 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
   const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_css<T, 0>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    case 1: hipLaunchKernelGGL((k_css<T, 1>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    case 2: hipLaunchKernelGGL((k_css<T, 2>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    default: hipLaunchKernelGGL((k_css<T, 3>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    case 0: hipLaunchKernelGGL((k_rows<T, 0>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    case 1: hipLaunchKernelGGL((k_rows<T, 1>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    case 2: hipLaunchKernelGGL((k_rows<T, 2>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+    default: hipLaunchKernelGGL((k_rows<T, 3>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
   }
 }
 template <typename T>
@@ -579,7 +590,8 @@ void dense_inverse(const DevLayout &D, const double *val, double *aug, double *f
   const int n = (int)D.N;
   const int64_t ld = 2 * (int64_t)n;
   check(hipMemsetAsync(aug, 0, sizeof(double) * n * ld, st), "memset aug");
-  hipLaunchKernelGGL(k_css_to_dense, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, val, aug, ld);
+  hipLaunchKernelGGL(k_css_to_dense, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, val,
+                     aug, ld);
   hipLaunchKernelGGL(k_dense_identity, dim3((n + 255) / 256), dim3(256), 0, st, D.boundary, D.nboundary, n, aug);
   const unsigned g = (unsigned)((n * ld + 255) / 256);
   for (int c = 0; c < n; c++) {
@@ -612,11 +624,11 @@ void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   if (D.order == 1)
-    hipLaunchKernelGGL(k_assemble<1>, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, D.n, nq,
-                       prm_dev, qpbuf, tab, val, val2, rhs);
+    hipLaunchKernelGGL(k_assemble<1>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, D.n,
+                       nq, prm_dev, qpbuf, tab, val, val2, rhs);
   else
-    hipLaunchKernelGGL(k_assemble<2>, dim3(D.nblocks_all), dim3(kBlock), 0, st, D.blocks_all, D.segs, D.M, D.n, nq,
-                       prm_dev, qpbuf, tab, val, val2, rhs);
+    hipLaunchKernelGGL(k_assemble<2>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, D.n,
+                       nq, prm_dev, qpbuf, tab, val, val2, rhs);
   hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
                      D.nboundary, D.M, prm_dev, rhs);
 }
@@ -638,6 +650,7 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
 // explicit instantiations
 #define PMX_INST(T)                                                                                         \
   template void fill<T>(T *, T, int64_t, hipStream_t);                                                      \
+  template void copy<T>(T *, const T *, int64_t, hipStream_t);                                              \
   template void axpy<T>(T *, const T *, const double *, double, double, int64_t, hipStream_t);              \
   template void scale_inv<T>(T *, const T *, const double *, int64_t, hipStream_t);                         \
   template void dot<T>(const T *, const T *, int64_t, double *, double *, int, hipStream_t);                \

@@ -28,6 +28,7 @@ struct DevLayout {
 void check(hipError_t e, const char *what);
 
 template <typename T> void fill(T *x, T v, int64_t n, hipStream_t st);
+template <typename T> void copy(T *y, const T *x, int64_t n, hipStream_t st);
 template <typename T>
 void axpy(T *y, const T *x, const double *alpha_dev, double sgn, double ac, int64_t n, hipStream_t st);
 template <typename T, typename U> void convert(T *y, const U *x, int64_t n, hipStream_t st);

@@ -204,3 +204,23 @@ def test_other_krylov(q1sys, solver):
     assert rel(s, ref) < 1e-9
     o, oinfo = O.solve(opt, rhs, sol0)
     assert rel(o, ref) < 1e-9
+
+
+def test_graph_replays_match_eager(q2sys):
+    """The MG cycle is replayed from a captured HIP graph: several replays,
+    with eager work and synchronisations in between, give the eager result."""
+    S, O, prm = q2sys
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    sol0 = np.zeros_like(rhs)
+    sol0[fine.active:] = rhs[fine.active:]
+    for precision in (0, 1):
+        base = dict(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=0.0, rel_target=1e-10,
+                    smoother=3, omega=1.0, precision=precision)
+        eager, _, rc0 = S.solve(P.solver_opt(eager=1, **base), rhs, sol0)
+        assert rc0 == 0
+        for rep in range(3):
+            g, _, rc = S.solve(P.solver_opt(**base), rhs, sol0)
+            assert rc == 0
+            assert np.array_equal(g, eager), (precision, rep)
+            S.matvec(g)  # eager work between replays
