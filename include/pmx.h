@@ -130,8 +130,35 @@ int pmx_system_solution(pmx_system *sys, double *sol);
  * Times `reps` back-to-back launches of one kernel on the context stream with
  * HIP events: 0 stencil SpMV, 1 defect, 2 Jacobi sweep, 3 SSOR sweep,
  * 4 assembly (all kernels of one level), 5 prolongation, 6 restriction,
- * 7 one MG V-cycle.  precision 0 FP64, 1 FP32.  out: ms per launch. */
+ * 7 one MG V-cycle, 8 one learned post-smoothing step (FP32), 9 the network's
+ * forward convolutions alone on the level's lattice (FP32/BF16).
+ * precision 0 FP64, 1 FP32.  out: ms per launch. */
 int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms);
+
+/* ---- learned post-smoother (no counterpart in the reference; it replaces
+ * the post-sweeps of TMGLevel3D::SOR/SSOR, src/FE/MGLevel3D.C:280-467, on the
+ * finest levels of FP32 cycles).  Network: 4 conv layers 3x3x3 with
+ * 1 -> 16 -> 16 -> 16 -> 1 channels and GELU, BF16 operands with FP32
+ * accumulation.  Parameters are FP32, per layer W[cout][27][cin] then b[cout],
+ * tap t = (dx+1) + 3 (dy+1) + 9 (dz+1).  Lattice arrays are x-fastest M^3. */
+int pmx_learned_nparams(int *n);
+int pmx_learned_init(pmx_system *sys, unsigned seed);
+/* set != 0: upload params; set == 0: download them */
+int pmx_learned_params(pmx_system *sys, int set, float *params);
+/* network output on an M^3 lattice */
+int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out);
+/* loss = sum over interior nodes (out - target)^2 / (M-2)^3 and its gradient
+ * with respect to the parameters (fresh, not accumulated) */
+int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *target, float *grads,
+                         double *loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
+int pmx_learned_adam(pmx_system *sys, double lr);
+/* train on pairs from plain cycles (opt: the cycle options of pmx_system_solve).
+ * topt: [0] samples, [1] steps, [2] learning rate, [3] seed, [4] finest levels
+ * that use the network.  tinfo: [0] first / [1] last mean loss, [2] s, [3] steps */
+int pmx_learned_train(pmx_system *sys, const double *opt, const double *topt, double *tinfo);
+/* one learned post-smoothing step on a level (reference numbering, sol in place) */
+int pmx_learned_post(pmx_system *sys, int level, double *sol, const double *f);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,14 @@ def lib():
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
         L.pmx_system_solution.argtypes = [vp, vp]
         L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
+        L.pmx_learned_nparams.argtypes = [vp]
+        L.pmx_learned_init.argtypes = [vp, ctypes.c_uint]
+        L.pmx_learned_params.argtypes = [vp, ip, vp]
+        L.pmx_learned_forward.argtypes = [vp, ip, vp, vp]
+        L.pmx_learned_gradient.argtypes = [vp, ip, vp, vp, vp, vp]
+        L.pmx_learned_adam.argtypes = [vp, dp]
+        L.pmx_learned_train.argtypes = [vp, vp, vp, vp]
+        L.pmx_learned_post.argtypes = [vp, ip, vp, vp]
         _lib = L
     return _lib
 
@@ -231,3 +239,56 @@ class System:
         ms = np.zeros(1)
         _ok(lib().pmx_time_kernel(self.h, kernel, level, precision, reps, _p(ms)), "pmx_time_kernel")
         return float(ms[0])
+
+    # ---- learned post-smoother -------------------------------------------
+    @staticmethod
+    def learned_nparams():
+        n = np.zeros(1, np.int32)
+        _ok(lib().pmx_learned_nparams(_p(n)), "pmx_learned_nparams")
+        return int(n[0])
+
+    def learned_init(self, seed=0):
+        _ok(lib().pmx_learned_init(self.h, seed), "pmx_learned_init")
+
+    def learned_params(self, params=None):
+        """Upload `params` (FP32) if given; return the current parameters."""
+        n = self.learned_nparams()
+        if params is not None:
+            p = np.ascontiguousarray(params, np.float32)
+            if p.size != n:
+                raise ValueError(f"expected {n} parameters")
+            _ok(lib().pmx_learned_params(self.h, 1, _p(p)), "pmx_learned_params")
+        out = np.zeros(n, np.float32)
+        _ok(lib().pmx_learned_params(self.h, 0, _p(out)), "pmx_learned_params")
+        return out
+
+    def learned_forward(self, u):
+        """Network output for an (M, M, M) array indexed [z, y, x]."""
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros_like(u)
+        _ok(lib().pmx_learned_forward(self.h, u.shape[0], _p(u), _p(out)), "pmx_learned_forward")
+        return out
+
+    def learned_gradient(self, u, t):
+        u = np.ascontiguousarray(u, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(self.learned_nparams(), np.float32)
+        loss = np.zeros(1)
+        _ok(lib().pmx_learned_gradient(self.h, u.shape[0], _p(u), _p(t), _p(g), _p(loss)), "pmx_learned_gradient")
+        return g, float(loss[0])
+
+    def learned_adam(self, lr):
+        _ok(lib().pmx_learned_adam(self.h, lr), "pmx_learned_adam")
+
+    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=1):
+        o = np.zeros(24)
+        o[:len(opt)] = opt
+        t = np.array([samples, steps, lr, seed, levels], float)
+        info = np.zeros(4)
+        _ok(lib().pmx_learned_train(self.h, _p(o), _p(t), _p(info)), "pmx_learned_train")
+        return info
+
+    def learned_post(self, sol, f, level=-1):
+        s = np.array(sol, float)
+        _ok(lib().pmx_learned_post(self.h, level, _p(s), _p(np.ascontiguousarray(f, float))), "pmx_learned_post")
+        return s

@@ -13,12 +13,15 @@
 
 #include "../../include/pmx.h"
 #include "pmx_internal.hpp"
+#include "pmx_learned.hpp"
 #include "pmx_ops.hpp"
 
 namespace pmx {
 
 static thread_local std::string g_err;
 // V(2,2) SSOR with the exact coarse solve, used when a single kernel is timed
+// FGMRES residual ratio above which a learned cycle counts as failed
+static const double kLearnStall = 0.99;
 static const double PMX_DEFAULT_VCYCLE[24] = {0, 1, 1, 0, 0, 3, 2, 2, 1.0, 17, 1, 0, 1, 1.0, 0, 0};
 
 template <typename T>
@@ -76,6 +79,9 @@ struct Level {
   bool have_val32 = false;
   DevBuf<double> ainv;                // coarsest level: explicit inverse
   int ainv_n = 0;
+  // learned post-smoother: 1/diag, correction and its image under A (FP32)
+  DevBuf<float> dinv, lc, lac;
+  bool have_dinv = false;
 };
 
 struct System {
@@ -102,7 +108,14 @@ struct System {
   };
   Graph graphs[2];
   bool use_graphs = true;
+  // learned post-smoother on the `learned_levels` finest levels (FP32 cycles)
+  LearnedNet net;
+  int learned_levels = 1;
+  bool learn_active = false;
+  int fallbacks = 0;
+  DevBuf<double> lscal;               // norm, (d, Ac), (Ac, Ac)
   ~System() {
+    net.release();
     for (auto &g : graphs)
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
   }
@@ -241,6 +254,26 @@ static void ensure_val32(System &Y) {
   }
 }
 
+// buffers of the learned post-smoother on the levels that use it
+static void ensure_learned(System &Y) {
+  ensure_val32(Y);
+  const int nl = (int)Y.lev.size();
+  Y.lscal.alloc(8);
+  for (int i = std::max(0, nl - Y.learned_levels); i < nl; i++) {
+    Level &lv = *Y.lev[i];
+    if (!lv.dinv.p) {
+      lv.dinv.alloc(lv.L.N);
+      lv.lc.alloc(lv.L.N);
+      lv.lac.alloc(lv.L.N);
+    }
+    if (!lv.have_dinv) {
+      learned_diag_inv(lv.D, lv.v32.val.p, lv.dinv.p, Y.ctx->st);
+      lv.have_dinv = true;
+    }
+    if (net_reserve(Y.net, lv.L.N, false)) Y.invalidate_graphs();
+  }
+}
+
 // ------------------------------------------------------------- multigrid
 template <typename T>
 static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
@@ -260,8 +293,35 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
     }
   }
 }
+// Learned post-smoothing of one level (replaces the post-sweeps of
+// TMGLevel3D; no counterpart in the reference): with d = f - A x and
+// u = D^-1 d, the network maps u / rms(u) to a correction c (rescaled by
+// rms(u)); x += alpha c with the minimal-residual step
+// alpha = max(0, (d, Ac) / (Ac, Ac)), so the level defect never grows in the
+// 2-norm (the step-length idea of TMGLevel3D::StepLengthControl,
+// MGLevel3D.C:614-863, applied to the network's correction).
+static void learned_post(System &Y, Level &lv) {
+  hipStream_t st = Y.ctx->st;
+  Vecs<float> &v = lv.v32;
+  const int64_t N = lv.L.N;
+  double *sc = Y.lscal.p, *part = Y.ctx->part.p;
+  const double sq = std::sqrt((double)lv.L.nactive);
+  css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+  boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
+  learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
+  dot<float>(lv.lc.p, lv.lc.p, N, part, sc, 1, st);
+  learned_scale_input(Y.net, lv.lc.p, sc, sq, N, st);
+  net_forward_act(Y.net, lv.L.M, false, st);
+  learned_output(Y.net, lv.L.M, sc, sq, lv.lc.p, st);
+  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
+  boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
+  dot<float>(v.d.p, lv.lac.p, N, part, sc + 1, 0, st);
+  dot<float>(lv.lac.p, lv.lac.p, N, part, sc + 2, 0, st);
+  learned_step(v.x.p, lv.lc.p, sc + 1, N, st);
+}
+
 template <typename T>
-static void mg_cycle(System &Y, int i) {
+static void mg_cycle(System &Y, int i, bool post = true) {
   hipStream_t st = Y.ctx->st;
   const double *o = Y.opt;
   Level &lv = *Y.lev[i];
@@ -289,7 +349,21 @@ static void mg_cycle(System &Y, int i) {
   prolongate<T>(cl.D, lv.D, cv.x.p, v.aux.p, st);
   boundary<T>(lv.D, 3, v.aux.p, nullptr, nullptr, st);  // CorrectNodes
   axpy<T>(v.x.p, v.aux.p, nullptr, 1.0, o[13], lv.L.N, st);
+  if (!post) return;
+  if constexpr (std::is_same<T, float>::value) {
+    if (Y.learn_active && i >= (int)Y.lev.size() - Y.learned_levels) {
+      learned_post(Y, lv);
+      return;
+    }
+  }
   level_smooth<T>(Y, lv, kind, o[8], (int)o[7]);
+}
+// TMultiGrid3D::SetRecursion (MultiGrid3D.C:697-725): gamma cycles per level
+static void set_recursion(System &Y) {
+  const int L = (int)Y.lev.size() - 1;
+  const int gam = (int)Y.opt[12];
+  Y.rec.assign(Y.lev.size() + 1, 1);
+  for (int k = 2; k <= L; k++) Y.rec[k] = gam > 0 ? gam : 2;
 }
 // one MG cycle on the finest level's f -> x (zero start, SetDirichletNodes)
 template <typename T>
@@ -300,9 +374,7 @@ static void mg_cycle_from_zero(System &Y) {
   Vecs<T> &v = vecs<T>(lv);
   fill<T>(v.x.p, (T)0, lv.L.N, st);
   boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);  // SetDirichletNodes
-  const int gam = (int)Y.opt[12];
-  Y.rec.assign(Y.lev.size() + 1, 1);
-  for (int k = 2; k <= L; k++) Y.rec[k] = gam > 0 ? gam : 2;
+  set_recursion(Y);
   mg_cycle<T>(Y, L);
 }
 // z = MG(v): TMultiGridScaIte::Iterate with zero start (MultiGridScaIte.C:89-130).
@@ -318,7 +390,7 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   System::Graph &g = Y.graphs[pi];
   double key[10];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
-  key[9] = 0;
+  key[9] = (pi == 1 && Y.learn_active) ? Y.learned_levels : 0;
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -402,8 +474,15 @@ static int fgmres(System &Y, T *x, const T *b, int m, int maxit, double rmin, do
       dot<T>(w, w, N, part, H + (size_t)i * ld + i + 1, 1, st);
       scale_inv<T>(V + (size_t)(i + 1) * N, w, H + (size_t)i * ld + i + 1, N, st);
       givens(H, cs, sn, s, i, ld, res, st);
+      const double r_prev = r;
       r = read_scalar(Y, res);
       if (!(r == r)) { *status = PMX_ERR_DIVERGED; *res_out = r; return j; }
+      if (Y.learn_active && r > kLearnStall * r_prev) {
+        // a cycle with the learned post-smoother failed to reduce the residual:
+        // plain post-smoothing from here on (the flexible method allows it)
+        Y.learn_active = false;
+        Y.fallbacks++;
+      }
       if (r < rmin || r < red * res0) {
         backsolve(H, s, yv, i, ld, st);
         multi_axpy<T, T>(x, Z, yv, i + 1, N, N, st);
@@ -542,6 +621,75 @@ static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double re
   return 1;
 }
 
+// Training of the learned post-smoother on (input, target) pairs from plain
+// cycles of the homogeneous problem (f = 0, exact solution 0): a random
+// interior start x0, then pre-smoothing and the coarse-grid correction of one
+// cycle on the finest level give x; the ideal post-correction is -x, the
+// network input the scaled D^-1 (f - A x).  Adam on the mean squared error.
+// topt: [0] samples, [1] steps, [2] learning rate, [3] seed, [4] levels using
+// the network.  tinfo: [0] mean loss of the first 10 steps, [1] of the last
+// 10, [2] seconds, [3] steps.
+static void learned_train(System &Y, const double *to, double *ti) {
+  hipStream_t st = Y.ctx->st;
+  const int ns = to[0] > 0 ? (int)to[0] : 16;
+  const int steps = to[1] > 0 ? (int)to[1] : 400;
+  const double lr = to[2] > 0 ? to[2] : 2e-3;
+  const uint32_t seed = (uint32_t)to[3];
+  Y.learned_levels = to[4] > 0 ? std::min((int)to[4], (int)Y.lev.size() - 1) : 1;
+  const int L = (int)Y.lev.size() - 1;
+  Level &lv = *Y.lev[L];
+  const int64_t N = lv.L.N;
+  const int M = lv.L.M;
+  const double sq = std::sqrt((double)lv.L.nactive);
+  Y.learn_active = false;
+  ensure_learned(Y);
+  if (!Y.net.ready) net_init(Y.net, seed ^ 0x5eedu, st);
+  if (net_reserve(Y.net, N, true)) Y.invalidate_graphs();
+  DevBuf<float> du, dt;
+  du.alloc((size_t)ns * N);
+  dt.alloc((size_t)ns * N);
+  DevBuf<float> losses;
+  losses.alloc(steps);
+  hipEvent_t e0, e1;
+  check(hipEventCreate(&e0), "event");
+  check(hipEventCreate(&e1), "event");
+  check(hipEventRecord(e0, st), "event");
+  Vecs<float> &v = lv.v32;
+  set_recursion(Y);
+  for (int k = 0; k < ns; k++) {
+    learned_random(v.x.p, M, seed * 7919u + (uint32_t)k, st);
+    fill<float>(v.f.p, 0.0f, N, st);
+    mg_cycle<float>(Y, L, false);
+    css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+    boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
+    learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
+    dot<float>(lv.lc.p, lv.lc.p, N, Y.ctx->part.p, Y.lscal.p, 1, st);
+    learned_target(lv.lc.p, M, Y.lscal.p, sq, 1.0f, du.p + (size_t)k * N, st);
+    learned_target(v.x.p, M, Y.lscal.p, sq, -1.0f, dt.p + (size_t)k * N, st);
+  }
+  for (int k = 0; k < steps; k++) {
+    const int smp = k % ns;
+    const double lrk = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * k / std::max(1, steps - 1))));
+    net_zero_grads(Y.net, st);
+    net_forward(Y.net, M, du.p + (size_t)smp * N, true, st);
+    net_backward(Y.net, M, dt.p + (size_t)smp * N, losses.p + k, st);
+    net_adam(Y.net, (float)lrk, st);
+  }
+  check(hipEventRecord(e1, st), "event");
+  check(hipEventSynchronize(e1), "event");
+  float ms = 0;
+  check(hipEventElapsedTime(&ms, e0, e1), "event");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::vector<float> h(steps);
+  check(hipMemcpy(h.data(), losses.p, sizeof(float) * steps, hipMemcpyDeviceToHost), "losses");
+  const int w = std::min(10, steps);
+  double a = 0, b = 0;
+  for (int k = 0; k < w; k++) { a += h[k]; b += h[steps - 1 - k]; }
+  if (ti) { ti[0] = a / w; ti[1] = b / w; ti[2] = ms * 1e-3; ti[3] = steps; }
+  Y.invalidate_graphs();
+}
+
 // Solve A sol = b on the finest level; sol/b are FP64 lattice vectors on the device.
 static int solve_device(System &Y, double *info) {
   hipStream_t st = Y.ctx->st;
@@ -572,8 +720,15 @@ static int solve_device(System &Y, double *info) {
     its = krylov<double>(Y, Y.sol.p, Y.b.p, maxit, rmin, Y.opt[4], &res, &res0, &status);
     outer = 1;
   } else {
-    // FP64 defect correction around an FP32 Krylov/MG solve
+    // FP64 defect correction around an FP32 Krylov/MG solve; precision 2
+    // replaces the post-smoother of the finest levels by the learned one
     ensure_val32(Y);
+    Y.fallbacks = 0;
+    if (prec == 2) {
+      if (!Y.net.ready) throw std::logic_error("learned smoother: train or load parameters first");
+      ensure_learned(Y);
+      Y.learn_active = true;
+    }
     Y.r64.alloc(N);
     DevBuf<float> r32, e32;
     r32.alloc(N);
@@ -596,6 +751,11 @@ static int solve_device(System &Y, double *info) {
       dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
       const double rn = read_scalar(Y, nr);
       outer++;
+      if (Y.learn_active && rn >= res) {
+        // the cycles with the learned post-smoother did not reduce the FP64 residual
+        Y.learn_active = false;
+        Y.fallbacks++;
+      }
       if (!(rn == rn) || rn > 1e10 * res0) { status = PMX_ERR_DIVERGED; res = rn; break; }
       res = rn;
       if (st_in == PMX_ERR_DIVERGED) { status = PMX_ERR_DIVERGED; break; }
@@ -609,8 +769,10 @@ static int solve_device(System &Y, double *info) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (info) {
-    info[0] = its; info[1] = res; info[2] = res0; info[3] = outer; info[4] = ms * 1e-3; info[5] = 0;
+    info[0] = its; info[1] = res; info[2] = res0; info[3] = outer; info[4] = ms * 1e-3;
+    info[5] = prec == 2 ? Y.fallbacks : 0;
   }
+  Y.learn_active = false;
   return status;
 }
 
@@ -825,6 +987,7 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
       assemble(lv.D, Y.prm.p, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr,
                lv.rhs.p, st);
       lv.have_val32 = false;
+      lv.have_dinv = false;
     }
     check(hipStreamSynchronize(st), "assemble");
     coarse_inverse(Y, *Y.lev.front());
@@ -1016,10 +1179,20 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 5: if (li > 0) prolongate<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.f.p, v.d.p, st); break;
           case 7: mg_prec<float>(Y, v.aux.p, v.d.p); break;
+          case 8: learned_post(Y, lv); break;
+          case 9: net_forward_act(Y.net, lv.L.M, false, st); break;
           default: break;
         }
       }
     };
+    if (kernel >= 8) {
+      if (!precision) throw std::invalid_argument("the learned smoother runs in FP32");
+      if (!Y.net.ready) throw std::logic_error("learned smoother: train or load parameters first");
+      const int keep = Y.learned_levels;
+      Y.learned_levels = (int)Y.lev.size() - li;
+      ensure_learned(Y);
+      Y.learned_levels = keep;
+    }
     one();
     check(hipEventRecord(e0, st), "event");
     for (int r = 0; r < reps; r++) one();
@@ -1030,6 +1203,115 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
     *ms = t / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+  })
+}
+
+int pmx_learned_nparams(int *n) {
+  PMX_TRY({
+    if (!n) throw std::invalid_argument("NULL argument");
+    *n = net_total_params();
+  })
+}
+int pmx_learned_init(pmx_system *sys, unsigned seed) {
+  PMX_TRY({
+    System &Y = sys->s;
+    net_init(Y.net, seed, Y.ctx->st);
+    check(hipStreamSynchronize(Y.ctx->st), "sync");
+    Y.invalidate_graphs();
+  })
+}
+int pmx_learned_params(pmx_system *sys, int set, float *params) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!params) throw std::invalid_argument("NULL argument");
+    hipStream_t st = Y.ctx->st;
+    const size_t bytes = sizeof(float) * net_total_params();
+    if (set) {
+      if (!Y.net.ready) net_init(Y.net, 0, st);
+      check(hipMemcpyAsync(Y.net.params, params, bytes, hipMemcpyHostToDevice, st), "h2d params");
+      net_sync_bf16(Y.net, st);
+      Y.invalidate_graphs();
+    } else {
+      if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+      check(hipMemcpyAsync(params, Y.net.params, bytes, hipMemcpyDeviceToHost, st), "d2h params");
+    }
+    check(hipStreamSynchronize(st), "sync");
+  })
+}
+int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!in || !out || M < 3 || M > 1025) throw std::invalid_argument("bad lattice or NULL argument");
+    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    hipStream_t st = Y.ctx->st;
+    const int64_t N = (int64_t)M * M * M;
+    if (net_reserve(Y.net, N, false)) Y.invalidate_graphs();
+    DevBuf<float> din;
+    din.upload(in, N, st);
+    net_forward(Y.net, M, din.p, false, st);
+    check(hipMemcpyAsync(out, Y.net.out, sizeof(float) * N, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipStreamSynchronize(st), "sync");
+  })
+}
+int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *target, float *grads,
+                         double *loss) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!in || !target || M < 3 || M > 1025) throw std::invalid_argument("bad lattice or NULL argument");
+    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    hipStream_t st = Y.ctx->st;
+    const int64_t N = (int64_t)M * M * M;
+    if (net_reserve(Y.net, N, true)) Y.invalidate_graphs();
+    DevBuf<float> din, dtg, dl;
+    din.upload(in, N, st);
+    dtg.upload(target, N, st);
+    dl.alloc(1);
+    net_zero_grads(Y.net, st);
+    net_forward(Y.net, M, din.p, true, st);
+    net_backward(Y.net, M, dtg.p, dl.p, st);
+    if (grads)
+      check(hipMemcpyAsync(grads, Y.net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
+            "d2h grads");
+    float l = 0;
+    check(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, st), "d2h loss");
+    check(hipStreamSynchronize(st), "sync");
+    if (loss) *loss = l;
+  })
+}
+int pmx_learned_adam(pmx_system *sys, double lr) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    net_adam(Y.net, (float)lr, Y.ctx->st);
+    check(hipStreamSynchronize(Y.ctx->st), "sync");
+    Y.invalidate_graphs();
+  })
+}
+int pmx_learned_train(pmx_system *sys, const double *opt, const double *topt, double *tinfo) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (!opt || !topt) throw std::invalid_argument("NULL argument");
+    if (Y.lev.size() < 2) throw std::invalid_argument("the learned smoother needs at least two levels");
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    learned_train(Y, topt, tinfo);
+  })
+}
+int pmx_learned_post(pmx_system *sys, int level, double *sol, const double *f) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    Level &lv = level_of(Y, level);
+    const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    const int keep = Y.learned_levels;
+    Y.learned_levels = (int)Y.lev.size() - li;
+    ensure_learned(Y);
+    Y.learned_levels = keep;
+    to_lattice<float>(Y, lv, sol, lv.v32.x.p);
+    to_lattice<float>(Y, lv, f, lv.v32.f.p);
+    learned_post(Y, lv);
+    from_lattice<float>(Y, lv, lv.v32.x.p, sol);
   })
 }
 

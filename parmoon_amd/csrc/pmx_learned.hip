@@ -1,0 +1,534 @@
+// pmx_learned.hip -- kernels of the learned post-smoother (see pmx_learned.hpp).
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16: a wavefront
+// owns 16 consecutive lattice nodes of one x-line (the 16 rows of the MFMA
+// A operand) and all 16 output channels (the columns of B); the K dimension
+// runs over (tap, input channel) pairs, 32 per instruction.  Activations are
+// stored node-major with the channels contiguous ([node][16] BF16, 32 bytes),
+// so the 8 consecutive K elements a lane supplies are one 16-byte load.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "pmx_learned.hpp"
+#include "pmx_ops.hpp"
+
+namespace pmx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+int net_total_params() {
+  int s = 0;
+  for (int l = 0; l < kNetLayers; l++) s += net_psize(l);
+  return s;
+}
+int net_param_offset(int l) {
+  int s = 0;
+  for (int k = 0; k < l; k++) s += net_psize(k);
+  return s;
+}
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                                           // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ float gelu(float v) { return 0.5f * v * (1.0f + erff(v * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float v) {
+  return 0.5f * (1.0f + erff(v * 0.70710678118654752f)) + v * 0.39894228040143268f * expf(-0.5f * v * v);
+}
+// tap t = (dx+1) + 3 (dy+1) + 9 (dz+1)
+__device__ __forceinline__ void tap_off(int t, int &dx, int &dy, int &dz) {
+  dx = t % 3 - 1;
+  dy = (t / 3) % 3 - 1;
+  dz = t / 9 - 1;
+}
+
+// ------------------------------------------------------------------ conv
+// MODE 0: hidden layer, out_bf = bf16(GELU(conv + b)), pre = conv + b (if set)
+// MODE 1: output layer (COUT = 1), out_f = conv + b
+// MODE 2: input-gradient convolution, out_bf = bf16(conv), no bias
+template <int CIN, int COUT, int MODE>
+__global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
+                                              const float *__restrict__ bias, float *__restrict__ pre,
+                                              uint16_t *__restrict__ out_bf, float *__restrict__ out_f) {
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int ntx = (M + 15) / 16;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + wave;
+  if (tile >= (int64_t)ntx * M * M) return;  // whole wavefront
+  const int tx = (int)(tile % ntx);
+  const int64_t r = tile / ntx;
+  const int y = (int)(r % M), z = (int)(r / M);
+  const int x0 = tx * 16;
+  const int row = l & 15, g = l >> 4;
+  const int xr = x0 + row;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  if (CIN == 1) {
+    u16x8 a, b;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int t = 8 * g + j;
+      uint16_t av = 0, bv = 0;
+      if (t < kTaps) {
+        int dx, dy, dz;
+        tap_off(t, dx, dy, dz);
+        const int xx = xr + dx, yy = y + dy, zz = z + dz;
+        if (xr < M && xx >= 0 && xx < M && yy >= 0 && yy < M && zz >= 0 && zz < M)
+          av = in[xx + (int64_t)M * (yy + (int64_t)M * zz)];
+        bv = w[row * kTaps + t];
+      }
+      a[j] = av;
+      b[j] = bv;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                  0, 0, 0);
+  } else {
+    const int cb = 8 * (g & 1);
+#pragma unroll 2
+    for (int c = 0; c < 14; c++) {
+      const int t = 2 * c + (g >> 1);
+      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0}, b = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (t < kTaps) {
+        int dx, dy, dz;
+        tap_off(t, dx, dy, dz);
+        const int xx = xr + dx, yy = y + dy, zz = z + dz;
+        if (xr < M && xx >= 0 && xx < M && yy >= 0 && yy < M && zz >= 0 && zz < M)
+          a = *reinterpret_cast<const u16x8 *>(in + (xx + (int64_t)M * (yy + (int64_t)M * zz)) * CIN + cb);
+        b = *reinterpret_cast<const u16x8 *>(w + ((int64_t)row * kTaps + t) * CIN + cb);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                    acc, 0, 0, 0);
+    }
+  }
+  // C: row 4g+i (node x0+4g+i), column l&15 (output channel)
+  const int co = l & 15;
+  if (co >= COUT) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int xn = x0 + 4 * g + i;
+    if (xn >= M) continue;
+    const int64_t node = xn + (int64_t)M * (y + (int64_t)M * z);
+    float v = acc[i];
+    if (MODE != 2) v += bias[co];
+    if (MODE == 0) {
+      if (pre) pre[node * COUT + co] = v;
+      out_bf[node * COUT + co] = f2bf(gelu(v));
+    } else if (MODE == 1) {
+      out_f[node] = v;
+    } else {
+      out_bf[node * COUT + co] = f2bf(v);
+    }
+  }
+}
+
+// FP32 -> BF16 input of the first layer
+__global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+// master FP32 parameters W[cout][27][cin] -> BF16 weights wb[16][27][cin]
+// (outputs padded to 16 with zeros) and the transposed, tap-flipped weights
+// wt[16][27][cout] of the input-gradient convolution: wt[c][t][o] = W[o][26-t][c]
+// (tap 26-t is the opposite offset of tap t), rows c >= cin zero.
+__global__ void k_make_bf16_weights(const float *__restrict__ W, int cin, int cout, uint16_t *__restrict__ wb,
+                                    uint16_t *__restrict__ wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 16 * kTaps * cin) {
+    const int c = i % cin, t = (i / cin) % kTaps, o = i / (cin * kTaps);
+    wb[i] = (o < cout) ? f2bf(W[(o * kTaps + t) * cin + c]) : (uint16_t)0;
+  }
+  if (i < 16 * kTaps * cout) {
+    const int o = i % cout, t = (i / cout) % kTaps, c = i / (cout * kTaps);
+    wt[i] = (c < cin) ? f2bf(W[(o * kTaps + (kTaps - 1 - t)) * cin + c]) : (uint16_t)0;
+  }
+}
+
+// ------------------------------------------------------------------ backward
+// loss = sum over interior nodes of (y - t)^2 / count;  gy = 2 (y - t) / count
+__global__ __launch_bounds__(256) void k_loss(int M, const float *__restrict__ y, const float *__restrict__ t,
+                                              float inv_count, uint16_t *__restrict__ gy, float *__restrict__ gyf,
+                                              double *__restrict__ part) {
+  const int64_t N = (int64_t)M * M * M;
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % M), yy = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
+    const bool interior = x > 0 && yy > 0 && z > 0 && x < M - 1 && yy < M - 1 && z < M - 1;
+    const float d = interior ? y[i] - t[i] : 0.f;
+    s += (double)d * d;
+    const float g = 2.f * d * inv_count;
+    gy[i] = f2bf(g);
+    gyf[i] = g;
+  }
+  __shared__ double sh[4];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) * inv_count;
+}
+// gradient through GELU: gpre = gpost * gelu'(pre) (BF16 out, [N][16])
+__global__ void k_gelu_back(const uint16_t *__restrict__ gpost, const float *__restrict__ pre,
+                            uint16_t *__restrict__ gpre, float *__restrict__ gpref, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = bf2f(gpost[i]) * gelu_d(pre[i]);
+  gpre[i] = f2bf(g);
+  gpref[i] = g;
+}
+__global__ void k_sum_parts(const double *__restrict__ p, int n, float *__restrict__ o) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += p[i];
+    o[0] = (float)s;
+  }
+}
+// weight and bias gradients of one layer: part[block][e] for e < cout*27*cin
+// (then cout bias entries), summed over the nodes of the block's tiles;
+// gp: FP32 gradient w.r.t. the layer output [N][cout]; a: BF16 layer input [N][cin]
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void k_wgrad(int M, const float *__restrict__ gp, const uint16_t *__restrict__ a,
+                                               float *__restrict__ part) {
+  constexpr int NE = COUT * kTaps * CIN + COUT;
+  constexpr int TX = 16, TY = 4, TZ = 4;
+  __shared__ float sg[TX * TY * TZ * COUT];
+  __shared__ float sa[(TX + 2) * (TY + 2) * (TZ + 2) * CIN];
+  const int ntx = (M + TX - 1) / TX, nty = (M + TY - 1) / TY, ntz = (M + TZ - 1) / TZ;
+  const int ntile = ntx * nty * ntz;
+  constexpr int EPT = (NE + 255) / 256;
+  float acc[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; k++) acc[k] = 0.f;
+  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int X0 = tx * TX, Y0 = ty * TY, Z0 = tz * TZ;
+    __syncthreads();
+    for (int q = threadIdx.x; q < TX * TY * TZ * COUT; q += 256) {
+      const int c = q % COUT, n = q / COUT;
+      const int xx = X0 + n % TX, yy = Y0 + (n / TX) % TY, zz = Z0 + n / (TX * TY);
+      sg[q] = (xx < M && yy < M && zz < M) ? gp[(xx + (int64_t)M * (yy + (int64_t)M * zz)) * COUT + c] : 0.f;
+    }
+    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
+    for (int q = threadIdx.x; q < HX * HY * HZ * CIN; q += 256) {
+      const int c = q % CIN, n = q / CIN;
+      const int xx = X0 - 1 + n % HX, yy = Y0 - 1 + (n / HX) % HY, zz = Z0 - 1 + n / (HX * HY);
+      sa[q] = (xx >= 0 && yy >= 0 && zz >= 0 && xx < M && yy < M && zz < M)
+                  ? bf2f(a[(xx + (int64_t)M * (yy + (int64_t)M * zz)) * CIN + c])
+                  : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < EPT; k++) {
+      const int e = threadIdx.x + 256 * k;
+      if (e >= NE) continue;
+      float s = 0.f;
+      if (e < COUT * kTaps * CIN) {
+        const int c = e % CIN, t = (e / CIN) % kTaps, o = e / (CIN * kTaps);
+        int dx, dy, dz;
+        tap_off(t, dx, dy, dz);
+        for (int n = 0; n < TX * TY * TZ; n++) {
+          const int nx = n % TX, ny = (n / TX) % TY, nz = n / (TX * TY);
+          s += sg[n * COUT + o] * sa[((nx + 1 + dx) + HX * ((ny + 1 + dy) + HY * (nz + 1 + dz))) * CIN + c];
+        }
+      } else {
+        const int o = e - COUT * kTaps * CIN;
+        for (int n = 0; n < TX * TY * TZ; n++) s += sg[n * COUT + o];
+      }
+      acc[k] += s;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < EPT; k++) {
+    const int e = threadIdx.x + 256 * k;
+    if (e < NE) part[(int64_t)blockIdx.x * NE + e] = acc[k];
+  }
+}
+// grads[e] += sum_b part[b][e] (fixed order)
+__global__ void k_reduce_parts(const float *__restrict__ part, int nb, int ne, float *__restrict__ grads) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  float s = 0.f;
+  for (int b = 0; b < nb; b++) s += part[(int64_t)b * ne + e];
+  grads[e] += s;
+}
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+// deterministic initialisation: uniform(-a, a) with a = sqrt(6 / fan_in) from a counter hash
+__global__ void k_init_params(float *__restrict__ p, int n, int off_w, int nw, float a, uint32_t seed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i >= off_w && i < off_w + nw) {
+    uint32_t h = (uint32_t)i * 747796405u + seed * 2891336453u + 1442695041u;
+    h ^= h >> 16; h *= 0x7feb352dU; h ^= h >> 15; h *= 0x846ca68bU; h ^= h >> 16;
+    p[i] = a * (2.0f * ((h >> 8) * (1.0f / 16777216.0f)) - 1.0f);
+  } else if (i >= off_w + nw && i < n) {
+    p[i] = 0.f;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+void LearnedNet::release() {
+  auto fr = [](void *&q) {
+    if (q) (void)hipFree(q);
+    q = nullptr;
+  };
+  fr((void *&)params); fr((void *&)grads); fr((void *&)adam_m); fr((void *&)adam_v);
+  fr((void *&)wbf); fr((void *&)wtbf); fr((void *&)part); fr((void *&)out);
+  for (int l = 0; l < kNetLayers; l++) { fr((void *&)act[l]); fr((void *&)pre[l]); }
+  fr((void *&)gbuf[0]); fr((void *&)gbuf[1]);
+  fr((void *&)gf); fr((void *&)lpart);
+  cap = 0;
+  part_cap = 0;
+  ready = false;
+}
+void LearnedData::release() {
+  if (u) (void)hipFree(u);
+  if (t) (void)hipFree(t);
+  u = t = nullptr;
+  ns = 0;
+}
+
+static const int kWStride = 16 * kTaps * 16;  // BF16 weight slots per layer (padded)
+static const int kWgradBlocks = 240;
+static const int kLossBlocks = 1024;
+
+void net_init(LearnedNet &net, uint32_t seed, hipStream_t st) {
+  const int P = net_total_params();
+  if (!net.params) {
+    check(hipMalloc(&net.params, sizeof(float) * P), "net params");
+    check(hipMalloc(&net.grads, sizeof(float) * P), "net grads");
+    check(hipMalloc(&net.adam_m, sizeof(float) * P), "adam m");
+    check(hipMalloc(&net.adam_v, sizeof(float) * P), "adam v");
+    check(hipMalloc(&net.wbf, sizeof(uint16_t) * kWStride * kNetLayers), "net wbf");
+    check(hipMalloc(&net.wtbf, sizeof(uint16_t) * kWStride * kNetLayers), "net wtbf");
+  }
+  for (int l = 0; l < kNetLayers; l++) {
+    const int off = net_param_offset(l);
+    const float a = std::sqrt(6.0f / (kTaps * net_cin(l))) * (l == kNetLayers - 1 ? 0.1f : 1.0f);
+    hipLaunchKernelGGL(k_init_params, dim3((net_psize(l) + 255) / 256), dim3(256), 0, st, net.params + off,
+                       net_psize(l), 0, net_wsize(l), a, seed + 977u * (uint32_t)l);
+  }
+  fill<float>(net.adam_m, 0.f, P, st);
+  fill<float>(net.adam_v, 0.f, P, st);
+  fill<float>(net.grads, 0.f, P, st);
+  net.adam_t = 0;
+  net_sync_bf16(net, st);
+  net.ready = true;
+}
+
+void net_sync_bf16(LearnedNet &net, hipStream_t st) {
+  for (int l = 0; l < kNetLayers; l++)
+    hipLaunchKernelGGL(k_make_bf16_weights, dim3((kWStride + 255) / 256), dim3(256), 0, st,
+                       net.params + net_param_offset(l), net_cin(l), net_cout(l), net.wbf + l * kWStride,
+                       net.wtbf + l * kWStride);
+}
+
+bool net_reserve(LearnedNet &net, int64_t N, bool training) {
+  bool changed = false;
+  if (N > net.cap) {
+    changed = true;
+    for (int l = 0; l < kNetLayers; l++) {
+      if (net.act[l]) (void)hipFree(net.act[l]);
+      if (net.pre[l]) (void)hipFree(net.pre[l]);
+      net.act[l] = nullptr;
+      net.pre[l] = nullptr;
+      check(hipMalloc(&net.act[l], sizeof(uint16_t) * N * net_cin(l)), "net act");
+    }
+    if (net.out) (void)hipFree(net.out);
+    check(hipMalloc(&net.out, sizeof(float) * N), "net out");
+    for (auto &g : net.gbuf) {
+      if (g) (void)hipFree(g);
+      g = nullptr;
+    }
+    if (net.gf) (void)hipFree(net.gf);
+    net.gf = nullptr;
+    net.cap = N;
+  }
+  if (training && !net.pre[0]) {
+    changed = true;
+    for (int l = 0; l < kNetLayers; l++) check(hipMalloc(&net.pre[l], sizeof(float) * net.cap * kNetCh), "net pre");
+    for (auto &g : net.gbuf) check(hipMalloc(&g, sizeof(uint16_t) * net.cap * kNetCh), "net grad buf");
+    check(hipMalloc(&net.gf, sizeof(float) * net.cap * kNetCh), "net grad scratch");
+    if (!net.lpart) check(hipMalloc(&net.lpart, sizeof(double) * kLossBlocks), "net loss partials");
+  }
+  const int64_t pc = (int64_t)kWgradBlocks * (16 * kTaps * 16 + 16);
+  if (training && net.part_cap < pc) {
+    if (net.part) (void)hipFree(net.part);
+    check(hipMalloc(&net.part, sizeof(float) * pc), "net part");
+    net.part_cap = pc;
+  }
+  return changed;
+}
+
+static unsigned conv_grid(int M) {
+  const int64_t tiles = (int64_t)((M + 15) / 16) * M * M;
+  return (unsigned)((tiles + 3) / 4);
+}
+
+void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st) {
+  const int64_t N = (int64_t)M * M * M;
+  if (N > net.cap) throw std::runtime_error("net_forward: lattice larger than reserved");
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, in, net.act[0], N);
+  net_forward_act(net, M, training, st);
+}
+
+void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
+  const unsigned g = conv_grid(M);
+  auto P = [&](int l) { return net.params + net_param_offset(l) + net_wsize(l); };  // bias
+  hipLaunchKernelGGL((k_conv<1, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride, P(0),
+                     training ? net.pre[0] : nullptr, net.act[1], nullptr);
+  hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[1], net.wbf + 1 * kWStride, P(1),
+                     training ? net.pre[1] : nullptr, net.act[2], nullptr);
+  hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[2], net.wbf + 2 * kWStride, P(2),
+                     training ? net.pre[2] : nullptr, net.act[3], nullptr);
+  hipLaunchKernelGGL((k_conv<16, 1, 1>), dim3(g), dim3(256), 0, st, M, net.act[3], net.wbf + 3 * kWStride, P(3),
+                     nullptr, nullptr, net.out);
+}
+
+void net_zero_grads(LearnedNet &net, hipStream_t st) { fill<float>(net.grads, 0.f, net_total_params(), st); }
+
+template <int CIN, int COUT>
+static void wgrad(LearnedNet &net, int M, const float *gp, const uint16_t *a, int layer, hipStream_t st) {
+  constexpr int NE = COUT * kTaps * CIN + COUT;
+  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(kWgradBlocks), dim3(256), 0, st, M, gp, a, net.part);
+  hipLaunchKernelGGL(k_reduce_parts, dim3((NE + 255) / 256), dim3(256), 0, st, net.part, kWgradBlocks, NE,
+                     net.grads + net_param_offset(layer));
+}
+
+void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st) {
+  const int64_t N = (int64_t)M * M * M;
+  const double inner = (double)(M - 2) * (M - 2) * (M - 2);
+  float *gf = net.gf;
+  double *lpart = net.lpart;
+  uint16_t *g0 = net.gbuf[0], *g1 = net.gbuf[1];
+  // output layer: gy (BF16 in g0[:N], FP32 in gf[:N])
+  hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, M, net.out, t, (float)(1.0 / inner), g0, gf, lpart);
+  wgrad<16, 1>(net, M, gf, net.act[3], 3, st);
+  const unsigned g = conv_grid(M);
+  // input gradient of layer 3's output: conv of gy with transposed weights (1 -> 16)
+  hipLaunchKernelGGL((k_conv<1, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + 3 * kWStride, nullptr, nullptr,
+                     g1, nullptr);
+  for (int l = 2; l >= 0; l--) {
+    // through GELU of layer l
+    hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((N * kNetCh + 255) / 256)), dim3(256), 0, st, g1, net.pre[l], g0,
+                       gf, N * kNetCh);
+    if (l == 0) wgrad<1, 16>(net, M, gf, net.act[0], 0, st);
+    else wgrad<16, 16>(net, M, gf, net.act[l], l, st);
+    if (l > 0)
+      hipLaunchKernelGGL((k_conv<16, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + l * kWStride, nullptr,
+                         nullptr, g1, nullptr);
+  }
+  if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, lpart, kLossBlocks, loss_dev);
+}
+
+void net_adam(LearnedNet &net, float lr, hipStream_t st) {
+  net.adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)net.adam_t), c2 = 1.f - std::pow(b2, (float)net.adam_t);
+  const int P = net_total_params();
+  hipLaunchKernelGGL(k_adam, dim3((P + 255) / 256), dim3(256), 0, st, net.params, net.grads, net.adam_m, net.adam_v, P,
+                     lr, b1, b2, 1e-8f, c1, c2);
+  net_sync_bf16(net, st);
+}
+
+// ------------------------------------------------------------------ use as a smoother
+__global__ void k_diag_inv(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
+                           const float *__restrict__ val, float *__restrict__ dinv) {
+  const BlockInfo bi = blocks[blockIdx.x];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
+  const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
+  const int64_t vb = s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
+  dinv[p] = 1.0f / val[vb + (int64_t)s.dslot * kSlice];
+}
+__global__ void k_learned_input(const float *__restrict__ dinv, const float *__restrict__ d, float *__restrict__ u,
+                                int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] = dinv[i] * d[i];
+}
+__global__ void k_scale_bf16(const float *__restrict__ u, const double *__restrict__ nrm, double sq, int64_t n,
+                             uint16_t *__restrict__ y) {
+  const double nv = nrm[0];
+  const float sc = nv > 0 ? (float)(sq / nv) : 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = f2bf(u[i] * sc);
+}
+// MODE 0: c = in * nrm / sq;  MODE 1: c = sign * in * sq / nrm  (interior only)
+template <int MODE>
+__global__ void k_interior_scale(const float *__restrict__ in, int M, const double *__restrict__ nrm, double sq,
+                                 float sign, float *__restrict__ c) {
+  const int64_t N = (int64_t)M * M * M;
+  const double nv = nrm[0];
+  float sc;
+  if (MODE == 0) sc = (float)(nv / sq);
+  else sc = nv > 0 ? (float)(sign * sq / nv) : 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
+    const bool interior = x > 0 && y > 0 && z > 0 && x < M - 1 && y < M - 1 && z < M - 1;
+    c[i] = interior ? in[i] * sc : 0.f;
+  }
+}
+__global__ void k_learned_step(float *__restrict__ x, const float *__restrict__ c, const double *__restrict__ p,
+                               int64_t n) {
+  const double a = p[1] > 0 ? fmax(0.0, p[0] / p[1]) : 0.0;
+  const float af = (float)a;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += af * c[i];
+}
+__global__ void k_learned_random(float *__restrict__ xv, int M, uint32_t seed) {
+  const int64_t N = (int64_t)M * M * M;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
+    const bool interior = x > 0 && y > 0 && z > 0 && x < M - 1 && y < M - 1 && z < M - 1;
+    uint32_t h = (uint32_t)i * 2654435761u ^ (seed * 0x9e3779b9u + 0x7f4a7c15u);
+    h ^= h >> 16; h *= 0x7feb352dU; h ^= h >> 15; h *= 0x846ca68bU; h ^= h >> 16;
+    xv[i] = interior ? 2.0f * ((h >> 8) * (1.0f / 16777216.0f)) - 1.0f : 0.f;
+  }
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 4096); }
+
+void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st) {
+  fill<float>(dinv, 0.f, D.N, st);
+  hipLaunchKernelGGL(k_diag_inv, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, val,
+                     dinv);
+}
+void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_learned_input, dim3(grid_for(n)), dim3(256), 0, st, dinv, d, u, n);
+}
+void learned_scale_input(LearnedNet &net, const float *u, const double *nrm, double sq, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_bf16, dim3(grid_for(n)), dim3(256), 0, st, u, nrm, sq, n, net.act[0]);
+}
+void learned_output(const LearnedNet &net, int M, const double *nrm, double sq, float *c, hipStream_t st) {
+  hipLaunchKernelGGL(k_interior_scale<0>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, net.out, M, nrm, sq,
+                     1.0f, c);
+}
+void learned_step(float *x, const float *c, const double *p, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_learned_step, dim3(grid_for(n)), dim3(256), 0, st, x, c, p, n);
+}
+void learned_random(float *x, int M, uint32_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(k_learned_random, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, seed);
+}
+void learned_target(const float *x, int M, const double *nrm, double sq, float sign, float *t, hipStream_t st) {
+  hipLaunchKernelGGL(k_interior_scale<1>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, nrm, sq, sign,
+                     t);
+}
+
+}  // namespace pmx

@@ -1,0 +1,91 @@
+// pmx_learned.hpp -- the learned post-smoother: a 4-layer 3x3x3 convolution
+// network (1 -> 16 -> 16 -> 16 -> 1 channels, GELU between layers) that maps
+// a diagonally scaled defect to a correction on the node lattice of a level.
+// BF16 weights and activations, FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x32_bf16), hand-written forward, backward and Adam kernels.
+// No counterpart in the reference (its smoothers are MGLevel3D.C:280-563).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace pmx {
+
+constexpr int kNetLayers = 4;
+constexpr int kNetCh = 16;   // hidden channels
+constexpr int kTaps = 27;    // 3x3x3 stencil of a layer
+
+// layer l maps cin(l) -> cout(l) channels
+inline int net_cin(int l) { return l == 0 ? 1 : kNetCh; }
+inline int net_cout(int l) { return l == kNetLayers - 1 ? 1 : kNetCh; }
+// FP32 master parameters of layer l: W[cout][tap][cin] then b[cout]
+inline int net_wsize(int l) { return net_cout(l) * kTaps * net_cin(l); }
+inline int net_psize(int l) { return net_wsize(l) + net_cout(l); }
+int net_total_params();
+int net_param_offset(int l);
+
+struct LearnedNet {
+  bool ready = false;
+  float *params = nullptr;   // all layers, FP32 master copy
+  float *grads = nullptr;
+  float *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *wbf = nullptr;   // per layer: BF16 weights padded to 16 outputs [16][27][cin]
+  uint16_t *wtbf = nullptr;  // per layer: transposed, tap-flipped BF16 weights (input gradient)
+  int adam_t = 0;
+  // activations for one lattice of N nodes (allocated for the largest use)
+  int64_t cap = 0;
+  uint16_t *act[kNetLayers] = {nullptr};  // layer inputs, BF16, [N][cin]
+  float *pre[kNetLayers] = {nullptr};     // pre-activations (training), [N][cout]
+  uint16_t *gbuf[2] = {nullptr, nullptr}; // BF16 gradients flowing backwards, [N][16]
+  float *part = nullptr;                  // weight-gradient partial sums
+  int64_t part_cap = 0;
+  float *out = nullptr;                   // network output [N]
+  float *gf = nullptr;                    // FP32 gradient of a layer output [N][16]
+  double *lpart = nullptr;                // loss partial sums
+  void release();
+};
+
+// device buffers for a training set: `ns` samples on an M^3 lattice
+struct LearnedData {
+  int M = 0, ns = 0;
+  float *u = nullptr;   // scaled input  (D^{-1} d / s)
+  float *t = nullptr;   // scaled target (e / s)
+  void release();
+};
+
+void net_init(LearnedNet &net, uint32_t seed, hipStream_t st);
+// returns true if buffers were (re)allocated: launch graphs that captured the
+// old activation buffers must then be rebuilt
+bool net_reserve(LearnedNet &net, int64_t N, bool training);
+// refresh BF16 copies from the FP32 master parameters
+void net_sync_bf16(LearnedNet &net, hipStream_t st);
+// forward on an M^3 lattice: in[N] (FP32) -> net.out[N] (FP32); training keeps pre-activations
+void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st);
+// the same from the BF16 input already in net.act[0]
+void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
+// backward for the loss mean((out - t)^2) over interior nodes; accumulates into net.grads
+// and returns (asynchronously) the loss into *loss_dev
+void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st);
+void net_zero_grads(LearnedNet &net, hipStream_t st);
+void net_adam(LearnedNet &net, float lr, hipStream_t st);
+
+struct DevLayout;
+// dinv[p] = 1 / a_pp on interior nodes, 0 on Dirichlet nodes (FP32, lattice order)
+void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st);
+// u = dinv * d (lattice), the unscaled network input
+void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st);
+// net.act[0] = bf16(u * sqrt_count / *nrm)   (nrm: device FP64 2-norm of u)
+void learned_scale_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, int64_t n,
+                         hipStream_t st);
+// c = net.out * (*nrm) / sqrt_count on interior nodes of an M^3 lattice, 0 on its boundary
+void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
+// x += alpha c with alpha = max(0, p[0] / p[1]) (minimal-residual step; 0 if p[1] == 0)
+void learned_step(float *x, const float *c, const double *p, int64_t n, hipStream_t st);
+// uniform(-1, 1) values on interior nodes of an M^3 lattice, 0 on the boundary
+void learned_random(float *x, int M, uint32_t seed, hipStream_t st);
+// t = sign * x * sqrt_count / *nrm on interior nodes, 0 on the boundary
+void learned_target(const float *x, int M, const double *nrm, double sqrt_count, float sign, float *t,
+                    hipStream_t st);
+
+}  // namespace pmx

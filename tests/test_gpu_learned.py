@@ -1,0 +1,131 @@
+"""Learned post-smoother on the GPU (libpmx through the C-ABI) against the oracle.
+
+Forward: BF16 operands, FP32 accumulation on MFMA vs the oracle with the same
+BF16 rounding and exact sums: relative 2-norm error <= 5e-3.  Gradients: the
+device also rounds the backward flow to BF16, so <= 3e-2 per layer (relative
+2-norm) with cosine similarity >= 0.999.  Adam: FP32 update of the same
+gradient, 1e-6 relative.  Post-smoothing step vs the oracle's FP64 restatement
+of it: update within 3e-2 (relative 2-norm), defect never larger than before.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+import parmoon_amd as pa
+from parmoon_amd import prm as P
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import learned_ref as LR  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def nrel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    S = pa.System(ctx, 1, 2, 1)
+    yield S
+    S.close()
+
+
+@pytest.mark.parametrize("M", [5, 9, 20, 33])
+def test_forward_vs_oracle(small, M):
+    print(f"[case] learned forward M={M}", flush=True)
+    rng = np.random.default_rng(M)
+    p = LR.init_params(M, scale_last=1.0).astype(np.float32)
+    small.learned_params(p)
+    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    out = small.learned_forward(u)
+    ref = LR.forward(p.astype(np.float64), u.astype(np.float64))
+    assert nrel(out, ref) < 5e-3, nrel(out, ref)
+
+
+@pytest.mark.parametrize("M", [6, 17])
+def test_gradient_vs_oracle(small, M):
+    print(f"[case] learned gradient M={M}", flush=True)
+    rng = np.random.default_rng(100 + M)
+    p = LR.init_params(7 + M, scale_last=1.0).astype(np.float32)
+    small.learned_params(p)
+    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    t = rng.standard_normal((M, M, M)).astype(np.float32)
+    g, loss = small.learned_gradient(u, t)
+    l_ref, g_ref = LR.backward(p.astype(np.float64), u.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - l_ref) < 1e-3 * l_ref
+    offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
+    for l in range(4):
+        a, b = g[offs[l]:offs[l + 1]], g_ref[offs[l]:offs[l + 1]]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        assert nrel(a, b) < 3e-2 and cos > 0.999, (l, nrel(a, b), cos)
+
+
+def test_adam_step(small):
+    print("[case] learned adam", flush=True)
+    M = 7
+    rng = np.random.default_rng(11)
+    p = LR.init_params(11).astype(np.float32)
+    small.learned_init(5)
+    small.learned_params(p)     # fresh moments come with init
+    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    t = rng.standard_normal((M, M, M)).astype(np.float32)
+    g, _ = small.learned_gradient(u, t)
+    small.learned_adam(1e-3)
+    p1 = small.learned_params()
+    ref, _, _ = LR.adam(p.astype(np.float64), g.astype(np.float64), 0.0, 0.0, 1, 1e-3)
+    assert np.max(np.abs(p1 - ref)) < 1e-6 * max(1.0, np.max(np.abs(ref)))
+
+
+def lattice_map(space, M):
+    xyz = space.dof_coords()
+    ijk = np.rint(xyz * (M - 1)).astype(int)
+    return ijk[:, 0] + M * (ijk[:, 1] + M * ijk[:, 2])
+
+
+def post_reference(O, params, x, f):
+    """FP64 restatement of one learned post-smoothing step (DOF numbering)."""
+    fine = O.levels[-1]
+    E = O.entries()
+    bi, gn, rp, kc = fine.arrays()
+    N, na = fine.N, fine.active
+    rows = np.repeat(np.arange(N), np.diff(rp))
+    diag = E[(kc == rows)]
+    d, _ = fine.defect(E, x, f)
+    u = np.where(np.arange(N) < na, d / diag, 0.0)
+    nrm, sq = np.linalg.norm(u), np.sqrt(na)
+    M = round(N ** (1 / 3))
+    lat = lattice_map(fine, M)
+    ul = np.zeros(M ** 3)
+    ul[lat] = u * sq / nrm
+    out = LR.forward(params.astype(np.float64), ul.reshape(M, M, M)).ravel()
+    c = np.where(np.arange(N) < na, out[lat] * nrm / sq, 0.0)
+    Ac = fine.matvec(E, c)
+    alpha = max(0.0, float(d @ Ac) / float(Ac @ Ac))
+    return x + alpha * c, d
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_post_step_vs_oracle(ctx, order):
+    print(f"[case] learned post step Q{order}", flush=True)
+    prm = P.config1b() if order == 1 else P.config2(nref=3)
+    S = pa.System(ctx, 1, 3, order)
+    S.assemble(prm)
+    O = orc.System(1, 3, order)
+    O.assemble(prm)
+    p = LR.init_params(21 + order).astype(np.float32)
+    S.learned_params(p)
+    fine = O.levels[-1]
+    rng = np.random.default_rng(order)
+    x = rng.standard_normal(fine.N)
+    f = O.rhs()
+    x[fine.active:] = f[fine.active:]
+    ref, d0 = post_reference(O, p, x, f)
+    got = S.learned_post(x, f)
+    assert nrel(got - x, ref - x) < 3e-2, nrel(got - x, ref - x)
+    d1, _ = fine.defect(O.entries(), got, f)
+    assert np.linalg.norm(d1) <= np.linalg.norm(d0) * (1 + 1e-6)
+    S.close()
