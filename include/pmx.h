@@ -116,6 +116,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [16] stop when ||r|| <= [16] * ||r0|| (r0 = initial FP64 defect; 0 = off)
  *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
  *  [18] 1 = launch the MG cycle kernel by kernel instead of replaying its HIP graph
+ *  [19] precision 2: number of finest levels with the learned post-smoother
+ *       (0 = the levels it was trained for)
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */
@@ -151,11 +153,17 @@ int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out);
  * with respect to the parameters (fresh, not accumulated) */
 int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *target, float *grads,
                          double *loss);
+/* residual loss on a level: c = network output on the level's lattice (0 on
+ * its boundary), loss = sum over interior nodes (in - D^-1 A c)^2 / (M-2)^3,
+ * and its parameter gradient (fresh).  in: the scaled defect, lattice order. */
+int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, float *grads, double *loss);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
 int pmx_learned_adam(pmx_system *sys, double lr);
 /* train on pairs from plain cycles (opt: the cycle options of pmx_system_solve).
- * topt: [0] samples, [1] steps, [2] learning rate, [3] seed, [4] finest levels
- * that use the network.  tinfo: [0] first / [1] last mean loss, [2] s, [3] steps */
+ * topt: [0] samples per level, [1] steps (alternating over the levels),
+ * [2] learning rate, [3] seed, [4] finest levels that use the network (0: all
+ * but the coarsest), [5] loss: 0 squared error against the cycle's error,
+ * 1 the scaled defect left after the correction (as pmx_learned_gradient_level).  tinfo: [0] first / [1] last mean loss, [2] s, [3] steps */
 int pmx_learned_train(pmx_system *sys, const double *opt, const double *topt, double *tinfo);
 /* one learned post-smoothing step on a level (reference numbering, sol in place) */
 int pmx_learned_post(pmx_system *sys, int level, double *sol, const double *f);

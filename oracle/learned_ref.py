@@ -16,10 +16,11 @@ algorithm to follow; the oracle states the network itself:
 * BF16 emulation (``bf16=True``): the input, every weight and every hidden
   activation is rounded to BF16 (round to nearest even) before it enters a
   convolution, the products are summed exactly (FP64 here, FP32 on MFMA);
-* loss = sum over interior nodes (out - target)^2 / (M-2)^3.
+* loss = sum over interior nodes (out - target)^2 / (M-2)^3, or the residual
+  loss of ``residual_head`` (the scaled defect left by the correction).
 
-Pins: ``backward`` is checked against central finite differences of
-``forward`` (FP64, no BF16 rounding), ``conv`` against torch's CPU conv3d in
+Pins: ``backward`` (with either loss head) is checked against central finite
+differences of the loss (FP64, no BF16 rounding), ``conv`` against torch's CPU conv3d in
 float64 (an independent library routine), and ``adam`` against the closed-form
 first step (|step| = lr for every parameter with a nonzero gradient).
 """
@@ -157,16 +158,40 @@ def loss(out, t):
     return float((d * d).sum() / (M - 2) ** 3)
 
 
-def backward(p, u, t, bf16_emulation=True):
-    """(loss, gradient of the loss w.r.t. the packed parameters).
+def mse_head(t):
+    """Squared-error loss against t over interior nodes: out -> (loss, dloss/dout)."""
+    def head(out):
+        M = out.shape[0]
+        return loss(out, t), np.where(interior_mask(M), 2.0 * (out - t) / (M - 2) ** 3, 0.0)
+    return head
+
+
+def residual_head(u, A, dinv):
+    """Residual loss of the smoother: c = out on interior nodes (0 on the
+    boundary), r = u - dinv * (A c) on interior nodes, loss = sum r^2 / (M-2)^3.
+    A: (M^3, M^3) sparse operator in lattice order; dinv: 1/diag, 0 on the boundary."""
+    def head(out):
+        M = out.shape[0]
+        m = interior_mask(M).ravel()
+        n = (M - 2) ** 3
+        c = np.where(m, out.ravel(), 0.0)
+        r = np.where(m, np.ravel(u) - dinv * (A @ c), 0.0)
+        gac = -2.0 / n * dinv * r
+        return float(r @ r / n), np.where(m, A.T @ gac, 0.0).reshape(out.shape)
+    return head
+
+
+def backward(p, u, t=None, bf16_emulation=True, head=None):
+    """(loss, gradient of the loss w.r.t. the packed parameters); the loss is
+    mse_head(t) unless another head is given.
 
     With BF16 emulation the rounding steps are treated as the identity
     (the gradient of the emulated network in the straight-through sense)."""
     r = bf16 if bf16_emulation else (lambda z: np.asarray(z, np.float64))
     layers = unpack(p)
     out, acts, pres = forward(p, u, bf16_emulation, keep=True)
-    M = out.shape[0]
-    g = np.where(interior_mask(M), 2.0 * (out - t) / (M - 2) ** 3, 0.0)[..., None]
+    lval, gout = (head or mse_head(t))(out)
+    g = gout[..., None]
     grads = [None] * 4
     for l in range(3, -1, -1):
         W, b = layers[l]
@@ -174,7 +199,7 @@ def backward(p, u, t, bf16_emulation=True):
         if l > 0:
             ga = conv_input_grad(g, r(W))
             g = ga * gelu_d(pres[l - 1])
-    return loss(out, t), pack(grads)
+    return lval, pack(grads)
 
 
 def adam(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):

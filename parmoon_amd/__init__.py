@@ -62,6 +62,7 @@ def lib():
         L.pmx_learned_forward.argtypes = [vp, ip, vp, vp]
         L.pmx_learned_gradient.argtypes = [vp, ip, vp, vp, vp, vp]
         L.pmx_learned_adam.argtypes = [vp, dp]
+        L.pmx_learned_gradient_level.argtypes = [vp, ip, vp, vp, vp]
         L.pmx_learned_train.argtypes = [vp, vp, vp, vp]
         L.pmx_learned_post.argtypes = [vp, ip, vp, vp]
         _lib = L
@@ -280,10 +281,18 @@ class System:
     def learned_adam(self, lr):
         _ok(lib().pmx_learned_adam(self.h, lr), "pmx_learned_adam")
 
-    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=1):
+    def learned_gradient_level(self, u, level=-1):
+        """Residual-loss gradient for the scaled defect u on a level's lattice."""
+        u = np.ascontiguousarray(u, np.float32)
+        g = np.zeros(self.learned_nparams(), np.float32)
+        loss = np.zeros(1)
+        _ok(lib().pmx_learned_gradient_level(self.h, level, _p(u), _p(g), _p(loss)), "pmx_learned_gradient_level")
+        return g, float(loss[0])
+
+    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=0, loss=1):
         o = np.zeros(24)
         o[:len(opt)] = opt
-        t = np.array([samples, steps, lr, seed, levels], float)
+        t = np.array([samples, steps, lr, seed, levels, loss], float)
         info = np.zeros(4)
         _ok(lib().pmx_learned_train(self.h, _p(o), _p(t), _p(info)), "pmx_learned_train")
         return info

@@ -621,58 +621,91 @@ static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double re
   return 1;
 }
 
+// One residual-loss gradient of the network on level lv for the scaled input
+// u (FP32 lattice): c = mask(net(u)), loss = mean over interior nodes of
+// (u - D^-1 A c)^2 -- the part of the scaled defect that the correction
+// leaves.  Accumulates into net.grads; the loss goes to *loss_dev.
+static void learned_res_step(System &Y, Level &lv, const float *u, float *loss_dev) {
+  hipStream_t st = Y.ctx->st;
+  const int M = lv.L.M;
+  Vecs<float> &v = lv.v32;
+  net_forward(Y.net, M, u, true, st);
+  learned_mask(Y.net.out, M, lv.lc.p, st);
+  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
+  boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
+  learned_res_loss(Y.net, M, u, lv.dinv.p, lv.lac.p, v.aux.p, st);
+  css_transpose(lv.D, v.val.p, v.aux.p, v.d.p, st);
+  net_backward_from(Y.net, M, v.d.p, st);
+  learned_loss_sum(Y.net, loss_dev, st);
+}
+
 // Training of the learned post-smoother on (input, target) pairs from plain
 // cycles of the homogeneous problem (f = 0, exact solution 0): a random
 // interior start x0, then pre-smoothing and the coarse-grid correction of one
 // cycle on the finest level give x; the ideal post-correction is -x, the
 // network input the scaled D^-1 (f - A x).  Adam on the mean squared error.
-// topt: [0] samples, [1] steps, [2] learning rate, [3] seed, [4] levels using
-// the network.  tinfo: [0] mean loss of the first 10 steps, [1] of the last
-// 10, [2] seconds, [3] steps.
+// topt: [0] samples per level, [1] steps (alternating over the levels),
+// [2] learning rate, [3] seed, [4] finest levels using the network (default:
+// all but the coarsest), [5] loss: 0 squared error against -x, 1 the scaled
+// defect left after the correction (learned_res_step).  tinfo: [0] mean loss
+// of the first 10 steps per level, [1] of the last ones, [2] seconds, [3] steps.
 static void learned_train(System &Y, const double *to, double *ti) {
   hipStream_t st = Y.ctx->st;
   const int ns = to[0] > 0 ? (int)to[0] : 16;
   const int steps = to[1] > 0 ? (int)to[1] : 400;
   const double lr = to[2] > 0 ? to[2] : 2e-3;
   const uint32_t seed = (uint32_t)to[3];
-  Y.learned_levels = to[4] > 0 ? std::min((int)to[4], (int)Y.lev.size() - 1) : 1;
   const int L = (int)Y.lev.size() - 1;
-  Level &lv = *Y.lev[L];
-  const int64_t N = lv.L.N;
-  const int M = lv.L.M;
-  const double sq = std::sqrt((double)lv.L.nactive);
+  Y.learned_levels = to[4] > 0 ? std::min((int)to[4], L) : L;
+  const bool residual_loss = to[5] != 0.0;
+  const int nl = Y.learned_levels;
   Y.learn_active = false;
   ensure_learned(Y);
   if (!Y.net.ready) net_init(Y.net, seed ^ 0x5eedu, st);
-  if (net_reserve(Y.net, N, true)) Y.invalidate_graphs();
+  if (net_reserve(Y.net, Y.lev[L]->L.N, true)) Y.invalidate_graphs();
+  // samples of level L - j at off[j] + k N_j
+  std::vector<size_t> off(nl + 1, 0);
+  for (int j = 0; j < nl; j++) off[j + 1] = off[j] + (size_t)ns * Y.lev[L - j]->L.N;
   DevBuf<float> du, dt;
-  du.alloc((size_t)ns * N);
-  dt.alloc((size_t)ns * N);
+  du.alloc(off[nl]);
+  if (!residual_loss) dt.alloc(off[nl]);
   DevBuf<float> losses;
   losses.alloc(steps);
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
   check(hipEventRecord(e0, st), "event");
-  Vecs<float> &v = lv.v32;
   set_recursion(Y);
-  for (int k = 0; k < ns; k++) {
-    learned_random(v.x.p, M, seed * 7919u + (uint32_t)k, st);
-    fill<float>(v.f.p, 0.0f, N, st);
-    mg_cycle<float>(Y, L, false);
-    css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
-    boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
-    learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
-    dot<float>(lv.lc.p, lv.lc.p, N, Y.ctx->part.p, Y.lscal.p, 1, st);
-    learned_target(lv.lc.p, M, Y.lscal.p, sq, 1.0f, du.p + (size_t)k * N, st);
-    learned_target(v.x.p, M, Y.lscal.p, sq, -1.0f, dt.p + (size_t)k * N, st);
+  for (int j = 0; j < nl; j++) {
+    Level &lv = *Y.lev[L - j];
+    Vecs<float> &v = lv.v32;
+    const int64_t N = lv.L.N;
+    const int M = lv.L.M;
+    const double sq = std::sqrt((double)lv.L.nactive);
+    for (int k = 0; k < ns; k++) {
+      learned_random(v.x.p, M, seed * 7919u + 104729u * (uint32_t)j + (uint32_t)k, st);
+      fill<float>(v.f.p, 0.0f, N, st);
+      mg_cycle<float>(Y, L - j, false);
+      css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+      boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
+      learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
+      dot<float>(lv.lc.p, lv.lc.p, N, Y.ctx->part.p, Y.lscal.p, 1, st);
+      learned_target(lv.lc.p, M, Y.lscal.p, sq, 1.0f, du.p + off[j] + (size_t)k * N, st);
+      if (!residual_loss) learned_target(v.x.p, M, Y.lscal.p, sq, -1.0f, dt.p + off[j] + (size_t)k * N, st);
+    }
   }
   for (int k = 0; k < steps; k++) {
-    const int smp = k % ns;
+    const int j = k % nl, smp = (k / nl) % ns;
+    Level &lv = *Y.lev[L - j];
+    const size_t o = off[j] + (size_t)smp * lv.L.N;
     const double lrk = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * k / std::max(1, steps - 1))));
     net_zero_grads(Y.net, st);
-    net_forward(Y.net, M, du.p + (size_t)smp * N, true, st);
-    net_backward(Y.net, M, dt.p + (size_t)smp * N, losses.p + k, st);
+    if (residual_loss) {
+      learned_res_step(Y, lv, du.p + o, losses.p + k);
+    } else {
+      net_forward(Y.net, lv.L.M, du.p + o, true, st);
+      net_backward(Y.net, lv.L.M, dt.p + o, losses.p + k, st);
+    }
     net_adam(Y.net, (float)lrk, st);
   }
   check(hipEventRecord(e1, st), "event");
@@ -683,7 +716,7 @@ static void learned_train(System &Y, const double *to, double *ti) {
   (void)hipEventDestroy(e1);
   std::vector<float> h(steps);
   check(hipMemcpy(h.data(), losses.p, sizeof(float) * steps, hipMemcpyDeviceToHost), "losses");
-  const int w = std::min(10, steps);
+  const int w = std::min(10 * nl, steps);
   double a = 0, b = 0;
   for (int k = 0; k < w; k++) { a += h[k]; b += h[steps - 1 - k]; }
   if (ti) { ti[0] = a / w; ti[1] = b / w; ti[2] = ms * 1e-3; ti[3] = steps; }
@@ -726,6 +759,7 @@ static int solve_device(System &Y, double *info) {
     Y.fallbacks = 0;
     if (prec == 2) {
       if (!Y.net.ready) throw std::logic_error("learned smoother: train or load parameters first");
+      if (Y.opt[19] > 0) Y.learned_levels = std::min((int)Y.opt[19], (int)Y.lev.size() - 1);
       ensure_learned(Y);
       Y.learn_active = true;
     }
@@ -1269,6 +1303,34 @@ int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *t
     net_zero_grads(Y.net, st);
     net_forward(Y.net, M, din.p, true, st);
     net_backward(Y.net, M, dtg.p, dl.p, st);
+    if (grads)
+      check(hipMemcpyAsync(grads, Y.net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
+            "d2h grads");
+    float l = 0;
+    check(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, st), "d2h loss");
+    check(hipStreamSynchronize(st), "sync");
+    if (loss) *loss = l;
+  })
+}
+int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, float *grads, double *loss) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (!in) throw std::invalid_argument("NULL argument");
+    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    Level &lv = level_of(Y, level);
+    const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    hipStream_t st = Y.ctx->st;
+    const int keep = Y.learned_levels;
+    Y.learned_levels = (int)Y.lev.size() - li;
+    ensure_learned(Y);
+    Y.learned_levels = keep;
+    if (net_reserve(Y.net, lv.L.N, true)) Y.invalidate_graphs();
+    DevBuf<float> din, dl;
+    din.upload(in, lv.L.N, st);
+    dl.alloc(1);
+    net_zero_grads(Y.net, st);
+    learned_res_step(Y, lv, din.p, dl.p);
     if (grads)
       check(hipMemcpyAsync(grads, Y.net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
             "d2h grads");

@@ -53,78 +53,120 @@ __device__ __forceinline__ void tap_off(int t, int &dx, int &dy, int &dz) {
 }
 
 // ------------------------------------------------------------------ conv
+// A workgroup (4 wavefronts) computes a brick of 16 x 4 x 4 output nodes.  The
+// input brick with its one-node halo (18 x 6 x 6 nodes, all channels) is
+// staged once in LDS; every wavefront then runs 4 x-lines of 16 nodes (4
+// independent MFMA accumulation chains) reading its A fragments from LDS.
+// The B fragments (weights) stay in registers for the whole brick.
 // MODE 0: hidden layer, out_bf = bf16(GELU(conv + b)), pre = conv + b (if set)
 // MODE 1: output layer (COUT = 1), out_f = conv + b
 // MODE 2: input-gradient convolution, out_bf = bf16(conv), no bias
+constexpr int kBX = 16, kBY = 4, kBZ = 4;
+constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2, kHalo = kHX * kHY * kHZ;
+
 template <int CIN, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                               const float *__restrict__ bias, float *__restrict__ pre,
                                               uint16_t *__restrict__ out_bf, float *__restrict__ out_f) {
-  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int ntx = (M + 15) / 16;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + wave;
-  if (tile >= (int64_t)ntx * M * M) return;  // whole wavefront
-  const int tx = (int)(tile % ntx);
-  const int64_t r = tile / ntx;
-  const int y = (int)(r % M), z = (int)(r / M);
-  const int x0 = tx * 16;
-  const int row = l & 15, g = l >> 4;
-  const int xr = x0 + row;
-  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  constexpr int NCH = CIN == 1 ? 1 : 14;  // K chunks of 32
+  __shared__ __attribute__((aligned(16))) uint16_t sh[kHalo * CIN];
+  const int nbx = (M + kBX - 1) / kBX, nby = (M + kBY - 1) / kBY;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+  const int X0 = bx * kBX, Y0 = by * kBY, Z0 = bz * kBZ;
+  // stage the halo brick (zero outside the lattice)
   if (CIN == 1) {
-    u16x8 a, b;
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int t = 8 * g + j;
-      uint16_t av = 0, bv = 0;
-      if (t < kTaps) {
-        int dx, dy, dz;
-        tap_off(t, dx, dy, dz);
-        const int xx = xr + dx, yy = y + dy, zz = z + dz;
-        if (xr < M && xx >= 0 && xx < M && yy >= 0 && yy < M && zz >= 0 && zz < M)
-          av = in[xx + (int64_t)M * (yy + (int64_t)M * zz)];
-        bv = w[row * kTaps + t];
-      }
-      a[j] = av;
-      b[j] = bv;
+    for (int q = threadIdx.x; q < kHalo; q += 256) {
+      const int hx = q % kHX, hy = (q / kHX) % kHY, hz = q / (kHX * kHY);
+      const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
+      sh[q] = (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M) ? in[x + (int64_t)M * (y + (int64_t)M * z)]
+                                                                      : (uint16_t)0;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
-                                                  0, 0, 0);
   } else {
-    const int cb = 8 * (g & 1);
-#pragma unroll 2
-    for (int c = 0; c < 14; c++) {
-      const int t = 2 * c + (g >> 1);
-      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0}, b = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (t < kTaps) {
-        int dx, dy, dz;
-        tap_off(t, dx, dy, dz);
-        const int xx = xr + dx, yy = y + dy, zz = z + dz;
-        if (xr < M && xx >= 0 && xx < M && yy >= 0 && yy < M && zz >= 0 && zz < M)
-          a = *reinterpret_cast<const u16x8 *>(in + (xx + (int64_t)M * (yy + (int64_t)M * zz)) * CIN + cb);
-        b = *reinterpret_cast<const u16x8 *>(w + ((int64_t)row * kTaps + t) * CIN + cb);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
-                                                    acc, 0, 0, 0);
+    for (int q = threadIdx.x; q < kHalo * 2; q += 256) {
+      const int n = q >> 1, half = q & 1;
+      const int hx = n % kHX, hy = (n / kHX) % kHY, hz = n / (kHX * kHY);
+      const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
+      u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M)
+        v = *reinterpret_cast<const u16x8 *>(in + (x + (int64_t)M * (y + (int64_t)M * z)) * CIN + 8 * half);
+      *reinterpret_cast<u16x8 *>(sh + n * CIN + 8 * half) = v;
     }
   }
-  // C: row 4g+i (node x0+4g+i), column l&15 (output channel)
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int row = l & 15, g = l >> 4;
+  // weights -> registers: chunk c holds k = 32 c + 8 g + j
+  u16x8 bw[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    u16x8 b = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (CIN == 1) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int t = 8 * g + j;
+        b[j] = t < kTaps ? w[row * kTaps + t] : (uint16_t)0;
+      }
+    } else {
+      const int t = 2 * c + (g >> 1);
+      if (t < kTaps) b = *reinterpret_cast<const u16x8 *>(w + ((int64_t)row * kTaps + t) * CIN + 8 * (g & 1));
+    }
+    bw[c] = b;
+  }
+  __syncthreads();
+  // x-lines (y, z) = (wave, 0..3) of the brick
+  floatx4 acc[kBZ];
+#pragma unroll
+  for (int q = 0; q < kBZ; q++) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+#pragma unroll
+    for (int q = 0; q < kBZ; q++) {
+      u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (CIN == 1) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int t = 8 * g + j;
+          if (t < kTaps) {
+            int dx, dy, dz;
+            tap_off(t, dx, dy, dz);
+            a[j] = sh[(row + 1 + dx) + kHX * ((wave + 1 + dy) + kHY * (q + 1 + dz))];
+          }
+        }
+      } else {
+        const int t = 2 * c + (g >> 1);
+        if (t < kTaps) {
+          int dx, dy, dz;
+          tap_off(t, dx, dy, dz);
+          a = *reinterpret_cast<const u16x8 *>(
+              sh + ((row + 1 + dx) + kHX * ((wave + 1 + dy) + kHY * (q + 1 + dz))) * CIN + 8 * (g & 1));
+        }
+      }
+      acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bw[c]),
+                                                       acc[q], 0, 0, 0);
+    }
+  }
+  // C fragment: row 4 g + i (node X0 + 4 g + i), column l & 15 (output channel)
   const int co = l & 15;
   if (co >= COUT) return;
+  const float bv = (MODE != 2) ? bias[co] : 0.f;
+  const int y = Y0 + wave;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int xn = x0 + 4 * g + i;
-    if (xn >= M) continue;
-    const int64_t node = xn + (int64_t)M * (y + (int64_t)M * z);
-    float v = acc[i];
-    if (MODE != 2) v += bias[co];
-    if (MODE == 0) {
-      if (pre) pre[node * COUT + co] = v;
-      out_bf[node * COUT + co] = f2bf(gelu(v));
-    } else if (MODE == 1) {
-      out_f[node] = v;
-    } else {
-      out_bf[node * COUT + co] = f2bf(v);
+  for (int q = 0; q < kBZ; q++) {
+    const int z = Z0 + q;
+    if (y >= M || z >= M) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int xn = X0 + 4 * g + i;
+      if (xn >= M) continue;
+      const int64_t node = xn + (int64_t)M * (y + (int64_t)M * z);
+      const float v = acc[q][i] + bv;
+      if (MODE == 0) {
+        if (pre) pre[node * COUT + co] = v;
+        out_bf[node * COUT + co] = f2bf(gelu(v));
+      } else if (MODE == 1) {
+        out_f[node] = v;
+      } else {
+        out_bf[node * COUT + co] = f2bf(v);
+      }
     }
   }
 }
@@ -376,8 +418,7 @@ bool net_reserve(LearnedNet &net, int64_t N, bool training) {
 }
 
 static unsigned conv_grid(int M) {
-  const int64_t tiles = (int64_t)((M + 15) / 16) * M * M;
-  return (unsigned)((tiles + 3) / 4);
+  return (unsigned)(((M + kBX - 1) / kBX) * ((M + kBY - 1) / kBY) * ((M + kBZ - 1) / kBZ));
 }
 
 void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st) {
@@ -410,14 +451,40 @@ static void wgrad(LearnedNet &net, int M, const float *gp, const uint16_t *a, in
                      net.grads + net_param_offset(layer));
 }
 
+static void backward_layers(LearnedNet &net, int M, hipStream_t st);
+
 void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st) {
   const int64_t N = (int64_t)M * M * M;
   const double inner = (double)(M - 2) * (M - 2) * (M - 2);
   float *gf = net.gf;
   double *lpart = net.lpart;
-  uint16_t *g0 = net.gbuf[0], *g1 = net.gbuf[1];
+  uint16_t *g0 = net.gbuf[0];
   // output layer: gy (BF16 in g0[:N], FP32 in gf[:N])
   hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, M, net.out, t, (float)(1.0 / inner), g0, gf, lpart);
+  backward_layers(net, M, st);
+  if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, lpart, kLossBlocks, loss_dev);
+}
+
+__global__ void k_split_grad(const float *__restrict__ gy, uint16_t *__restrict__ gb, float *__restrict__ gf,
+                             int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    gb[i] = f2bf(gy[i]);
+    gf[i] = gy[i];
+  }
+}
+
+void net_backward_from(LearnedNet &net, int M, const float *gy, hipStream_t st) {
+  const int64_t N = (int64_t)M * M * M;
+  hipLaunchKernelGGL(k_split_grad, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, gy,
+                     net.gbuf[0], net.gf, N);
+  backward_layers(net, M, st);
+}
+
+// the layers' backward pass, from gy in gbuf[0] (BF16) and gf (FP32)
+static void backward_layers(LearnedNet &net, int M, hipStream_t st) {
+  const int64_t N = (int64_t)M * M * M;
+  float *gf = net.gf;
+  uint16_t *g0 = net.gbuf[0], *g1 = net.gbuf[1];
   wgrad<16, 1>(net, M, gf, net.act[3], 3, st);
   const unsigned g = conv_grid(M);
   // input gradient of layer 3's output: conv of gy with transposed weights (1 -> 16)
@@ -433,7 +500,6 @@ void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipSt
       hipLaunchKernelGGL((k_conv<16, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + l * kWStride, nullptr,
                          nullptr, g1, nullptr);
   }
-  if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, lpart, kLossBlocks, loss_dev);
 }
 
 void net_adam(LearnedNet &net, float lr, hipStream_t st) {
@@ -529,6 +595,90 @@ void learned_random(float *x, int M, uint32_t seed, hipStream_t st) {
 void learned_target(const float *x, int M, const double *nrm, double sq, float sign, float *t, hipStream_t st) {
   hipLaunchKernelGGL(k_interior_scale<1>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, nrm, sq, sign,
                      t);
+}
+
+// ------------------------------------------------------------------ residual loss
+__global__ void k_mask(const float *__restrict__ in, int M, float *__restrict__ out) {
+  const int64_t N = (int64_t)M * M * M;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
+    const bool interior = x > 0 && y > 0 && z > 0 && x < M - 1 && y < M - 1 && z < M - 1;
+    out[i] = interior ? in[i] : 0.f;
+  }
+}
+// r = u - dinv * Ac on interior nodes; g = -2 dinv r / count; part = sum r^2 / count
+__global__ __launch_bounds__(256) void k_res_loss(int M, const float *__restrict__ u, const float *__restrict__ dinv,
+                                                  const float *__restrict__ ac, float inv_count,
+                                                  float *__restrict__ g, double *__restrict__ part) {
+  const int64_t N = (int64_t)M * M * M;
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
+    const bool interior = x > 0 && y > 0 && z > 0 && x < M - 1 && y < M - 1 && z < M - 1;
+    const float r = interior ? u[i] - dinv[i] * ac[i] : 0.f;
+    s += (double)r * r;
+    g[i] = -2.f * inv_count * dinv[i] * r;
+  }
+  __shared__ double sh[4];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) * inv_count;
+}
+// y_j = sum over interior rows i of a_ij g_i, for interior j (0 on the boundary):
+// the transpose of the stored stencil operator restricted to interior nodes.
+// Row i holds a_ij at the slot of offset j - i when |j - i| is within its extents.
+template <int E>
+__global__ __launch_bounds__(256) void k_css_transpose(int M, int S, const DevSegment *__restrict__ segs,
+                                                       const int *__restrict__ segidx, const float *__restrict__ val,
+                                                       const float *__restrict__ g, float *__restrict__ y) {
+  const int64_t N = (int64_t)M * M * M;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+    const int x = (int)(j % M), yy = (int)((j / M) % M), z = (int)(j / ((int64_t)M * M));
+    if (x == 0 || yy == 0 || z == 0 || x == M - 1 || yy == M - 1 || z == M - 1) { y[j] = 0.f; continue; }
+    float acc = 0.f;
+    for (int dz = -E; dz <= E; dz++) {
+      const int iz = z + dz;
+      if (iz < 1 || iz > M - 2) continue;
+      for (int dy = -E; dy <= E; dy++) {
+        const int iy = yy + dy;
+        if (iy < 1 || iy > M - 2) continue;
+        for (int dx = -E; dx <= E; dx++) {
+          const int ix = x + dx;
+          if (ix < 1 || ix > M - 2) continue;
+          const int si = segidx[(ix % S) + S * ((iy % S) + S * (iz % S))];
+          const DevSegment s = segs[si];
+          // offset of column j in row i is -d
+          if (dx < -s.ex || dx > s.ex || dy < -s.ey || dy > s.ey || dz < -s.ez || dz > s.ez) continue;
+          const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
+          const int idx = (ix - s.x0) / S + s.cx * ((iy - s.y0) / S + s.cy * ((iz - s.z0) / S));
+          const int64_t vb = s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
+          acc += val[vb + (int64_t)slot * kSlice] * g[ix + (int64_t)M * (iy + (int64_t)M * iz)];
+        }
+      }
+    }
+    y[j] = acc;
+  }
+}
+
+void learned_mask(const float *in, int M, float *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, in, M, out);
+}
+void learned_res_loss(LearnedNet &net, int M, const float *u, const float *dinv, const float *ac, float *g,
+                      hipStream_t st) {
+  const double inner = (double)(M - 2) * (M - 2) * (M - 2);
+  hipLaunchKernelGGL(k_res_loss, dim3(kLossBlocks), dim3(256), 0, st, M, u, dinv, ac, (float)(1.0 / inner), g,
+                     net.lpart);
+}
+void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, net.lpart, kLossBlocks, loss_dev);
+}
+void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st) {
+  const unsigned grid = (unsigned)std::min<int64_t>((D.N + 255) / 256, 8192);
+  if (D.order == 2)
+    hipLaunchKernelGGL(k_css_transpose<2>, dim3(grid), dim3(256), 0, st, D.M, D.S, D.segs, D.segidx, val, g, y);
+  else
+    hipLaunchKernelGGL(k_css_transpose<1>, dim3(grid), dim3(256), 0, st, D.M, D.S, D.segs, D.segidx, val, g, y);
 }
 
 }  // namespace pmx

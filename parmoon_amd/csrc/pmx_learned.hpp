@@ -67,6 +67,8 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
 // backward for the loss mean((out - t)^2) over interior nodes; accumulates into net.grads
 // and returns (asynchronously) the loss into *loss_dev
 void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st);
+// backward from a given FP32 gradient of the loss w.r.t. net.out (accumulates into net.grads)
+void net_backward_from(LearnedNet &net, int M, const float *gy, hipStream_t st);
 void net_zero_grads(LearnedNet &net, hipStream_t st);
 void net_adam(LearnedNet &net, float lr, hipStream_t st);
 
@@ -82,6 +84,15 @@ void learned_scale_input(LearnedNet &net, const float *u, const double *nrm, dou
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
 // x += alpha c with alpha = max(0, p[0] / p[1]) (minimal-residual step; 0 if p[1] == 0)
 void learned_step(float *x, const float *c, const double *p, int64_t n, hipStream_t st);
+// out = in on interior nodes of an M^3 lattice, 0 on its boundary
+void learned_mask(const float *in, int M, float *out, hipStream_t st);
+// residual loss mean over interior nodes of (u - dinv * Ac)^2: its gradient
+// w.r.t. Ac into g, partial sums into net.lpart (learned_loss_sum adds them up)
+void learned_res_loss(LearnedNet &net, int M, const float *u, const float *dinv, const float *ac, float *g,
+                      hipStream_t st);
+void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st);
+// y = transpose of the interior stencil operator applied to g (0 on the boundary)
+void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st);
 // uniform(-1, 1) values on interior nodes of an M^3 lattice, 0 on the boundary
 void learned_random(float *x, int M, uint32_t seed, hipStream_t st);
 // t = sign * x * sqrt_count / *nrm on interior nodes, 0 on the boundary

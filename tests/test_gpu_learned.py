@@ -129,3 +129,43 @@ def test_post_step_vs_oracle(ctx, order):
     d1, _ = fine.defect(O.entries(), got, f)
     assert np.linalg.norm(d1) <= np.linalg.norm(d0) * (1 + 1e-6)
     S.close()
+
+
+def lattice_operator(O, M):
+    """Oracle operator and 1/diag of the finest level in lattice order."""
+    import scipy.sparse as sp
+    fine = O.levels[-1]
+    E = O.entries()
+    bi, gn, rp, kc = fine.arrays()
+    lat = lattice_map(fine, M)
+    A = sp.csr_matrix((E, kc, rp), shape=(fine.N, fine.N)).tocoo()
+    A_lat = sp.csr_matrix((A.data, (lat[A.row], lat[A.col])), shape=(fine.N, fine.N))
+    interior = np.zeros(fine.N, bool)
+    interior[lat[:fine.active]] = True
+    dinv = np.where(interior, 1.0 / A_lat.diagonal(), 0.0)
+    return A_lat, dinv
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_residual_gradient_vs_oracle(ctx, order):
+    print(f"[case] learned residual-loss gradient Q{order}", flush=True)
+    prm = P.config1b() if order == 1 else P.config2(nref=3)
+    S = pa.System(ctx, 1, 3, order)
+    S.assemble(prm)
+    O = orc.System(1, 3, order)
+    O.assemble(prm)
+    M = round(O.levels[-1].N ** (1 / 3))
+    A, dinv = lattice_operator(O, M)
+    p = LR.init_params(31 + order, scale_last=1.0).astype(np.float32)
+    S.learned_params(p)
+    u = np.random.default_rng(40 + order).standard_normal((M, M, M)).astype(np.float32)
+    g, loss = S.learned_gradient_level(u)
+    l_ref, g_ref = LR.backward(p.astype(np.float64), u.astype(np.float64),
+                               head=LR.residual_head(u.astype(np.float64), A, dinv))
+    assert abs(loss - l_ref) < 1e-3 * l_ref, (loss, l_ref)
+    offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
+    for l in range(4):
+        a, b = g[offs[l]:offs[l + 1]], g_ref[offs[l]:offs[l + 1]]
+        cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        assert nrel(a, b) < 3e-2 and cos > 0.999, (l, nrel(a, b), cos)
+    S.close()

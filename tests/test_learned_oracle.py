@@ -94,3 +94,36 @@ def test_adam_first_step():
     assert np.allclose(np.abs(step[nz]), 1e-3, rtol=1e-6)
     assert np.all(np.sign(step[nz]) == np.sign(g[nz]))
     assert step[3] == 0.0
+
+
+def test_residual_head_vs_finite_differences():
+    import scipy.sparse as sp
+    rng = np.random.default_rng(8)
+    M = 5
+    N = M ** 3
+    A = sp.random(N, N, density=0.05, random_state=9, format="csr") + sp.eye(N) * 4.0
+    m = LR.interior_mask(M).ravel()
+    dinv = np.where(m, 1.0 / A.diagonal(), 0.0)
+    p = LR.init_params(10, scale_last=1.0)
+    u = rng.standard_normal((M, M, M))
+    head = LR.residual_head(u, A, dinv)
+    l0, g = LR.backward(p, u, bf16_emulation=False, head=head)
+    offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
+    idx = np.concatenate([rng.choice(np.arange(offs[l], offs[l + 1]), 8, replace=False) for l in range(4)])
+    h = 1e-5
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (head(LR.forward(pp, u, False))[0] - head(LR.forward(pm, u, False))[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(g))), (i, fd, g[i])
+    # with a zero correction the loss is the scaled defect itself
+    pz = p.copy()
+    k = 0
+    for l in range(4):
+        n = LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1]
+        if l == 3:
+            pz[k:k + n] = 0.0
+        k += n
+    lz, _ = LR.backward(pz, u, bf16_emulation=False, head=head)
+    assert abs(lz - float((u[1:-1, 1:-1, 1:-1] ** 2).sum()) / (M - 2) ** 3) < 1e-12
