@@ -128,6 +128,24 @@ int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, doub
 int pmx_system_solve_resident(pmx_system *sys, const double *opt, double *info);
 int pmx_system_solution(pmx_system *sys, double *sol);
 
+/* ---- time stepping --------------------------------------------------------
+ * TSystemTCD3D (src/System/SystemTCD3D.C:208-428) as driven by the time loop
+ * of 3DPrograms/TCD3D_ParMooN.C:565-663.
+ * init: AssembleMRhs at prm[PMX_PRM_TIME] (mass matrix on every level and the
+ * rhs, which becomes oldrhs); the solution is sol0 (reference numbering) or,
+ * if sol0 is NULL, the nodal interpolant of the example's initial function;
+ * its Dirichlet values are those of the rhs. */
+int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0);
+/* One step to the time prm[PMX_PRM_TIME] (the caller advances it by tau):
+ * AssembleARhs with the stiffness form prm[PMX_PRM_FORM], AssembleSystMat
+ * (B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol on active rows,
+ * rhs on Dirichlet rows; system matrix M + tau th1 A on every level), Solve
+ * with opt (as pmx_system_solve; info likewise), oldrhs = rhs.
+ * td: [0] tau [1] theta1 [2] theta2 [3] theta3 [4] theta4
+ * (SetTimeDiscParameters, src/FE/TimeDiscRout.C:59-140).  The solution stays
+ * on the device (pmx_system_solution reads it). */
+int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const double *opt, double *info);
+
 /* ---- measurement ---------------------------------------------------------
  * Times `reps` back-to-back launches of one kernel on the context stream with
  * HIP events: 0 stencil SpMV, 1 defect, 2 Jacobi sweep, 3 SSOR sweep,
@@ -145,8 +163,12 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
  * tap t = (dx+1) + 3 (dy+1) + 9 (dz+1).  Lattice arrays are x-fastest M^3. */
 int pmx_learned_nparams(int *n);
 int pmx_learned_init(pmx_system *sys, unsigned seed);
-/* set != 0: upload params; set == 0: download them */
+/* Each level above the coarsest has its own network.  init: fresh draws for
+ * all of them.  params: set != 0 uploads, set == 0 downloads (finest level's
+ * network; _level for any level).  forward / gradient / adam act on the
+ * finest level's network. */
 int pmx_learned_params(pmx_system *sys, int set, float *params);
+int pmx_learned_params_level(pmx_system *sys, int level, int set, float *params);
 /* network output on an M^3 lattice */
 int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out);
 /* loss = sum over interior nodes (out - target)^2 / (M-2)^3 and its gradient
@@ -160,10 +182,13 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
 int pmx_learned_adam(pmx_system *sys, double lr);
 /* train on pairs from plain cycles (opt: the cycle options of pmx_system_solve).
- * topt: [0] samples per level, [1] steps (alternating over the levels),
+ * topt: [0] samples per level, [1] Adam steps per level (levels interleaved),
  * [2] learning rate, [3] seed, [4] finest levels that use the network (0: all
  * but the coarsest), [5] loss: 0 squared error against the cycle's error,
- * 1 the scaled defect left after the correction (as pmx_learned_gradient_level).  tinfo: [0] first / [1] last mean loss, [2] s, [3] steps */
+ * 1 the scaled defect left after the correction (as pmx_learned_gradient_level).
+ * tinfo (16 doubles): [0] first / [1] last mean loss of the finest network,
+ * [2] s, [3] total steps, [4 + 2j] / [5 + 2j] first / last mean loss of the
+ * network of level L - j. */
 int pmx_learned_train(pmx_system *sys, const double *opt, const double *topt, double *tinfo);
 /* one learned post-smoothing step on a level (reference numbering, sol in place) */
 int pmx_learned_post(pmx_system *sys, int level, double *sol, const double *f);

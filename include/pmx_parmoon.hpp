@@ -1,0 +1,211 @@
+// pmx_parmoon.hpp -- C++ classes over the libpmx C-ABI with the names and
+// call order of ParMooN's CD3D/TCD3D surface, for host code written against
+// that surface.  Header-only; link with libpmx.so.
+//
+//   TFESpace3D         GetN_DegreesOfFreedom / GetActiveBound / GetN_Cells /
+//                      GetBeginIndex / GetGlobalNumbers   (include/FE/FESpace3D.h)
+//   TSquareStructure3D GetN_Rows / GetN_Entries / GetActiveBound / GetRowPtr /
+//                      GetKCol                            (include/FE/SquareStructure3D.h)
+//   TSystemCD3D        ctor(N_levels, ...), Init, Assemble, Solve
+//                                                         (include/System/SystemCD3D.h:129-140)
+//   TSystemTCD3D       AssembleMRhs, AssembleARhs, AssembleSystMat, Solve,
+//                      RestoreMassMat                     (include/System/SystemTCD3D.h)
+//
+// Differences: parameters come from an explicit block (pmx_prm.h) and an
+// option vector (pmx.h) instead of TDatabase; errors throw pmx::Error
+// instead of calling exit(); the spaces are the unit cube refined
+// nref0 .. nref0 + N_levels - 1 times.
+#ifndef PMX_PARMOON_HPP
+#define PMX_PARMOON_HPP
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pmx.h"
+
+namespace pmx {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string &what) : std::runtime_error(what), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(int rc, const char *what) {
+  if (rc != PMX_OK) throw Error(rc, std::string(what) + ": " + pmx_last_error());
+}
+
+class Context {
+ public:
+  explicit Context(int device = 0) { check(pmx_ctx_create(device, &h_), "pmx_ctx_create"); }
+  ~Context() { pmx_ctx_destroy(h_); }
+  Context(const Context &) = delete;
+  Context &operator=(const Context &) = delete;
+  pmx_ctx *handle() const { return h_; }
+
+ private:
+  pmx_ctx *h_ = nullptr;
+};
+
+// Scalar Q1/Q2 space of the unit cube refined nref times (host only).
+class TFESpace3D {
+ public:
+  TFESpace3D(int nref, int order) : nref_(nref), order_(order) {
+    check(pmx_host_space_dims(nref, order, dims_), "pmx_host_space_dims");
+  }
+  int GetN_DegreesOfFreedom() const { return (int)dims_[0]; }
+  int GetActiveBound() const { return (int)dims_[1]; }
+  int GetN_Cells() const { return (int)dims_[2]; }
+  int GetN_BaseFunct() const { return (int)dims_[4]; }
+  int GetOrder() const { return order_; }
+  int GetRefinements() const { return nref_; }
+  const int *GetBeginIndex() { load(); return bi_.data(); }
+  const int *GetGlobalNumbers() { load(); return gn_.data(); }
+  const int *RowPtr() { load(); return rp_.data(); }
+  const int *KCol() { load(); return kc_.data(); }
+  long long N_Entries() const { return dims_[3]; }
+
+ private:
+  void load() {
+    if (!bi_.empty()) return;
+    bi_.resize(dims_[2] + 1);
+    gn_.resize(dims_[2] * dims_[4]);
+    rp_.resize(dims_[0] + 1);
+    kc_.resize(dims_[3]);
+    check(pmx_host_space_arrays(nref_, order_, bi_.data(), gn_.data(), rp_.data(), kc_.data()),
+          "pmx_host_space_arrays");
+  }
+  int nref_, order_;
+  long long dims_[8] = {0};
+  std::vector<int> bi_, gn_, rp_, kc_;
+};
+
+// CSR pattern of a space (TSquareStructure3D(space) followed by Sort()).
+class TSquareStructure3D {
+ public:
+  explicit TSquareStructure3D(TFESpace3D &space) : space_(space) {}
+  int GetN_Rows() const { return space_.GetN_DegreesOfFreedom(); }
+  int GetN_Columns() const { return space_.GetN_DegreesOfFreedom(); }
+  int GetN_Entries() const { return (int)space_.N_Entries(); }
+  int GetActiveBound() const { return space_.GetActiveBound(); }
+  const int *GetRowPtr() { return space_.RowPtr(); }
+  const int *GetKCol() { return space_.KCol(); }
+
+ private:
+  TFESpace3D &space_;
+};
+
+// TSystemCD3D: levels 0 .. N_levels-1 (coarsest first), solution and rhs
+// arrays of the finest level owned by the caller (SolArray / RhsArray).
+class TSystemCD3D {
+ public:
+  TSystemCD3D(Context &ctx, int N_levels, int nref_coarsest, int order, double *sol, double *rhs)
+      : sol_(sol), rhs_(rhs), nlev_(N_levels) {
+    check(pmx_system_create(ctx.handle(), nref_coarsest, N_levels, order, &h_), "pmx_system_create");
+    long long d[8];
+    check(pmx_system_dims(h_, -1, d), "pmx_system_dims");
+    n_dof_ = (int)d[0];
+    n_active_ = (int)d[1];
+  }
+  ~TSystemCD3D() { pmx_system_destroy(h_); }
+  TSystemCD3D(const TSystemCD3D &) = delete;
+  TSystemCD3D &operator=(const TSystemCD3D &) = delete;
+
+  // Init(BilinearCoeffs, BoundCondition, BoundValue, ...): the coefficient
+  // functions, the discretisation and the boundary data are the example and
+  // form fields of the parameter block.
+  void Init(const double *prm) { prm_.assign(prm, prm + PMX_PRM_LEN); }
+  // TSystemCD3D::Assemble (SystemCD3D.C:340-365): all levels on the device;
+  // the finest rhs goes to RhsArray and its Dirichlet values to SolArray.
+  void Assemble() {
+    if (prm_.empty()) throw Error(PMX_ERR_STATE, "Init first");
+    check(pmx_system_assemble(h_, prm_.data()), "pmx_system_assemble");
+    check(pmx_system_rhs(h_, -1, rhs_), "pmx_system_rhs");
+    for (int i = n_active_; i < n_dof_; i++) sol_[i] = rhs_[i];
+  }
+  // TSystemCD3D::Solve(sol, rhs) with the solver / MG options of opt (pmx.h);
+  // returns the number of iterations like TItMethod::Iterate.
+  int Solve(const double *opt, double *info = nullptr) {
+    double inf[8];
+    const int rc = pmx_system_solve(h_, opt, rhs_, sol_, inf);
+    if (info)
+      for (int i = 0; i < 8; i++) info[i] = inf[i];
+    check(rc, "pmx_system_solve");
+    return (int)inf[0];
+  }
+  // TSquareMatrix3D::GetEntries() of a level (CSR order of its structure)
+  void GetEntries(int level, double *entries) { check(pmx_matrix_entries(h_, level, 0, entries), "entries"); }
+  int GetN_DegreesOfFreedom() const { return n_dof_; }
+  int GetActiveBound() const { return n_active_; }
+  int GetN_Levels() const { return nlev_; }
+  pmx_system *handle() const { return h_; }
+
+ private:
+  pmx_system *h_ = nullptr;
+  double *sol_, *rhs_;
+  int nlev_, n_dof_ = 0, n_active_ = 0;
+  std::vector<double> prm_;
+};
+
+// TSystemTCD3D with the time loop of 3DPrograms/TCD3D_ParMooN.C:565-663:
+//   AssembleMRhs();                       once
+//   per step: AssembleARhs(); AssembleSystMat(); Solve(sol); RestoreMassMat();
+// The device performs AssembleARhs + AssembleSystMat + Solve as one step
+// (pmx_tcd_step) when Solve is called; RestoreMassMat has nothing to undo
+// because the mass matrix is kept apart.
+class TSystemTCD3D {
+ public:
+  TSystemTCD3D(Context &ctx, int N_levels, int nref_coarsest, int order, double *sol)
+      : sol_(sol) {
+    check(pmx_system_create(ctx.handle(), nref_coarsest, N_levels, order, &h_), "pmx_system_create");
+  }
+  ~TSystemTCD3D() { pmx_system_destroy(h_); }
+  TSystemTCD3D(const TSystemTCD3D &) = delete;
+  TSystemTCD3D &operator=(const TSystemTCD3D &) = delete;
+
+  void Init(const double *prm, const double *opt) {
+    prm_.assign(prm, prm + PMX_PRM_LEN);
+    opt_.assign(opt, opt + 24);
+  }
+  // mass matrix and rhs at the current time; sol (the initial solution) is
+  // taken as it is, with the Dirichlet values of the rhs
+  void AssembleMRhs() { check(pmx_tcd_init(h_, prm_.data(), sol_), "pmx_tcd_init"); }
+  // time and theta of the coming step (TDatabase::TimeDB CURRENTTIME,
+  // CURRENTTIMESTEPLENGTH, THETA1..4 after SetTimeDiscParameters)
+  void AssembleARhs(double time, double tau, double th1, double th2, double th3, double th4) {
+    prm_[PMX_PRM_TIME] = time;
+    td_[0] = tau; td_[1] = th1; td_[2] = th2; td_[3] = th3; td_[4] = th4;
+    pending_ = true;
+  }
+  void AssembleSystMat() {
+    if (!pending_) throw Error(PMX_ERR_STATE, "AssembleARhs first");
+  }
+  int Solve(double *info = nullptr) {
+    if (!pending_) throw Error(PMX_ERR_STATE, "AssembleARhs first");
+    double inf[8];
+    const int rc = pmx_tcd_step(h_, prm_.data(), td_, opt_.data(), inf);
+    pending_ = false;
+    if (info)
+      for (int i = 0; i < 8; i++) info[i] = inf[i];
+    check(rc, "pmx_tcd_step");
+    check(pmx_system_solution(h_, sol_), "pmx_system_solution");
+    return (int)inf[0];
+  }
+  void RestoreMassMat() {}
+  pmx_system *handle() const { return h_; }
+
+ private:
+  pmx_system *h_ = nullptr;
+  double *sol_;
+  std::vector<double> prm_, opt_;
+  double td_[5] = {0};
+  bool pending_ = false;
+};
+
+}  // namespace pmx
+
+#endif

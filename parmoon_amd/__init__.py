@@ -56,9 +56,12 @@ def lib():
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
         L.pmx_system_solution.argtypes = [vp, vp]
         L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
+        L.pmx_tcd_init.argtypes = [vp, vp, vp]
+        L.pmx_tcd_step.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_learned_nparams.argtypes = [vp]
         L.pmx_learned_init.argtypes = [vp, ctypes.c_uint]
         L.pmx_learned_params.argtypes = [vp, ip, vp]
+        L.pmx_learned_params_level.argtypes = [vp, ip, ip, vp]
         L.pmx_learned_forward.argtypes = [vp, ip, vp, vp]
         L.pmx_learned_gradient.argtypes = [vp, ip, vp, vp, vp, vp]
         L.pmx_learned_adam.argtypes = [vp, dp]
@@ -241,6 +244,22 @@ class System:
         _ok(lib().pmx_time_kernel(self.h, kernel, level, precision, reps, _p(ms)), "pmx_time_kernel")
         return float(ms[0])
 
+    # ---- time stepping (TSystemTCD3D) --------------------------------------
+    def tcd_init(self, prm_block, sol0=None):
+        s0 = None if sol0 is None else np.ascontiguousarray(sol0, float)
+        _ok(lib().pmx_tcd_init(self.h, _p(np.ascontiguousarray(prm_block, float)), _p(s0)), "pmx_tcd_init")
+
+    def tcd_step(self, prm_block, td, opt, check=True):
+        """One time step to prm_block[TIME]; td = (tau, theta1..theta4)."""
+        o = np.zeros(24)
+        o[:len(opt)] = opt
+        info = np.zeros(8)
+        rc = lib().pmx_tcd_step(self.h, _p(np.ascontiguousarray(prm_block, float)),
+                                _p(np.ascontiguousarray(td, float)), _p(o), _p(info))
+        if check:
+            _ok(rc, "pmx_tcd_step")
+        return info, rc
+
     # ---- learned post-smoother -------------------------------------------
     @staticmethod
     def learned_nparams():
@@ -251,16 +270,16 @@ class System:
     def learned_init(self, seed=0):
         _ok(lib().pmx_learned_init(self.h, seed), "pmx_learned_init")
 
-    def learned_params(self, params=None):
-        """Upload `params` (FP32) if given; return the current parameters."""
+    def learned_params(self, params=None, level=-1):
+        """Upload `params` (FP32) of a level's network if given; return them."""
         n = self.learned_nparams()
         if params is not None:
             p = np.ascontiguousarray(params, np.float32)
             if p.size != n:
                 raise ValueError(f"expected {n} parameters")
-            _ok(lib().pmx_learned_params(self.h, 1, _p(p)), "pmx_learned_params")
+            _ok(lib().pmx_learned_params_level(self.h, level, 1, _p(p)), "pmx_learned_params")
         out = np.zeros(n, np.float32)
-        _ok(lib().pmx_learned_params(self.h, 0, _p(out)), "pmx_learned_params")
+        _ok(lib().pmx_learned_params_level(self.h, level, 0, _p(out)), "pmx_learned_params")
         return out
 
     def learned_forward(self, u):
@@ -293,7 +312,7 @@ class System:
         o = np.zeros(24)
         o[:len(opt)] = opt
         t = np.array([samples, steps, lr, seed, levels, loss], float)
-        info = np.zeros(4)
+        info = np.zeros(16)
         _ok(lib().pmx_learned_train(self.h, _p(o), _p(t), _p(info)), "pmx_learned_train")
         return info
 

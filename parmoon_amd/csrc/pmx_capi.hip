@@ -73,6 +73,7 @@ struct Level {
   DevBuf<BlockInfo> blocks_all, blocks_colour;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
+  DevBuf<double> mass;                // time stepping: mass matrix M
   DevBuf<double> rhs;                 // assembled rhs (lattice order)
   Vecs<double> v64;
   Vecs<float> v32;
@@ -94,6 +95,8 @@ struct System {
   int nq = 0;
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
+  DevBuf<double> oldrhs, oldsol, tmp1, tmp2;  // time stepping (finest level)
+  bool tcd_ready = false;
   DevBuf<double> krylov64;
   DevBuf<float> krylov32;
   DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
@@ -108,14 +111,17 @@ struct System {
   };
   Graph graphs[2];
   bool use_graphs = true;
-  // learned post-smoother on the `learned_levels` finest levels (FP32 cycles)
-  LearnedNet net;
+  // learned post-smoother on the `learned_levels` finest levels (FP32 cycles),
+  // one network per level
+  static constexpr int kMaxLevels = 10;
+  LearnedNet nets[kMaxLevels];
   int learned_levels = 1;
+  int trained_levels = 1;  // levels of the last training
   bool learn_active = false;
   int fallbacks = 0;
   DevBuf<double> lscal;               // norm, (d, Ac), (Ac, Ac)
   ~System() {
-    net.release();
+    for (auto &n : nets) n.release();
     for (auto &g : graphs)
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
   }
@@ -246,9 +252,11 @@ static void ensure_val32(System &Y) {
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (lv.have_val32) continue;
-    Y.invalidate_graphs();
-    lv.v32.val.alloc(lv.L.nvals);
-    for (auto *b : {&lv.v32.x, &lv.v32.f, &lv.v32.d, &lv.v32.aux}) b->alloc(lv.L.N);
+    if (!lv.v32.val.p) {
+      Y.invalidate_graphs();
+      lv.v32.val.alloc(lv.L.nvals);
+      for (auto *b : {&lv.v32.x, &lv.v32.f, &lv.v32.d, &lv.v32.aux}) b->alloc(lv.L.N);
+    }
     convert<float, double>(lv.v32.val.p, lv.v64.val.p, lv.L.nvals, Y.ctx->st);
     lv.have_val32 = true;
   }
@@ -270,7 +278,7 @@ static void ensure_learned(System &Y) {
       learned_diag_inv(lv.D, lv.v32.val.p, lv.dinv.p, Y.ctx->st);
       lv.have_dinv = true;
     }
-    if (net_reserve(Y.net, lv.L.N, false)) Y.invalidate_graphs();
+    if (net_reserve(Y.nets[i], lv.L.N, false)) Y.invalidate_graphs();
   }
 }
 
@@ -300,7 +308,7 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
 // alpha = max(0, (d, Ac) / (Ac, Ac)), so the level defect never grows in the
 // 2-norm (the step-length idea of TMGLevel3D::StepLengthControl,
 // MGLevel3D.C:614-863, applied to the network's correction).
-static void learned_post(System &Y, Level &lv) {
+static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   hipStream_t st = Y.ctx->st;
   Vecs<float> &v = lv.v32;
   const int64_t N = lv.L.N;
@@ -310,9 +318,9 @@ static void learned_post(System &Y, Level &lv) {
   boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
   learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
   dot<float>(lv.lc.p, lv.lc.p, N, part, sc, 1, st);
-  learned_scale_input(Y.net, lv.lc.p, sc, sq, N, st);
-  net_forward_act(Y.net, lv.L.M, false, st);
-  learned_output(Y.net, lv.L.M, sc, sq, lv.lc.p, st);
+  learned_scale_input(net, lv.lc.p, sc, sq, N, st);
+  net_forward_act(net, lv.L.M, false, st);
+  learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
   boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
   dot<float>(v.d.p, lv.lac.p, N, part, sc + 1, 0, st);
@@ -352,7 +360,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learn_active && i >= (int)Y.lev.size() - Y.learned_levels) {
-      learned_post(Y, lv);
+      learned_post(Y, lv, Y.nets[i]);
       return;
     }
   }
@@ -625,18 +633,18 @@ static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double re
 // u (FP32 lattice): c = mask(net(u)), loss = mean over interior nodes of
 // (u - D^-1 A c)^2 -- the part of the scaled defect that the correction
 // leaves.  Accumulates into net.grads; the loss goes to *loss_dev.
-static void learned_res_step(System &Y, Level &lv, const float *u, float *loss_dev) {
+static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float *u, float *loss_dev) {
   hipStream_t st = Y.ctx->st;
   const int M = lv.L.M;
   Vecs<float> &v = lv.v32;
-  net_forward(Y.net, M, u, true, st);
-  learned_mask(Y.net.out, M, lv.lc.p, st);
+  net_forward(net, M, u, true, st);
+  learned_mask(net.out, M, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
   boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
-  learned_res_loss(Y.net, M, u, lv.dinv.p, lv.lac.p, v.aux.p, st);
+  learned_res_loss(net, M, u, lv.dinv.p, lv.lac.p, v.aux.p, st);
   css_transpose(lv.D, v.val.p, v.aux.p, v.d.p, st);
-  net_backward_from(Y.net, M, v.d.p, st);
-  learned_loss_sum(Y.net, loss_dev, st);
+  net_backward_from(net, M, v.d.p, st);
+  learned_loss_sum(net, loss_dev, st);
 }
 
 // Training of the learned post-smoother on (input, target) pairs from plain
@@ -644,7 +652,7 @@ static void learned_res_step(System &Y, Level &lv, const float *u, float *loss_d
 // interior start x0, then pre-smoothing and the coarse-grid correction of one
 // cycle on the finest level give x; the ideal post-correction is -x, the
 // network input the scaled D^-1 (f - A x).  Adam on the mean squared error.
-// topt: [0] samples per level, [1] steps (alternating over the levels),
+// topt: [0] samples per level, [1] Adam steps per level (levels interleaved),
 // [2] learning rate, [3] seed, [4] finest levels using the network (default:
 // all but the coarsest), [5] loss: 0 squared error against -x, 1 the scaled
 // defect left after the correction (learned_res_step).  tinfo: [0] mean loss
@@ -659,18 +667,23 @@ static void learned_train(System &Y, const double *to, double *ti) {
   Y.learned_levels = to[4] > 0 ? std::min((int)to[4], L) : L;
   const bool residual_loss = to[5] != 0.0;
   const int nl = Y.learned_levels;
+  Y.trained_levels = nl;
   Y.learn_active = false;
   ensure_learned(Y);
-  if (!Y.net.ready) net_init(Y.net, seed ^ 0x5eedu, st);
-  if (net_reserve(Y.net, Y.lev[L]->L.N, true)) Y.invalidate_graphs();
+  for (int j = 0; j < nl; j++) {
+    LearnedNet &net = Y.nets[L - j];
+    if (!net.ready) net_init(net, (seed ^ 0x5eedu) + 31u * (uint32_t)j, st);
+    if (net_reserve(net, Y.lev[L - j]->L.N, true)) Y.invalidate_graphs();
+  }
   // samples of level L - j at off[j] + k N_j
   std::vector<size_t> off(nl + 1, 0);
   for (int j = 0; j < nl; j++) off[j + 1] = off[j] + (size_t)ns * Y.lev[L - j]->L.N;
   DevBuf<float> du, dt;
   du.alloc(off[nl]);
   if (!residual_loss) dt.alloc(off[nl]);
+  const int total = steps * nl;
   DevBuf<float> losses;
-  losses.alloc(steps);
+  losses.alloc(total);
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
@@ -694,19 +707,20 @@ static void learned_train(System &Y, const double *to, double *ti) {
       if (!residual_loss) learned_target(v.x.p, M, Y.lscal.p, sq, -1.0f, dt.p + off[j] + (size_t)k * N, st);
     }
   }
-  for (int k = 0; k < steps; k++) {
-    const int j = k % nl, smp = (k / nl) % ns;
+  for (int k = 0; k < total; k++) {
+    const int j = k % nl, kk = k / nl, smp = kk % ns;
     Level &lv = *Y.lev[L - j];
+    LearnedNet &net = Y.nets[L - j];
     const size_t o = off[j] + (size_t)smp * lv.L.N;
-    const double lrk = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * k / std::max(1, steps - 1))));
-    net_zero_grads(Y.net, st);
+    const double lrk = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * kk / std::max(1, steps - 1))));
+    net_zero_grads(net, st);
     if (residual_loss) {
-      learned_res_step(Y, lv, du.p + o, losses.p + k);
+      learned_res_step(Y, lv, net, du.p + o, losses.p + k);
     } else {
-      net_forward(Y.net, lv.L.M, du.p + o, true, st);
-      net_backward(Y.net, lv.L.M, dt.p + o, losses.p + k, st);
+      net_forward(net, lv.L.M, du.p + o, true, st);
+      net_backward(net, lv.L.M, dt.p + o, losses.p + k, st);
     }
-    net_adam(Y.net, (float)lrk, st);
+    net_adam(net, (float)lrk, st);
   }
   check(hipEventRecord(e1, st), "event");
   check(hipEventSynchronize(e1), "event");
@@ -714,12 +728,21 @@ static void learned_train(System &Y, const double *to, double *ti) {
   check(hipEventElapsedTime(&ms, e0, e1), "event");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  std::vector<float> h(steps);
-  check(hipMemcpy(h.data(), losses.p, sizeof(float) * steps, hipMemcpyDeviceToHost), "losses");
-  const int w = std::min(10 * nl, steps);
-  double a = 0, b = 0;
-  for (int k = 0; k < w; k++) { a += h[k]; b += h[steps - 1 - k]; }
-  if (ti) { ti[0] = a / w; ti[1] = b / w; ti[2] = ms * 1e-3; ti[3] = steps; }
+  std::vector<float> h(total);
+  check(hipMemcpy(h.data(), losses.p, sizeof(float) * total, hipMemcpyDeviceToHost), "losses");
+  // mean loss of the first / last 10 steps of each level's network
+  if (ti) {
+    const int w = std::min(10, steps);
+    for (int j = 0; j < nl && 4 + 2 * j + 1 < 16; j++) {
+      double a = 0, b = 0;
+      for (int q = 0; q < w; q++) { a += h[(size_t)q * nl + j]; b += h[(size_t)(steps - 1 - q) * nl + j]; }
+      if (j == 0) { ti[0] = a / w; ti[1] = b / w; }
+      ti[4 + 2 * j] = a / w;
+      ti[5 + 2 * j] = b / w;
+    }
+    ti[2] = ms * 1e-3;
+    ti[3] = total;
+  }
   Y.invalidate_graphs();
 }
 
@@ -758,8 +781,9 @@ static int solve_device(System &Y, double *info) {
     ensure_val32(Y);
     Y.fallbacks = 0;
     if (prec == 2) {
-      if (!Y.net.ready) throw std::logic_error("learned smoother: train or load parameters first");
-      if (Y.opt[19] > 0) Y.learned_levels = std::min((int)Y.opt[19], (int)Y.lev.size() - 1);
+      Y.learned_levels = std::min(Y.opt[19] > 0 ? (int)Y.opt[19] : Y.trained_levels, (int)Y.lev.size() - 1);
+      for (int i = (int)Y.lev.size() - Y.learned_levels; i < (int)Y.lev.size(); i++)
+        if (!Y.nets[i].ready) throw std::logic_error("learned smoother: train or load parameters first");
       ensure_learned(Y);
       Y.learn_active = true;
     }
@@ -1029,6 +1053,82 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
     Y.assembled = true;
   })
 }
+// ---- time stepping: TSystemTCD3D (src/System/SystemTCD3D.C:208-428) driven as
+// 3DPrograms/TCD3D_ParMooN.C:565-663.  The mass matrix is kept apart, so the
+// system matrix M + tau theta1 A is rebuilt each step and no RestoreMassMat
+// pass is needed.
+static void assemble_into(System &Y, const double *prm, bool mass) {
+  hipStream_t st = Y.ctx->st;
+  Y.prm.upload(prm, PMX_PRM_LEN, st);
+  std::vector<double> qref, tab;
+  int nq;
+  make_tables(Y.order, prm[PMX_PRM_LINEAR] != 0, qref, tab, nq);
+  Y.qref.upload(qref.data(), qref.size(), st);
+  Y.tab.upload(tab.data(), tab.size(), st);
+  Y.nq = nq;
+  Y.qpbuf.alloc((size_t)Y.lev.back()->Nb.ncells * nq * 8);
+  for (auto &lp : Y.lev) {
+    Level &lv = *lp;
+    if (mass) lv.mass.alloc(lv.L.nvals);
+    assemble(lv.D, Y.prm.p, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
+  }
+}
+
+int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!prm) throw std::invalid_argument("prm is NULL");
+    hipStream_t st = Y.ctx->st;
+    std::vector<double> p(prm, prm + PMX_PRM_LEN);
+    p[PMX_PRM_FORM] = PMX_FORM_MASS;
+    assemble_into(Y, p.data(), true);  // AssembleMRhs (SystemTCD3D.C:208-227)
+    Level &lv = *Y.lev.back();
+    const int64_t N = lv.L.N;
+    for (auto *b : {&Y.oldrhs, &Y.oldsol, &Y.tmp1, &Y.tmp2}) b->alloc(N);
+    check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+    if (sol0) to_lattice<double>(Y, lv, sol0, Y.sol.p);
+    else interpolate(lv.D, Y.prm.p, Y.sol.p, st);
+    boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // Dirichlet values of the rhs
+    check(hipStreamSynchronize(st), "tcd init");
+    Y.tcd_ready = true;
+  })
+}
+int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const double *opt, double *info) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!prm || !td || !opt) throw std::invalid_argument("NULL argument");
+    if (!Y.tcd_ready) throw std::logic_error("pmx_tcd_init first");
+    const int form = (int)prm[PMX_PRM_FORM];
+    if (form != PMX_FORM_TCD_A && form != PMX_FORM_GALERKIN && form != PMX_FORM_SDFEM)
+      throw std::invalid_argument("time stepping takes a one-matrix stiffness form");
+    hipStream_t st = Y.ctx->st;
+    const double tau = td[0], th1 = td[1], th2 = td[2], th3 = td[3], th4 = td[4];
+    Level &lv = *Y.lev.back();
+    const int64_t N = lv.L.N;
+    check(hipMemcpyAsync(Y.oldsol.p, Y.sol.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+    assemble_into(Y, prm, false);                                // AssembleARhs (SystemTCD3D.C:229-246)
+    boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // sol Dirichlet = rhs
+    // AssembleSystMat: B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol
+    css_apply<double>(lv.D, 0, lv.mass.p, Y.oldsol.p, nullptr, Y.tmp1.p, 0.0, -1, st);
+    css_apply<double>(lv.D, 0, lv.v64.val.p, Y.oldsol.p, nullptr, Y.tmp2.p, 0.0, -1, st);
+    tcd_rhs(lv.D, Y.oldrhs.p, lv.rhs.p, Y.tmp1.p, Y.tmp2.p, tau * th3, tau * th4, tau * th2, Y.b.p, st);
+    // system matrices M + tau th1 A on every level
+    for (auto &lp : Y.lev) {
+      Level &l = *lp;
+      mass_plus(l.v64.val.p, l.mass.p, tau * th1, l.L.nvals, st);
+      l.have_val32 = false;
+      l.have_dinv = false;
+    }
+    check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+    check(hipStreamSynchronize(st), "tcd assemble");
+    coarse_inverse(Y, *Y.lev.front());
+    Y.assembled = true;
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    const int rc = solve_device(Y, info);
+    if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+
 int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
   PMX_TRY({
     System &Y = sys->s;
@@ -1213,15 +1313,15 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 5: if (li > 0) prolongate<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<float>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v32.f.p, v.d.p, st); break;
           case 7: mg_prec<float>(Y, v.aux.p, v.d.p); break;
-          case 8: learned_post(Y, lv); break;
-          case 9: net_forward_act(Y.net, lv.L.M, false, st); break;
+          case 8: learned_post(Y, lv, Y.nets[li]); break;
+          case 9: net_forward_act(Y.nets[li], lv.L.M, false, st); break;
           default: break;
         }
       }
     };
     if (kernel >= 8) {
       if (!precision) throw std::invalid_argument("the learned smoother runs in FP32");
-      if (!Y.net.ready) throw std::logic_error("learned smoother: train or load parameters first");
+      if (!Y.nets[li].ready) throw std::logic_error("learned smoother: train or load parameters first");
       const int keep = Y.learned_levels;
       Y.learned_levels = (int)Y.lev.size() - li;
       ensure_learned(Y);
@@ -1249,41 +1349,48 @@ int pmx_learned_nparams(int *n) {
 int pmx_learned_init(pmx_system *sys, unsigned seed) {
   PMX_TRY({
     System &Y = sys->s;
-    net_init(Y.net, seed, Y.ctx->st);
+    for (int i = 1; i < (int)Y.lev.size(); i++) net_init(Y.nets[i], seed + 31u * (uint32_t)i, Y.ctx->st);
     check(hipStreamSynchronize(Y.ctx->st), "sync");
     Y.invalidate_graphs();
   })
 }
-int pmx_learned_params(pmx_system *sys, int set, float *params) {
+int pmx_learned_params_level(pmx_system *sys, int level, int set, float *params) {
   PMX_TRY({
     System &Y = sys->s;
     if (!params) throw std::invalid_argument("NULL argument");
+    (void)level_of(Y, level);
+    const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    LearnedNet &net = Y.nets[li];
     hipStream_t st = Y.ctx->st;
     const size_t bytes = sizeof(float) * net_total_params();
     if (set) {
-      if (!Y.net.ready) net_init(Y.net, 0, st);
-      check(hipMemcpyAsync(Y.net.params, params, bytes, hipMemcpyHostToDevice, st), "h2d params");
-      net_sync_bf16(Y.net, st);
+      if (!net.ready) net_init(net, 0, st);
+      check(hipMemcpyAsync(net.params, params, bytes, hipMemcpyHostToDevice, st), "h2d params");
+      net_sync_bf16(net, st);
       Y.invalidate_graphs();
     } else {
-      if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
-      check(hipMemcpyAsync(params, Y.net.params, bytes, hipMemcpyDeviceToHost, st), "d2h params");
+      if (!net.ready) throw std::logic_error("learned smoother: no parameters");
+      check(hipMemcpyAsync(params, net.params, bytes, hipMemcpyDeviceToHost, st), "d2h params");
     }
     check(hipStreamSynchronize(st), "sync");
   })
 }
+int pmx_learned_params(pmx_system *sys, int set, float *params) {
+  return pmx_learned_params_level(sys, -1, set, params);
+}
 int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out) {
   PMX_TRY({
     System &Y = sys->s;
+    LearnedNet &net = Y.nets[Y.lev.size() - 1];
     if (!in || !out || M < 3 || M > 1025) throw std::invalid_argument("bad lattice or NULL argument");
-    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    if (!net.ready) throw std::logic_error("learned smoother: no parameters");
     hipStream_t st = Y.ctx->st;
     const int64_t N = (int64_t)M * M * M;
-    if (net_reserve(Y.net, N, false)) Y.invalidate_graphs();
+    if (net_reserve(net, N, false)) Y.invalidate_graphs();
     DevBuf<float> din;
     din.upload(in, N, st);
-    net_forward(Y.net, M, din.p, false, st);
-    check(hipMemcpyAsync(out, Y.net.out, sizeof(float) * N, hipMemcpyDeviceToHost, st), "d2h");
+    net_forward(net, M, din.p, false, st);
+    check(hipMemcpyAsync(out, net.out, sizeof(float) * N, hipMemcpyDeviceToHost, st), "d2h");
     check(hipStreamSynchronize(st), "sync");
   })
 }
@@ -1291,20 +1398,21 @@ int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *t
                          double *loss) {
   PMX_TRY({
     System &Y = sys->s;
+    LearnedNet &net = Y.nets[Y.lev.size() - 1];
     if (!in || !target || M < 3 || M > 1025) throw std::invalid_argument("bad lattice or NULL argument");
-    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
+    if (!net.ready) throw std::logic_error("learned smoother: no parameters");
     hipStream_t st = Y.ctx->st;
     const int64_t N = (int64_t)M * M * M;
-    if (net_reserve(Y.net, N, true)) Y.invalidate_graphs();
+    if (net_reserve(net, N, true)) Y.invalidate_graphs();
     DevBuf<float> din, dtg, dl;
     din.upload(in, N, st);
     dtg.upload(target, N, st);
     dl.alloc(1);
-    net_zero_grads(Y.net, st);
-    net_forward(Y.net, M, din.p, true, st);
-    net_backward(Y.net, M, dtg.p, dl.p, st);
+    net_zero_grads(net, st);
+    net_forward(net, M, din.p, true, st);
+    net_backward(net, M, dtg.p, dl.p, st);
     if (grads)
-      check(hipMemcpyAsync(grads, Y.net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
+      check(hipMemcpyAsync(grads, net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
             "d2h grads");
     float l = 0;
     check(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, st), "d2h loss");
@@ -1317,22 +1425,23 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
     System &Y = sys->s;
     if (!Y.assembled) throw std::logic_error("assemble first");
     if (!in) throw std::invalid_argument("NULL argument");
-    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
     Level &lv = level_of(Y, level);
     const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    LearnedNet &net = Y.nets[li];
+    if (!net.ready) throw std::logic_error("learned smoother: no parameters");
     hipStream_t st = Y.ctx->st;
     const int keep = Y.learned_levels;
     Y.learned_levels = (int)Y.lev.size() - li;
     ensure_learned(Y);
     Y.learned_levels = keep;
-    if (net_reserve(Y.net, lv.L.N, true)) Y.invalidate_graphs();
+    if (net_reserve(net, lv.L.N, true)) Y.invalidate_graphs();
     DevBuf<float> din, dl;
     din.upload(in, lv.L.N, st);
     dl.alloc(1);
-    net_zero_grads(Y.net, st);
-    learned_res_step(Y, lv, din.p, dl.p);
+    net_zero_grads(net, st);
+    learned_res_step(Y, lv, net, din.p, dl.p);
     if (grads)
-      check(hipMemcpyAsync(grads, Y.net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
+      check(hipMemcpyAsync(grads, net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),
             "d2h grads");
     float l = 0;
     check(hipMemcpyAsync(&l, dl.p, sizeof(float), hipMemcpyDeviceToHost, st), "d2h loss");
@@ -1343,8 +1452,9 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
 int pmx_learned_adam(pmx_system *sys, double lr) {
   PMX_TRY({
     System &Y = sys->s;
-    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
-    net_adam(Y.net, (float)lr, Y.ctx->st);
+    LearnedNet &net = Y.nets[Y.lev.size() - 1];
+    if (!net.ready) throw std::logic_error("learned smoother: no parameters");
+    net_adam(net, (float)lr, Y.ctx->st);
     check(hipStreamSynchronize(Y.ctx->st), "sync");
     Y.invalidate_graphs();
   })
@@ -1363,16 +1473,16 @@ int pmx_learned_post(pmx_system *sys, int level, double *sol, const double *f) {
   PMX_TRY({
     System &Y = sys->s;
     if (!Y.assembled) throw std::logic_error("assemble first");
-    if (!Y.net.ready) throw std::logic_error("learned smoother: no parameters");
     Level &lv = level_of(Y, level);
     const int li = level < 0 ? level + (int)Y.lev.size() : level;
+    if (!Y.nets[li].ready) throw std::logic_error("learned smoother: no parameters");
     const int keep = Y.learned_levels;
     Y.learned_levels = (int)Y.lev.size() - li;
     ensure_learned(Y);
     Y.learned_levels = keep;
     to_lattice<float>(Y, lv, sol, lv.v32.x.p);
     to_lattice<float>(Y, lv, f, lv.v32.f.p);
-    learned_post(Y, lv);
+    learned_post(Y, lv, Y.nets[li]);
     from_lattice<float>(Y, lv, lv.v32.x.p, sol);
   })
 }

@@ -42,8 +42,8 @@ int node_colour(const Layout &L, int x, int y, int z) {
 }
 
 void build_layout(Layout &L, int nref, int order, int block) {
-  if (order != 1 && order != 2) throw std::runtime_error("order must be 1 or 2");
-  if (nref < 1 || nref > 10) throw std::runtime_error("nref out of range");
+  if (order != 1 && order != 2) throw std::invalid_argument("order must be 1 or 2");
+  if (nref < 1 || nref > 10) throw std::invalid_argument("nref out of range");
   L.nref = nref;
   L.order = order;
   L.n = 1 << nref;

@@ -633,6 +633,46 @@ void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int
                      D.nboundary, D.M, prm_dev, rhs);
 }
 
+// ============================================================ time stepping
+// nodal interpolation of the example's exact (initial) function on the lattice
+// (TFEFunction3D::Interpolate for Lagrange nodes)
+__global__ void k_interpolate(const double *__restrict__ prm, int M, double *__restrict__ x) {
+  const int64_t N = (int64_t)M * M * M;
+  const double h = 1.0 / (M - 1);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = (int)(i % M), b = (int)((i / M) % M), c = (int)(i / ((int64_t)M * M));
+    x[i] = example_exact(prm, a * h, b * h, c * h);
+  }
+}
+// val = mass + c * val (MatAdd of Blas2.C:128-150 on the stored stencil values)
+__global__ void k_mass_plus(double *__restrict__ val, const double *__restrict__ mass, double c, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    val[i] = mass[i] + c * val[i];
+}
+// right-hand side of TSystemTCD3D::AssembleSystMat (SystemTCD3D.C:255-283):
+// active rows B = t3 oldrhs + t4 rhs + (M - t2 A) oldsol, Dirichlet rows B = rhs
+__global__ void k_tcd_rhs(int M, const double *__restrict__ oldrhs, const double *__restrict__ rhs,
+                          const double *__restrict__ m_old, const double *__restrict__ a_old, double t3, double t4,
+                          double t2, double *__restrict__ B) {
+  const int64_t N = (int64_t)M * M * M;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = (int)(i % M), b = (int)((i / M) % M), c = (int)(i / ((int64_t)M * M));
+    const bool interior = a > 0 && b > 0 && c > 0 && a < M - 1 && b < M - 1 && c < M - 1;
+    B[i] = interior ? (t3 * oldrhs[i] + t4 * rhs[i]) + (m_old[i] - t2 * a_old[i]) : rhs[i];
+  }
+}
+void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st) {
+  hipLaunchKernelGGL(k_interpolate, dim3(grid_for(D.N)), dim3(kBlock), 0, st, prm_dev, D.M, x);
+}
+void mass_plus(double *val, const double *mass, double c, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_mass_plus, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, c, n);
+}
+void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
+             double t3, double t4, double t2, double *B, hipStream_t st) {
+  hipLaunchKernelGGL(k_tcd_rhs, dim3(grid_for(D.N)), dim3(kBlock), 0, st, D.M, oldrhs, rhs, m_old, a_old, t3, t4,
+                     t2, B);
+}
+
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st) {
   hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, st, H, cs, sn, s, i, ld, res);
 }

@@ -56,6 +56,12 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
                 const T *val, double *out, hipStream_t st);
 void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int nq, const double *tab,
               double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st);
+// time stepping (TSystemTCD3D): nodal interpolation of the example's initial
+// function, val = mass + c val, and the Crank-Nicolson / theta right-hand side
+void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st);
+void mass_plus(double *val, const double *mass, double c, int64_t n, hipStream_t st);
+void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
+             double t3, double t4, double t2, double *B, hipStream_t st);
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st);
 void backsolve(const double *H, const double *s, double *y, int k, int ld, hipStream_t st);
 template <typename T, typename X>
