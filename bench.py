@@ -5,13 +5,17 @@ eps = 1e-4, b = curl of a seeded Fourier potential with max|b| = 1, f = 1,
 homogeneous Dirichlet, 27-point rule), 5 GMG levels 4^3..64^3.  A step
 assembles every level on the device and solves A x = b to FP64 relative
 residual 1e-10 with V(2,2) multicolour-SSOR GMG preconditioning FGMRES(30):
-  * plain form   : FP64 operator, MG and Krylov vectors;
-  * reduced form : FP32 operator/MG/Krylov vectors inside an FP64 defect
-                   correction (FP64 residual, Hessenberg/Givens, coarse solve).
-The metric is the reduced form's time to 1e-10 (seconds, lower is better),
-with the plain form, assembly DOF/s and the SpMV roofline alongside.
+  * fp64    : FP64 operator, MG and Krylov vectors (the plain method);
+  * fp32    : FP32 operator/MG/Krylov vectors inside an FP64 defect
+              correction (FP64 residual, Hessenberg/Givens, coarse solve);
+  * learned : fp32 with the learned post-smoother on the finest levels
+              (trained once before the timed steps; training time reported
+              separately, not part of the step).
+The metric is the learned form's time to 1e-10 (seconds, lower is better),
+with the plain FP64 time, assembly DOF/s and the SpMV roofline alongside.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nref 6] [--no-cpu]
+    python bench.py --profile-kernels     # SpMV + a 1 GiB copy only (rocprofv3 --pmc runs)
 """
 import argparse
 import json
@@ -25,6 +29,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# HBM bytes per FP32 SpMV launch at 64^3 Q2 from rocprofv3 FETCH_SIZE + WRITE_SIZE,
+# scaled by the same counters of a copy of known size (None until measured)
+TRAFFIC_BYTES = None
 
 
 def cpu_baseline(nref_cpu=4, seed=2024):
@@ -58,6 +65,21 @@ def cpu_baseline(nref_cpu=4, seed=2024):
             "iters": int(info[0]), "final_rel_res": float(info[1] / r0)}
 
 
+def profile_kernels(S, nref):
+    """Short run for hardware-counter passes: 5 FP32 SpMV launches on the
+    finest level and 5 copies of 1 GiB (the calibration kernel of known size)."""
+    import torch
+    S.time_kernel(0, -1, 1, reps=5)
+    a = torch.empty(2 ** 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"[bench] profile-kernels: 5 x SpMV 2^{nref} Q2 FP32, 5 x copy of {a.numel() * 4} bytes",
+          file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,7 +87,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nref", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,fp64")
+    ap.add_argument("--forms", default="learned,fp32,fp64")
+    ap.add_argument("--learned-levels", type=int, default=1)
+    ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--profile-kernels", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -82,20 +107,35 @@ def main():
     d = S.dims(-1)
     N = int(d[0])
     print(f"[bench] 2^{nref} Q2: N={N} stencil values={int(d[6])}", file=sys.stderr, flush=True)
+    S.assemble(prm)
+    if args.profile_kernels:
+        profile_kernels(S, nref)
+        return
 
-    def opt(precision):
+    def opt(precision, learned_levels=0):
         # res_norm_min is set per solve from the initial residual (relative 1e-10)
         return P.solver_opt(solver=16, restart=30, maxit=400, res_norm_min=0.0, red_factor=0.0,
-                            smoother=3, omega=1.0, pre=2, post=2, precision=precision, rel_target=1e-10)
+                            smoother=3, omega=1.0, pre=2, post=2, precision=precision, rel_target=1e-10,
+                            learned_levels=learned_levels)
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    prec_of = {"fp64": 0, "fp32": 1}
+    prec_of = {"fp64": 0, "fp32": 1, "learned": 2}
+    train = None
+    if "learned" in forms:
+        t0 = time.perf_counter()
+        tinfo = S.learned_train(opt(1), samples=16, steps=args.train_steps, lr=2e-3, seed=1,
+                                levels=args.learned_levels)
+        train = {"seconds": round(float(tinfo[2]), 3), "wall_s": round(time.perf_counter() - t0, 3),
+                 "adam_steps": int(tinfo[3]), "levels": args.learned_levels,
+                 "loss_first": round(float(tinfo[0]), 4), "loss_last": round(float(tinfo[1]), 4),
+                 "samples_per_level": 16, "loss": "scaled defect left after the correction"}
+        print(f"[bench] learned smoother trained: {train}", file=sys.stderr, flush=True)
 
     def step(form):
         t0 = time.perf_counter()
         S.assemble(prm)
         t1 = time.perf_counter()
-        info, rc = S.solve_resident(opt(prec_of[form]), check=False)
+        info, rc = S.solve_resident(opt(prec_of[form], args.learned_levels), check=False)
         t2 = time.perf_counter()
         if rc != 0:
             print(f"[bench] {form} solve rc={rc} info={info}", file=sys.stderr, flush=True)
@@ -113,10 +153,10 @@ def main():
             ts_sol.append(info[4])
             infos.append(info)
             print(f"[bench] {form} step {k}: assembly {ta:.4f}s solve {info[4]:.4f}s (wall {tsol:.4f}s) "
-                  f"iters {int(info[0])} outer {int(info[3])} rel.res {info[1] / info[2]:.2e}",
-                  file=sys.stderr, flush=True)
+                  f"iters {int(info[0])} outer {int(info[3])} rel.res {info[1] / info[2]:.2e} "
+                  f"fallbacks {int(info[5])}", file=sys.stderr, flush=True)
         res[form] = {"t_asm": float(np.median(ts_asm)), "t_solve": float(np.median(ts_sol)),
-                     "iters": int(infos[-1][0]), "outer": int(infos[-1][3]),
+                     "iters": int(infos[-1][0]), "outer": int(infos[-1][3]), "fallbacks": int(infos[-1][5]),
                      "rel_res": float(infos[-1][1] / infos[-1][2]), "rc": int(rc)}
 
     # SpMV roofline on the finest level (FP32 and FP64 stencil values)
@@ -129,12 +169,18 @@ def main():
         roof[prec] = (bytes_alg / (ms * 1e-3) / 1e9, ms, bytes_alg)
     kern = {name: S.time_kernel(k, -1, 1, reps=10) for name, k in
             (("ssor_sweep_fp32_ms", 3), ("vcycle_fp32_ms", 7), ("restrict_fp32_ms", 6), ("prolong_fp32_ms", 5))}
+    kern["assembly_finest_ms"] = S.time_kernel(4, -1, 0, reps=3)
+    if train is not None:
+        kern["learned_post_finest_ms"] = S.time_kernel(8, -1, 1, reps=10)
+        kern["net_forward_finest_ms"] = S.time_kernel(9, -1, 1, reps=10)
 
-    main_form = "fp32" if "fp32" in res else forms[0]
+    main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
     t_main = res[main_form]["t_solve"]
     t_asm = res[main_form]["t_asm"]
     steps_ms = (res[main_form]["t_asm"] + res[main_form]["t_solve"]) * 1e3
     achieved, ms_spmv, traffic_alg = roof[1]
+    t64 = res.get("fp64", {}).get("t_solve")
+    flops_asm = 180792.0 * (2 ** nref) ** 3  # Q2, 27-point rule, B^T D B model per cell
     out = {
         "metric": "time-to-1e-10 FP64 residual (s), learned+FP32 vs plain FP64 GMG; assembly DOF/s",
         "value": round(t_main, 5),
@@ -144,24 +190,28 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-in-f64" if main_form == "fp32" else "f64",
+        "dtype": {"learned": "f32-in-f64 + bf16 net", "fp32": "f32-in-f64", "fp64": "f64"}[main_form],
         "data": "synthetic (seeded Fourier convection field)",
-        "config": {"workload": f"CD3D SUPG Q2 {2 ** nref}^3 (config 2), 5-level GMG V(2,2) multicolour SSOR, "
-                               f"FGMRES(30), FP64 rel. residual 1e-10", "dofs": N, "parallelism": "1 GPU"},
-        "plain_fp64_time_s": res.get("fp64", {}).get("t_solve"),
+        "config": {"workload": f"CD3D SUPG Q2 {2 ** nref}^3, eps 1e-4, curl-Fourier b, 5-level GMG V(2,2) "
+                               f"multicolour SSOR, FGMRES(30), FP64 rel. residual 1e-10", "dofs": N,
+                   "form": main_form, "parallelism": "1 GPU"},
+        "plain_fp64_time_s": t64,
         "fp32_time_s": res.get("fp32", {}).get("t_solve"),
-        "speedup_fp32_vs_fp64": (res["fp64"]["t_solve"] / res["fp32"]["t_solve"]) if "fp64" in res and "fp32" in res else None,
-        "learned_smoother": "not in this measurement",
+        "learned_time_s": res.get("learned", {}).get("t_solve"),
+        "speedup_vs_plain_fp64": round(t64 / t_main, 3) if t64 else None,
+        "learned_training": train,
         "assembly_dofs_per_s": round(N / t_asm, 1),
         "assembly_s": round(t_asm, 5),
+        "assembly_kernels_gflops": round(flops_asm / (kern["assembly_finest_ms"] * 1e-3) / 1e9, 1),
         "iterations": {f: res[f]["iters"] for f in res},
+        "fallbacks": {f: res[f]["fallbacks"] for f in res},
         "final_rel_residual": {f: res[f]["rel_res"] for f in res},
         "spmv_fp64_GBs": round(roof[0][0], 1),
         "kernels_ms": {"spmv_fp32": round(ms_spmv, 4), "spmv_fp64": round(roof[0][1], 4),
                        **{k: round(v, 4) for k, v in kern.items()}},
-        "roofline": {"bound": "hbm", "kernel": "k_css<float,0> (stencil SpMV, finest level)",
+        "roofline": {"bound": "hbm", "kernel": "k_rows<float,0> (stencil SpMV, finest level)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES,
                      "algorithmic_bytes": int(traffic_alg)},
     }
     if not args.no_cpu:

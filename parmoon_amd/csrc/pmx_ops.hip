@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <stdexcept>
 
@@ -554,7 +555,7 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
   const BlockInfo *blk = colour < 0 ? D.blocks_all : D.blocks_colour + D.colour_off[colour];
   const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
-  switch (mode) {
+switch (mode) {
     case 0: hipLaunchKernelGGL((k_rows<T, 0>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
     case 1: hipLaunchKernelGGL((k_rows<T, 1>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
     case 2: hipLaunchKernelGGL((k_rows<T, 2>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
