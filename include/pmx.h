@@ -117,7 +117,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
  *  [18] 1 = launch the MG cycle kernel by kernel instead of replaying its HIP graph
  *  [19] precision 2: number of finest levels with the learned post-smoother
- *       (0 = the levels it was trained for)
+ *       (0 = the levels it was trained for); each of the opt[7] post-sweeps
+ *       of those levels becomes one learned step
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

@@ -71,6 +71,7 @@ struct Level {
   DevBuf<int> boundary, segidx;
   DevBuf<DevSegment> segs;
   DevBuf<BlockInfo> blocks_all, blocks_colour;
+  DevBuf<int> colour_off;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
   DevBuf<double> mass;                // time stepping: mass matrix M
@@ -165,6 +166,7 @@ static void setup_level(System &Y, Level &lv, int nref) {
   }
   D.colour_off[L.ncolours] = (int)bc.size();
   lv.blocks_colour.upload(bc.data(), bc.size(), st);
+  lv.colour_off.upload(D.colour_off.data(), D.colour_off.size(), st);
   lv.boundary.upload(L.boundary.data(), L.boundary.size(), st);
   lv.lat2dof.upload(lv.Nb.lat2dof.data(), lv.Nb.lat2dof.size(), st);
   lv.dof2lat.upload(lv.Nb.dof2lat.data(), lv.Nb.dof2lat.size(), st);
@@ -172,6 +174,7 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
   D.blocks_colour = lv.blocks_colour.p;
+  D.colour_off_dev = lv.colour_off.p;
 
   lv.rhs.alloc(L.N);
   lv.v64.val.alloc(L.nvals);
@@ -287,6 +290,11 @@ template <typename T>
 static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
   hipStream_t st = Y.ctx->st;
   Vecs<T> &v = vecs<T>(lv);
+  if (kind != 1 && lv.L.N <= kSmallSweepNodes) {
+    // one launch for all sweeps of a small level (launch latency dominates there)
+    if (sweeps > 0) small_sweep<T>(lv.D, kind == 3, omega, sweeps, v.val.p, v.x.p, v.f.p, st);
+    return;
+  }
   for (int s = 0; s < sweeps; s++) {
     if (kind == 1) {
       copy<T>(v.aux.p, v.x.p, lv.L.N, st);
@@ -360,7 +368,8 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learn_active && i >= (int)Y.lev.size() - Y.learned_levels) {
-      learned_post(Y, lv, Y.nets[i]);
+      // one learned step per post-smoothing sweep of the plain cycle
+      for (int k = 0; k < std::max(1, (int)o[7]); k++) learned_post(Y, lv, Y.nets[i]);
       return;
     }
   }
