@@ -37,6 +37,7 @@ typedef struct pmx_system pmx_system;
 
 /* ---- context ----------------------------------------------------------- */
 int pmx_ctx_create(int device, pmx_ctx **out);
+/* fails with PMX_ERR_STATE while systems created on the context are alive */
 int pmx_ctx_destroy(pmx_ctx *ctx);
 int pmx_ctx_sync(pmx_ctx *ctx);
 const char *pmx_last_error(void);
@@ -117,8 +118,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
  *  [18] 1 = launch the MG cycle kernel by kernel instead of replaying its HIP graph
  *  [19] precision 2: number of finest levels with the learned post-smoother
- *       (0 = the levels it was trained for); each of the opt[7] post-sweeps
- *       of those levels becomes one learned step
+ *       (0 = the levels it was trained for)
+ *  [20] precision 2: learned steps per post-smoothing (default 1)
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

@@ -7,6 +7,7 @@ immediately -- there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -100,14 +101,18 @@ class Context:
 
     def __init__(self, device=0):
         self.h = ctypes.c_void_p()
+        self._systems = weakref.WeakSet()
         _ok(lib().pmx_ctx_create(device, ctypes.byref(self.h)), "pmx_ctx_create")
 
     def sync(self):
         _ok(lib().pmx_ctx_sync(self.h), "pmx_ctx_sync")
 
     def close(self):
+        """Destroys the context after the systems created on it."""
         if self.h:
-            lib().pmx_ctx_destroy(self.h)
+            for s in list(self._systems):
+                s.close()
+            _ok(lib().pmx_ctx_destroy(self.h), "pmx_ctx_destroy")
             self.h = ctypes.c_void_p()
 
     def __del__(self):
@@ -133,6 +138,7 @@ class System:
         self.ctx = ctx
         self.h = ctypes.c_void_p()
         _ok(lib().pmx_system_create(ctx.h, nref0, nlev, order, ctypes.byref(self.h)), "pmx_system_create")
+        ctx._systems.add(self)
         self.nlev, self.order, self.nref0 = nlev, order, nref0
 
     def close(self):

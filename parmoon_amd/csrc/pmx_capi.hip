@@ -52,6 +52,7 @@ struct DevBuf {
 
 struct Ctx {
   int device = 0;
+  int systems = 0;  // live systems on this context
   hipStream_t st = nullptr;
   DevBuf<double> part;  // reduction partials
   double *pinned = nullptr;
@@ -290,11 +291,6 @@ template <typename T>
 static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
   hipStream_t st = Y.ctx->st;
   Vecs<T> &v = vecs<T>(lv);
-  if (kind != 1 && lv.L.N <= kSmallSweepNodes) {
-    // one launch for all sweeps of a small level (launch latency dominates there)
-    if (sweeps > 0) small_sweep<T>(lv.D, kind == 3, omega, sweeps, v.val.p, v.x.p, v.f.p, st);
-    return;
-  }
   for (int s = 0; s < sweeps; s++) {
     if (kind == 1) {
       copy<T>(v.aux.p, v.x.p, lv.L.N, st);
@@ -368,8 +364,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learn_active && i >= (int)Y.lev.size() - Y.learned_levels) {
-      // one learned step per post-smoothing sweep of the plain cycle
-      for (int k = 0; k < std::max(1, (int)o[7]); k++) learned_post(Y, lv, Y.nets[i]);
+      for (int k = 0; k < std::max(1, (int)o[20]); k++) learned_post(Y, lv, Y.nets[i]);
       return;
     }
   }
@@ -927,6 +922,7 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
 int pmx_ctx_destroy(pmx_ctx *ctx) {
   PMX_TRY({
     if (!ctx) return PMX_OK;
+    if (ctx->c.systems > 0) throw std::logic_error("pmx_ctx_destroy: destroy the context's systems first");
     (void)hipStreamSynchronize(ctx->c.st);
     ctx->c.part.release();
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
@@ -987,6 +983,7 @@ int pmx_system_create(pmx_ctx *ctx, int nref0, int nlev, int order, pmx_system *
       delete h;
       throw;
     }
+    ctx->c.systems++;
     *out = h;
   })
 }
@@ -994,6 +991,7 @@ int pmx_system_destroy(pmx_system *sys) {
   PMX_TRY({
     if (!sys) return PMX_OK;
     (void)hipStreamSynchronize(sys->s.ctx->st);
+    sys->s.ctx->systems--;
     delete sys;
   })
 }

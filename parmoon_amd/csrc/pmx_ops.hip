@@ -89,54 +89,6 @@ __global__ __launch_bounds__(kBlock) void k_rows(const BlockInfo *__restrict__ b
 #undef PMX_ROW
 }
 
-constexpr int kSmallThreads = 512;
-template <typename T>
-__global__ __launch_bounds__(kSmallThreads) void k_sweep_small(const BlockInfo *__restrict__ blocks,
-                                                      const int *__restrict__ coff, int ncol,
-                                                      const DevSegment *__restrict__ segs, int M, int N,
-                                                      const int *__restrict__ bnd, int nbd,
-                                                      const T *__restrict__ val, T *__restrict__ xg,
-                                                      const T *__restrict__ f, double omega, int sweeps,
-                                                      int symmetric) {
-  constexpr int R = kRowLanes;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  T *xs = reinterpret_cast<T *>(smem);
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += kSmallThreads) xs[i] = xg[i];
-  __syncthreads();
-  for (int b = tid; b < nbd; b += kSmallThreads) xs[bnd[b]] = f[bnd[b]];  // Dirichlet x = f
-  __syncthreads();
-  const int sub = tid / (kBlock), rid = (tid % kBlock) / R, part = tid % R;
-  for (int s = 0; s < sweeps; s++) {
-    for (int pass = 0; pass <= symmetric; pass++) {
-      for (int cc = 0; cc < ncol; cc++) {
-        const int c = pass == 0 ? cc : ncol - 1 - cc;
-        for (int e = coff[c] + sub; e < coff[c + 1]; e += kSmallThreads / kBlock) {
-          const BlockInfo bi = blocks[e];
-          const DevSegment sg = segs[bi.seg];
-          const int idx = bi.base + rid;
-          if (idx < sg.count) {
-#define PMX_ROW(EX, EY, EZ) row_split<T, EX, EY, EZ, R, 3>(sg, idx, part, M, val, xs, f, xs, omega)
-            switch (sg.ex * 100 + sg.ey * 10 + sg.ez) {
-              case 111: PMX_ROW(1, 1, 1); break;
-              case 211: PMX_ROW(2, 1, 1); break;
-              case 121: PMX_ROW(1, 2, 1); break;
-              case 221: PMX_ROW(2, 2, 1); break;
-              case 112: PMX_ROW(1, 1, 2); break;
-              case 212: PMX_ROW(2, 1, 2); break;
-              case 122: PMX_ROW(1, 2, 2); break;
-              default: PMX_ROW(2, 2, 2); break;
-            }
-#undef PMX_ROW
-          }
-        }
-        __syncthreads();
-      }
-    }
-  }
-  for (int i = tid; i < N; i += kSmallThreads) xg[i] = xs[i];
-}
-
 // ============================================================ transfers
 // 1D taps of the nodal interpolation from the coarse to the fine lattice
 // (the prolongation matrices of FEDatabase3D.C:1656 for Q1/Q2 Lagrange,
@@ -611,14 +563,6 @@ switch (mode) {
   }
 }
 template <typename T>
-void small_sweep(const DevLayout &D, int symmetric, double omega, int sweeps, const T *val, T *x, const T *f,
-                 hipStream_t st) {
-  if (D.N > kSmallSweepNodes) throw std::invalid_argument("small_sweep: level too large");
-  hipLaunchKernelGGL(k_sweep_small<T>, dim3(1), dim3(kSmallThreads), sizeof(T) * D.N, st, D.blocks_colour,
-                     D.colour_off_dev, D.ncolours, D.segs, D.M, (int)D.N, D.boundary, D.nboundary, val, x, f, omega,
-                     sweeps, symmetric);
-}
-template <typename T>
 void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st) {
   const int g = (D.nboundary + kBlock - 1) / kBlock;
   switch (mode) {
@@ -754,7 +698,6 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void css_apply<T>(const DevLayout &, int, const T *, const T *, const T *, T *, double, int,     \
                              hipStream_t);                                                                  \
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
-  template void small_sweep<T>(const DevLayout &, int, double, int, const T *, T *, const T *, hipStream_t);  \
   template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
   template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t);            \
   template void gemv<T>(int, int64_t, const double *, const T *, T *, hipStream_t);                         \

@@ -42,14 +42,6 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st);
-// Whole SOR (symmetric = 0) or SSOR (1) sweeps of a small level in one
-// workgroup with x held in LDS: Dirichlet x = f, then the colours in order
-// (and back for SSOR), `sweeps` times.  Same arithmetic and order as the
-// per-colour launches.  Levels up to kSmallSweepNodes nodes.
-constexpr int kSmallSweepNodes = 6000;
-template <typename T>
-void small_sweep(const DevLayout &D, int symmetric, double omega, int sweeps, const T *val, T *x, const T *f,
-                 hipStream_t st);
 // Dirichlet rows: mode 0 y=x, 1 y=f-x, 2 y=f, 3 y=0
 template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st);
 template <typename T>
