@@ -11,8 +11,11 @@ algorithm to follow; the oracle states the network itself:
 * tap t = (dx+1) + 3 (dy+1) + 9 (dz+1); nodes outside the lattice read as 0;
 * layer l: pre = sum_t sum_c W[o, t, c] a[z+dz, y+dy, x+dx, c] + b[o];
   hidden layers a_{l+1} = GELU(pre) (exact erf form), the last layer is linear;
-* channels 1 -> 16 -> 16 -> 16 -> 1; parameters packed per layer as
+* channels 8 -> 16 -> 16 -> 16 -> 1; parameters packed per layer as
   W[cout][27][cin] followed by b[cout], FP32 on the device;
+* input channels per node: the scaled defect u, then the six nearest
+  couplings of the row divided by its diagonal, a(p,p+e)/a(p,p) for
+  e = +x, -x, +y, -y, +z, -z (0 on Dirichlet nodes), then 0 (``features``);
 * BF16 emulation (``bf16=True``): the input, every weight and every hidden
   activation is rounded to BF16 (round to nearest even) before it enters a
   convolution, the products are summed exactly (FP64 here, FP32 on MFMA);
@@ -28,7 +31,7 @@ import numpy as np
 from scipy.special import erf
 
 TAPS = 27
-CH = [1, 16, 16, 16, 1]
+CH = [8, 16, 16, 16, 1]
 
 
 def offsets():
@@ -127,11 +130,37 @@ def conv_weight_grad(g, a):
     return dW, g2.sum(axis=0)
 
 
-def forward(p, u, bf16_emulation=True, keep=False):
-    """Network output (M, M, M) for the input u (M, M, M)."""
+def features(A, M):
+    """(M, M, M, 7) operator features of a lattice-ordered sparse matrix A:
+    a(p, p + e) / a(p, p) for e = +x, -x, +y, -y, +z, -z on interior nodes,
+    then a zero channel."""
+    A = A.tocsr()
+    N = M ** 3
+    d = A.diagonal()
+    m = interior_mask(M).ravel()
+    out = np.zeros((N, 7))
+    idx = np.nonzero(m)[0]
+    for q, off in enumerate((1, -1, M, -M, M * M, -M * M)):
+        out[idx, q] = np.asarray(A[idx, idx + off]).ravel() / d[idx]
+    return out.reshape(M, M, M, 7)
+
+
+def net_input(u, feat=None):
+    """Stack the scaled defect (M, M, M) and the features (M, M, M, 7) into the
+    (M, M, M, 8) network input (features 0 if not given)."""
+    u = np.asarray(u, np.float64)
+    if u.ndim == 4:
+        return u
+    f = np.zeros(u.shape + (7,)) if feat is None else feat
+    return np.concatenate([u[..., None], f], axis=-1)
+
+
+def forward(p, u, bf16_emulation=True, keep=False, feat=None):
+    """Network output (M, M, M) for the input u: (M, M, M, 8), or the scaled
+    defect (M, M, M) with the features `feat` (M, M, M, 7) (0 if None)."""
     r = bf16 if bf16_emulation else (lambda z: np.asarray(z, np.float64))
     layers = unpack(p)
-    a = r(np.asarray(u, np.float64)[..., None])
+    a = r(net_input(u, feat))
     acts, pres = [a], []
     for l, (W, b) in enumerate(layers):
         pre = conv(a, r(W), b)
@@ -181,7 +210,7 @@ def residual_head(u, A, dinv):
     return head
 
 
-def backward(p, u, t=None, bf16_emulation=True, head=None):
+def backward(p, u, t=None, bf16_emulation=True, head=None, feat=None):
     """(loss, gradient of the loss w.r.t. the packed parameters); the loss is
     mse_head(t) unless another head is given.
 
@@ -189,7 +218,7 @@ def backward(p, u, t=None, bf16_emulation=True, head=None):
     (the gradient of the emulated network in the straight-through sense)."""
     r = bf16 if bf16_emulation else (lambda z: np.asarray(z, np.float64))
     layers = unpack(p)
-    out, acts, pres = forward(p, u, bf16_emulation, keep=True)
+    out, acts, pres = forward(p, u, bf16_emulation, keep=True, feat=feat)
     lval, gout = (head or mse_head(t))(out)
     g = gout[..., None]
     grads = [None] * 4

@@ -289,14 +289,19 @@ class System:
         return out
 
     def learned_forward(self, u):
-        """Network output for an (M, M, M) array indexed [z, y, x]."""
+        """Network output (M, M, M) for the input u (M, M, M, 8) indexed [z, y, x, channel]."""
         u = np.ascontiguousarray(u, np.float32)
-        out = np.zeros_like(u)
+        if u.ndim != 4 or u.shape[3] != 8:
+            raise ValueError("input must be (M, M, M, 8)")
+        out = np.zeros(u.shape[:3], np.float32)
         _ok(lib().pmx_learned_forward(self.h, u.shape[0], _p(u), _p(out)), "pmx_learned_forward")
         return out
 
     def learned_gradient(self, u, t):
+        """Squared-error gradient for the input u (M, M, M, 8) and target t (M, M, M)."""
         u = np.ascontiguousarray(u, np.float32)
+        if u.ndim != 4 or u.shape[3] != 8:
+            raise ValueError("input must be (M, M, M, 8)")
         t = np.ascontiguousarray(t, np.float32)
         g = np.zeros(self.learned_nparams(), np.float32)
         loss = np.zeros(1)

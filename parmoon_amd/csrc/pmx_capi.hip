@@ -80,10 +80,12 @@ struct Level {
   Vecs<double> v64;
   Vecs<float> v32;
   bool have_val32 = false;
-  DevBuf<double> ainv;                // coarsest level: explicit inverse
+  DevBuf<double> ainv;                // coarsest level: A_II^-1 [I, -A_IB] (n_I x N)
+  DevBuf<int> cidx, iidx;             // lattice node <-> interior row of ainv
   int ainv_n = 0;
   // learned post-smoother: 1/diag, correction and its image under A (FP32)
   DevBuf<float> dinv, lc, lac;
+  DevBuf<uint16_t> feat;              // network input features [N][8] (BF16)
   bool have_dinv = false;
 };
 
@@ -235,14 +237,31 @@ static void make_tables(int order, bool linear, std::vector<double> &qref, std::
 // as the exact coarse solve (SC_COARSE_SMOOTHER_SCALAR 17, MGLevel3D.C:865-905,
 // which refactors a dense copy on every call; here it is formed once per assembly)
 static void coarse_inverse(System &Y, Level &lv) {
-  const int n = (int)lv.L.N;
-  if (n > 6000) { lv.ainv_n = 0; return; }
-  DevBuf<double> fac;
-  fac.alloc(n);
-  lv.ainv.alloc((size_t)n * 2 * n);
-  dense_inverse(lv.D, lv.v64.val.p, lv.ainv.p, fac.p, Y.ctx->st);
-  check(hipStreamSynchronize(Y.ctx->st), "coarse inverse");
-  lv.ainv_n = n;
+  const int N = (int)lv.L.N;
+  const int nI = (int)lv.L.nactive;
+  if (nI > 6000) { lv.ainv_n = 0; return; }
+  hipStream_t st = Y.ctx->st;
+  if (!lv.cidx.p) {
+    const int M = lv.L.M;
+    std::vector<int> c(N, -1), ii;
+    for (int z = 1; z < M - 1; z++)
+      for (int y = 1; y < M - 1; y++)
+        for (int x = 1; x < M - 1; x++) {
+          const int p = x + M * (y + M * z);
+          c[p] = (int)ii.size();
+          ii.push_back(p);
+        }
+    lv.cidx.upload(c.data(), c.size(), st);
+    lv.iidx.upload(ii.data(), ii.size(), st);
+  }
+  DevBuf<double> aug;
+  DevBuf<int> piv;
+  aug.alloc((size_t)nI * (nI + N));
+  piv.alloc(nI);
+  lv.ainv.alloc((size_t)nI * N);
+  coarse_inverse_dense(lv.D, lv.v64.val.p, lv.cidx.p, nI, aug.p, piv.p, lv.ainv.p, st);
+  check(hipStreamSynchronize(st), "coarse inverse");
+  lv.ainv_n = nI;
 }
 
 template <typename T> Vecs<T> &vecs(Level &lv);
@@ -277,9 +296,11 @@ static void ensure_learned(System &Y) {
       lv.dinv.alloc(lv.L.N);
       lv.lc.alloc(lv.L.N);
       lv.lac.alloc(lv.L.N);
+      lv.feat.alloc((size_t)lv.L.N * kNetIn);
     }
     if (!lv.have_dinv) {
       learned_diag_inv(lv.D, lv.v32.val.p, lv.dinv.p, Y.ctx->st);
+      learned_features(lv.D, lv.v32.val.p, lv.feat.p, Y.ctx->st);
       lv.have_dinv = true;
     }
     if (net_reserve(Y.nets[i], lv.L.N, false)) Y.invalidate_graphs();
@@ -322,7 +343,7 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
   learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
   dot<float>(lv.lc.p, lv.lc.p, N, part, sc, 1, st);
-  learned_scale_input(net, lv.lc.p, sc, sq, N, st);
+  learned_build_input(net, lv.lc.p, sc, sq, lv.feat.p, N, st);
   net_forward_act(net, lv.L.M, false, st);
   learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
@@ -341,7 +362,8 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if (i == 0) {
     const int cs = (int)o[9];
     if (cs == 17 && lv.ainv_n > 0) {
-      gemv<T>(lv.ainv_n, 2 * (int64_t)lv.ainv_n, lv.ainv.p + lv.ainv_n, v.f.p, v.x.p, st);
+      gemv_coarse<T>(lv.ainv_n, (int)lv.L.N, lv.ainv.p, lv.iidx.p, v.f.p, v.x.p, st);
+      boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
     } else {
       level_smooth<T>(Y, lv, cs == 17 ? 3 : cs, o[8], std::max(1, (int)o[10]));
     }
@@ -641,7 +663,8 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
   hipStream_t st = Y.ctx->st;
   const int M = lv.L.M;
   Vecs<float> &v = lv.v32;
-  net_forward(net, M, u, true, st);
+  learned_build_input(net, u, nullptr, 1.0, lv.feat.p, lv.L.N, st);
+  net_forward_act(net, M, true, st);
   learned_mask(net.out, M, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
   boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
@@ -721,7 +744,8 @@ static void learned_train(System &Y, const double *to, double *ti) {
     if (residual_loss) {
       learned_res_step(Y, lv, net, du.p + o, losses.p + k);
     } else {
-      net_forward(net, lv.L.M, du.p + o, true, st);
+      learned_build_input(net, du.p + o, nullptr, 1.0, lv.feat.p, lv.L.N, st);
+      net_forward_act(net, lv.L.M, true, st);
       net_backward(net, lv.L.M, dt.p + o, losses.p + k, st);
     }
     net_adam(net, (float)lrk, st);
@@ -1395,7 +1419,7 @@ int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out) {
     const int64_t N = (int64_t)M * M * M;
     if (net_reserve(net, N, false)) Y.invalidate_graphs();
     DevBuf<float> din;
-    din.upload(in, N, st);
+    din.upload(in, (size_t)N * kNetIn, st);
     net_forward(net, M, din.p, false, st);
     check(hipMemcpyAsync(out, net.out, sizeof(float) * N, hipMemcpyDeviceToHost, st), "d2h");
     check(hipStreamSynchronize(st), "sync");
@@ -1412,7 +1436,7 @@ int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *t
     const int64_t N = (int64_t)M * M * M;
     if (net_reserve(net, N, true)) Y.invalidate_graphs();
     DevBuf<float> din, dtg, dl;
-    din.upload(in, N, st);
+    din.upload(in, (size_t)N * kNetIn, st);
     dtg.upload(target, N, st);
     dl.alloc(1);
     net_zero_grads(net, st);

@@ -78,7 +78,10 @@ template <int CIN, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                               const float *__restrict__ bias, float *__restrict__ pre,
                                               uint16_t *__restrict__ out_bf, float *__restrict__ out_f) {
-  constexpr int NCH = CIN == 1 ? 1 : 14;  // K chunks of 32
+  // K chunks of 32: CIN 1 -> taps 0..26 in one chunk; CIN 8 -> 4 taps per
+  // chunk (7 chunks); CIN 16 -> 2 taps per chunk (14 chunks)
+  constexpr int TPC = 32 / CIN;
+  constexpr int NCH = CIN == 1 ? 1 : (kTaps + TPC - 1) / TPC;
   __shared__ __attribute__((aligned(16))) uint16_t sh[kHalo * CIN];
   const int nbx = (M + kBX - 1) / kBX, nby = (M + kBY - 1) / kBY;
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
@@ -92,14 +95,15 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
                                                                       : (uint16_t)0;
     }
   } else {
-    for (int q = threadIdx.x; q < kHalo * 2; q += 256) {
-      const int n = q >> 1, half = q & 1;
+    constexpr int PIECES = CIN / 8;  // 16-byte pieces per node
+    for (int q = threadIdx.x; q < kHalo * PIECES; q += 256) {
+      const int n = q / PIECES, piece = q % PIECES;
       const int hx = n % kHX, hy = (n / kHX) % kHY, hz = n / (kHX * kHY);
       const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
       u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
       if (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M)
-        v = *reinterpret_cast<const u16x8 *>(in + (x + (int64_t)M * (y + (int64_t)M * z)) * CIN + 8 * half);
-      *reinterpret_cast<u16x8 *>(sh + n * CIN + 8 * half) = v;
+        v = *reinterpret_cast<const u16x8 *>(in + (x + (int64_t)M * (y + (int64_t)M * z)) * CIN + 8 * piece);
+      *reinterpret_cast<u16x8 *>(sh + n * CIN + 8 * piece) = v;
     }
   }
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -116,8 +120,9 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
         b[j] = t < kTaps ? w[row * kTaps + t] : (uint16_t)0;
       }
     } else {
-      const int t = 2 * c + (g >> 1);
-      if (t < kTaps) b = *reinterpret_cast<const u16x8 *>(w + ((int64_t)row * kTaps + t) * CIN + 8 * (g & 1));
+      const int t = TPC * c + g / (4 / TPC);
+      const int cb = 8 * (g % (4 / TPC));
+      if (t < kTaps) b = *reinterpret_cast<const u16x8 *>(w + ((int64_t)row * kTaps + t) * CIN + cb);
     }
     bw[c] = b;
   }
@@ -142,12 +147,13 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
           }
         }
       } else {
-        const int t = 2 * c + (g >> 1);
+        const int t = TPC * c + g / (4 / TPC);
+        const int cb = 8 * (g % (4 / TPC));
         if (t < kTaps) {
           int dx, dy, dz;
           tap_off(t, dx, dy, dz);
           a = *reinterpret_cast<const u16x8 *>(
-              sh + ((row + 1 + dx) + kHX * ((wave + 1 + dy) + kHY * (q + 1 + dz))) * CIN + 8 * (g & 1));
+              sh + ((row + 1 + dx) + kHX * ((wave + 1 + dy) + kHY * (q + 1 + dz))) * CIN + cb);
         }
       }
       acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bw[c]),
@@ -459,14 +465,15 @@ static unsigned conv_grid(int M) {
 void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st) {
   const int64_t N = (int64_t)M * M * M;
   if (N > net.cap) throw std::runtime_error("net_forward: lattice larger than reserved");
-  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, in, net.act[0], N);
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((N * kNetIn + 255) / 256)), dim3(256), 0, st, in, net.act[0],
+                     N * kNetIn);
   net_forward_act(net, M, training, st);
 }
 
 void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
   const unsigned g = conv_grid(M);
   auto P = [&](int l) { return net.params + net_param_offset(l) + net_wsize(l); };  // bias
-  hipLaunchKernelGGL((k_conv<1, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride, P(0),
+  hipLaunchKernelGGL((k_conv<kNetIn, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride, P(0),
                      training ? net.pre[0] : nullptr, net.act[1], nullptr);
   hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[1], net.wbf + 1 * kWStride, P(1),
                      training ? net.pre[1] : nullptr, net.act[2], nullptr);
@@ -529,7 +536,7 @@ static void backward_layers(LearnedNet &net, int M, hipStream_t st) {
     // through GELU of layer l
     hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((N * kNetCh + 255) / 256)), dim3(256), 0, st, g1, net.pre[l], g0,
                        gf, N * kNetCh);
-    if (l == 0) wgrad<1, 16>(net, M, gf, net.act[0], 0, st);
+    if (l == 0) wgrad<kNetIn, 16>(net, M, gf, net.act[0], 0, st);
     else wgrad<16, 16>(net, M, gf, net.act[l], l, st);
     if (l > 0)
       hipLaunchKernelGGL((k_conv<16, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + l * kWStride, nullptr,
@@ -564,13 +571,41 @@ __global__ void k_learned_input(const float *__restrict__ dinv, const float *__r
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     u[i] = dinv[i] * d[i];
 }
-__global__ void k_scale_bf16(const float *__restrict__ u, const double *__restrict__ nrm, double sq, int64_t n,
-                             uint16_t *__restrict__ y) {
-  const double nv = nrm[0];
-  const float sc = nv > 0 ? (float)(sq / nv) : 0.f;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    y[i] = f2bf(u[i] * sc);
+// act0[p] = feat[p] with channel 0 = bf16(u[p] * sq / *nrm) (no scaling if nrm is null)
+__global__ void k_build_input(const float *__restrict__ u, const double *__restrict__ nrm, double sq,
+                              const uint16_t *__restrict__ feat, int64_t n, uint16_t *__restrict__ act0) {
+  float sc = 1.0f;
+  if (nrm) {
+    const double nv = nrm[0];
+    sc = nv > 0 ? (float)(sq / nv) : 0.f;
+  }
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    u16x8 v = *reinterpret_cast<const u16x8 *>(feat + i * kNetIn);
+    v[0] = f2bf(u[i] * sc);
+    *reinterpret_cast<u16x8 *>(act0 + i * kNetIn) = v;
+  }
 }
+// the six nearest couplings of every interior row, divided by its diagonal
+__global__ void k_features(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
+                           const float *__restrict__ val, uint16_t *__restrict__ feat) {
+  const BlockInfo bi = blocks[blockIdx.x];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
+  const int64_t p = (s.x0 + s.stride * i) + (int64_t)M * ((s.y0 + s.stride * j) + (int64_t)M * (s.z0 + s.stride * k));
+  const int64_t vb = s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
+  const int wx = 2 * s.ex + 1, wy = 2 * s.ey + 1;
+  auto slot = [&](int dx, int dy, int dz) { return (dx + s.ex) + wx * ((dy + s.ey) + wy * (dz + s.ez)); };
+  const float dinv = 1.0f / val[vb + (int64_t)s.dslot * kSlice];
+  const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    v[1 + q] = f2bf(val[vb + (int64_t)slot(off[q][0], off[q][1], off[q][2]) * kSlice] * dinv);
+  *reinterpret_cast<u16x8 *>(feat + p * kNetIn) = v;
+}
+
 // MODE 0: c = in * nrm / sq;  MODE 1: c = sign * in * sq / nrm  (interior only)
 template <int MODE>
 __global__ void k_interior_scale(const float *__restrict__ in, int M, const double *__restrict__ nrm, double sq,
@@ -614,8 +649,14 @@ void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStre
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_learned_input, dim3(grid_for(n)), dim3(256), 0, st, dinv, d, u, n);
 }
-void learned_scale_input(LearnedNet &net, const float *u, const double *nrm, double sq, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_scale_bf16, dim3(grid_for(n)), dim3(256), 0, st, u, nrm, sq, n, net.act[0]);
+void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sq, const uint16_t *feat,
+                         int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_input, dim3(grid_for(n)), dim3(256), 0, st, u, nrm, sq, feat, n, net.act[0]);
+}
+void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipStream_t st) {
+  fill<uint16_t>(feat, (uint16_t)0, D.N * kNetIn, st);
+  hipLaunchKernelGGL(k_features, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, val,
+                     feat);
 }
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sq, float *c, hipStream_t st) {
   hipLaunchKernelGGL(k_interior_scale<0>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, net.out, M, nrm, sq,

@@ -1,6 +1,9 @@
 // pmx_learned.hpp -- the learned post-smoother: a 4-layer 3x3x3 convolution
-// network (1 -> 16 -> 16 -> 16 -> 1 channels, GELU between layers) that maps
+// network (8 -> 16 -> 16 -> 16 -> 1 channels, GELU between layers) that maps
 // a diagonally scaled defect to a correction on the node lattice of a level.
+// Input channels per node: [u, a(p,p+ex)/a(p,p), a(p,p-ex)/a(p,p), (same for
+// y and z), 0] -- the scaled defect and the six nearest couplings of the row,
+// which tell the network the local convection direction and strength.
 // BF16 weights and activations, FP32 accumulation on MFMA
 // (v_mfma_f32_16x16x32_bf16), hand-written forward, backward and Adam kernels.
 // No counterpart in the reference (its smoothers are MGLevel3D.C:280-563).
@@ -14,10 +17,11 @@ namespace pmx {
 
 constexpr int kNetLayers = 4;
 constexpr int kNetCh = 16;   // hidden channels
+constexpr int kNetIn = 8;    // input channels: scaled defect + 6 operator features + 0
 constexpr int kTaps = 27;    // 3x3x3 stencil of a layer
 
 // layer l maps cin(l) -> cout(l) channels
-inline int net_cin(int l) { return l == 0 ? 1 : kNetCh; }
+inline int net_cin(int l) { return l == 0 ? kNetIn : kNetCh; }
 inline int net_cout(int l) { return l == kNetLayers - 1 ? 1 : kNetCh; }
 // FP32 master parameters of layer l: W[cout][tap][cin] then b[cout]
 inline int net_wsize(int l) { return net_cout(l) * kTaps * net_cin(l); }
@@ -60,7 +64,7 @@ void net_init(LearnedNet &net, uint32_t seed, hipStream_t st);
 bool net_reserve(LearnedNet &net, int64_t N, bool training);
 // refresh BF16 copies from the FP32 master parameters
 void net_sync_bf16(LearnedNet &net, hipStream_t st);
-// forward on an M^3 lattice: in[N] (FP32) -> net.out[N] (FP32); training keeps pre-activations
+// forward on an M^3 lattice: in[N][kNetIn] (FP32) -> net.out[N] (FP32); training keeps pre-activations
 void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st);
 // the same from the BF16 input already in net.act[0]
 void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
@@ -77,9 +81,13 @@ struct DevLayout;
 void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st);
 // u = dinv * d (lattice), the unscaled network input
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st);
-// net.act[0] = bf16(u * sqrt_count / *nrm)   (nrm: device FP64 2-norm of u)
-void learned_scale_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, int64_t n,
-                         hipStream_t st);
+// net.act[0] = feat with channel 0 = bf16(u * sqrt_count / *nrm) (nrm: device
+// FP64 2-norm of u; nrm == nullptr: no scaling)
+void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
+                         int64_t n, hipStream_t st);
+// operator features of a level: feat[p][1..6] = bf16(a(p, p +- e_k) / a(p,p)) on
+// interior nodes (x+, x-, y+, y-, z+, z-), channels 0 and 7 and boundary nodes 0
+void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipStream_t st);
 // c = net.out * (*nrm) / sqrt_count on interior nodes of an M^3 lattice, 0 on its boundary
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
 // x += alpha c with alpha = max(0, p[0] / p[1]) (minimal-residual step; 0 if p[1] == 0)

@@ -366,25 +366,30 @@ __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int *__restric
 }
 
 // ============================================================ dense coarse solve
-// y = Ainv x with the explicit inverse of the coarsest matrix (FP64), one wavefront per row
+// The coarsest operator is A = [A_II A_IB; 0 I] (interior rows I, Dirichlet
+// rows B identity), so A^-1 f = [A_II^-1 (f_I - A_IB f_B); f_B].  The device
+// keeps W = A_II^-1 [I_I, -A_IB] as an n_I x N matrix in lattice column order:
+// y_I = W f (one wavefront per row, FP64), y_B = f_B.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gemv(int n, int64_t ld, const double *__restrict__ Ainv,
-                                                 const T *__restrict__ x, T *__restrict__ y) {
+__global__ __launch_bounds__(kBlock) void k_gemv_coarse(int nI, int N, const double *__restrict__ W,
+                                                        const int *__restrict__ iidx, const T *__restrict__ x,
+                                                        T *__restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6;
-  if (row >= n) return;
+  if (row >= nI) return;
   double s = 0;
-  const double *a = Ainv + (int64_t)row * ld;
-  for (int j = lane; j < n; j += 64) s += a[j] * (double)x[j];
+  const double *a = W + (int64_t)row * N;
+  for (int j = lane; j < N; j += 64) s += a[j] * (double)x[j];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  if (lane == 0) y[row] = (T)s;
+  if (lane == 0) y[iidx[row]] = (T)s;
 }
 
 // ============================================================ coarse inverse
-// Dense operator of a level in lattice order, row-major into the left half of
-// the augmented [A | I] (leading dimension 2n).
-__global__ void k_css_to_dense(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
-                               const double *__restrict__ val, double *__restrict__ aug, int64_t ld) {
+// Augmented [A_II | R] (n_I x (n_I + N), row-major), R[i][p] = 1 for the
+// interior node p of row i, -a(i, p) for Dirichlet columns p.
+__global__ void k_css_to_aug(const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
+                             const int *__restrict__ cidx, int nI, const double *__restrict__ val,
+                             double *__restrict__ aug, int64_t ld) {
   const BlockInfo bi = blocks[blockIdx.x];
   const DevSegment s = segs[bi.seg];
   const int idx = bi.base + threadIdx.x;
@@ -392,63 +397,78 @@ __global__ void k_css_to_dense(const BlockInfo *__restrict__ blocks, const DevSe
   const int i = idx % s.cx, t = idx / s.cx, j = t % s.cy, k = t / s.cy;
   const int x = s.x0 + s.stride * i, y = s.y0 + s.stride * j, z = s.z0 + s.stride * k;
   const int p = x + M * (y + M * z);
+  const int r = cidx[p];
+  double *row = aug + (int64_t)r * ld;
+  row[nI + p] = 1.0;
   int slot = 0;
   for (int dz = -s.ez; dz <= s.ez; dz++)
     for (int dy = -s.ey; dy <= s.ey; dy++)
-      for (int dx = -s.ex; dx <= s.ex; dx++, slot++)
-        aug[(int64_t)p * ld + (x + dx) + M * ((y + dy) + M * (z + dz))] = val[vbase(s, idx) + (int64_t)slot * kSlice];
+      for (int dx = -s.ex; dx <= s.ex; dx++, slot++) {
+        const int q = (x + dx) + M * ((y + dy) + M * (z + dz));
+        const double v = val[vbase(s, idx) + (int64_t)slot * kSlice];
+        const int cq = cidx[q];
+        if (cq >= 0) row[cq] = v;
+        else row[nI + q] = -v;
+      }
 }
-__global__ void k_dense_identity(const int *__restrict__ bnd, int nbd, int n, double *__restrict__ aug) {
-  const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (id < nbd) aug[(int64_t)bnd[id] * 2 * n + bnd[id]] = 1.0;   // Dirichlet row of A
-  if (id < n) aug[id * 2 * n + n + id] = 1.0;                    // right half: I
-}
-// Gauss-Jordan step c with partial pivoting on [A | I] (n x 2n, row-major):
-// (1) one block finds the pivot row and swaps it into row c
-__global__ __launch_bounds__(1024) void k_gj_pivot(double *__restrict__ aug, int n, int c, double *__restrict__ fac) {
-  __shared__ double bv[1024];
-  __shared__ int bi[1024];
-  const int64_t ld = 2 * (int64_t)n;
-  double best = -1;
-  int arg = c;
-  for (int r = c + threadIdx.x; r < n; r += 1024) {
-    const double a = fabs(aug[r * ld + c]);
+// Gauss-Jordan step c with partial pivoting, in one launch: every block finds
+// the same pivot row (largest |a(r, c)| among rows not yet used as pivots,
+// lowest index on ties), then eliminates column c from the other rows of its
+// share.  Rows are not swapped or scaled (the pivot row and column c are
+// read-only during the step); piv[c] records the pivot row.
+__global__ __launch_bounds__(256) void k_gj_step(double *__restrict__ aug, int nI, int64_t ld, int c,
+                                                 int *__restrict__ piv) {
+  __shared__ unsigned char used[8192];
+  __shared__ double bv[256];
+  __shared__ int bix[256];
+  for (int r = threadIdx.x; r < nI; r += 256) used[r] = 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < c; k += 256) used[piv[k]] = 1;
+  __syncthreads();
+  double best = -1.0;
+  int arg = nI;
+  for (int r = threadIdx.x; r < nI; r += 256) {
+    if (used[r]) continue;
+    const double a = fabs(aug[(int64_t)r * ld + c]);
     if (a > best) { best = a; arg = r; }
   }
   bv[threadIdx.x] = best;
-  bi[threadIdx.x] = arg;
+  bix[threadIdx.x] = arg;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
+  for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       const double b2 = bv[threadIdx.x + o];
-      const int i2 = bi[threadIdx.x + o];
-      if (b2 > bv[threadIdx.x] || (b2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) { bv[threadIdx.x] = b2; bi[threadIdx.x] = i2; }
+      const int i2 = bix[threadIdx.x + o];
+      if (b2 > bv[threadIdx.x] || (b2 == bv[threadIdx.x] && i2 < bix[threadIdx.x])) {
+        bv[threadIdx.x] = b2;
+        bix[threadIdx.x] = i2;
+      }
     }
     __syncthreads();
   }
-  const int p = bi[0];
-  if (p != c)
-    for (int j = threadIdx.x; j < ld; j += 1024) {
-      const double t = aug[(int64_t)c * ld + j];
-      aug[(int64_t)c * ld + j] = aug[(int64_t)p * ld + j];
-      aug[(int64_t)p * ld + j] = t;
-    }
-  __syncthreads();
-  // scale the pivot row, then save the column of multipliers
-  const double inv = 1.0 / aug[(int64_t)c * ld + c];
-  __syncthreads();
-  for (int j = threadIdx.x; j < ld; j += 1024) aug[(int64_t)c * ld + j] *= inv;
-  for (int r = threadIdx.x; r < n; r += 1024) fac[r] = (r == c) ? 0.0 : aug[(int64_t)r * ld + c];
+  const int pr = bix[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) piv[c] = pr;
+  const double *prow = aug + (int64_t)pr * ld;
+  const double pinv = 1.0 / prow[c];
+  // this block's rows: r = blockIdx.x * 4 + 0..3, columns c+1 .. ld-1
+  const int64_t w = ld - (c + 1);
+  for (int q = 0; q < 4; q++) {
+    const int r = blockIdx.x * 4 + q;
+    if (r >= nI || r == pr) continue;
+    double *row = aug + (int64_t)r * ld;
+    const double f = row[c] * pinv;
+    if (f == 0.0) continue;
+    for (int64_t j = threadIdx.x; j < w; j += 256) row[c + 1 + j] -= f * prow[c + 1 + j];
+  }
 }
-// (2) eliminate column c from every other row; the pivot row is read-only here
-__global__ __launch_bounds__(256) void k_gj_eliminate(double *__restrict__ aug, int n, int c,
-                                                       const double *__restrict__ fac) {
-  const int64_t ld = 2 * (int64_t)n;
+// W[i][p] = aug[piv[i]][nI + p] / aug[piv[i]][i]
+__global__ void k_gj_finish(const double *__restrict__ aug, int nI, int N, int64_t ld, const int *__restrict__ piv,
+                            double *__restrict__ W) {
   const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (id >= (int64_t)n * ld) return;
-  const int r = id / ld, j = id % ld;
-  const double f = fac[r];
-  if (r != c && f != 0.0) aug[id] -= f * aug[(int64_t)c * ld + j];
+  if (id >= (int64_t)nI * N) return;
+  const int i = (int)(id / N), p = (int)(id % N);
+  const double *row = aug + (int64_t)piv[i] * ld;
+  W[id] = row[nI + p] / row[i];
 }
 
 // ============================================================ FGMRES scalars
@@ -581,24 +601,22 @@ void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipSt
   hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M, F.order);
 }
 template <typename T>
-void gemv(int n, int64_t ld, const double *Ainv, const T *x, T *y, hipStream_t st) {
-  const int g = (int)(((int64_t)n * 64 + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_gemv<T>, dim3(g), dim3(kBlock), 0, st, n, ld, Ainv, x, y);
+void gemv_coarse(int nI, int N, const double *W, const int *iidx, const T *x, T *y, hipStream_t st) {
+  const unsigned g = (unsigned)((nI * 64 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_gemv_coarse<T>, dim3(g), dim3(kBlock), 0, st, nI, N, W, iidx, x, y);
 }
-// explicit inverse of a level's operator by Gauss-Jordan on the device;
-// aug: n x 2n workspace, the inverse is its right half (leading dimension 2n)
-void dense_inverse(const DevLayout &D, const double *val, double *aug, double *fac, hipStream_t st) {
-  const int n = (int)D.N;
-  const int64_t ld = 2 * (int64_t)n;
-  check(hipMemsetAsync(aug, 0, sizeof(double) * n * ld, st), "memset aug");
-  hipLaunchKernelGGL(k_css_to_dense, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, val,
-                     aug, ld);
-  hipLaunchKernelGGL(k_dense_identity, dim3((n + 255) / 256), dim3(256), 0, st, D.boundary, D.nboundary, n, aug);
-  const unsigned g = (unsigned)((n * ld + 255) / 256);
-  for (int c = 0; c < n; c++) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, st, aug, n, c, fac);
-    hipLaunchKernelGGL(k_gj_eliminate, dim3(g), dim3(256), 0, st, aug, n, c, fac);
-  }
+void coarse_inverse_dense(const DevLayout &D, const double *val, const int *cidx, int nI, double *aug, int *piv,
+                          double *W, hipStream_t st) {
+  if (nI > 8192) throw std::invalid_argument("coarse inverse: too many interior nodes");
+  const int N = (int)D.N;
+  const int64_t ld = nI + (int64_t)N;
+  fill<double>(aug, 0.0, nI * ld, st);
+  hipLaunchKernelGGL(k_css_to_aug, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, cidx,
+                     nI, val, aug, ld);
+  const unsigned g = (unsigned)((nI + 3) / 4);
+  for (int c = 0; c < nI; c++) hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, st, aug, nI, ld, c, piv);
+  hipLaunchKernelGGL(k_gj_finish, dim3((unsigned)(((int64_t)nI * N + 255) / 256)), dim3(256), 0, st, aug, nI, N, ld,
+                     piv, W);
 }
 template <typename T, typename U>
 void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st) {
@@ -700,12 +718,13 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
   template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
   template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t);            \
-  template void gemv<T>(int, int64_t, const double *, const T *, T *, hipStream_t);                         \
+  template void gemv_coarse<T>(int, int, const double *, const int *, const T *, T *, hipStream_t);         \
   template void csr_spmv<T>(int, const int *, const int *, const T *, const T *, T *, hipStream_t);         \
   template void css_to_csr<T>(const DevLayout &, const int *, const int *, const int *, int64_t, const T *, \
                               double *, hipStream_t);
 PMX_INST(double)
 PMX_INST(float)
+template void fill<uint16_t>(uint16_t *, uint16_t, int64_t, hipStream_t);
 template void convert<float, double>(float *, const double *, int64_t, hipStream_t);
 template void convert<double, float>(double *, const float *, int64_t, hipStream_t);
 template void add_convert<double, float>(double *, const float *, int64_t, hipStream_t);

@@ -47,8 +47,14 @@ template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T 
 template <typename T>
 void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st);
 template <typename T> void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st);
-template <typename T> void gemv(int n, int64_t ld, const double *Ainv, const T *x, T *y, hipStream_t st);
-void dense_inverse(const DevLayout &D, const double *val, double *aug, double *fac, hipStream_t st);
+// coarsest-level solve: y_I = W x with W = A_II^-1 [I, -A_IB] (n_I x N, lattice
+// columns; iidx: interior row -> lattice node); y_B = x_B is set separately
+template <typename T>
+void gemv_coarse(int nI, int N, const double *W, const int *iidx, const T *x, T *y, hipStream_t st);
+// W of the level by Gauss-Jordan with partial pivoting (cidx: lattice node ->
+// interior row or -1; aug: n_I x (n_I + N) workspace; piv: n_I ints)
+void coarse_inverse_dense(const DevLayout &D, const double *val, const int *cidx, int nI, double *aug, int *piv,
+                          double *W, hipStream_t st);
 template <typename T, typename U> void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st);
 template <typename T>
 void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st);

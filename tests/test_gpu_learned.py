@@ -40,7 +40,7 @@ def test_forward_vs_oracle(small, M):
     rng = np.random.default_rng(M)
     p = LR.init_params(M, scale_last=1.0).astype(np.float32)
     small.learned_params(p)
-    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
     out = small.learned_forward(u)
     ref = LR.forward(p.astype(np.float64), u.astype(np.float64))
     assert nrel(out, ref) < 5e-3, nrel(out, ref)
@@ -52,7 +52,7 @@ def test_gradient_vs_oracle(small, M):
     rng = np.random.default_rng(100 + M)
     p = LR.init_params(7 + M, scale_last=1.0).astype(np.float32)
     small.learned_params(p)
-    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
     t = rng.standard_normal((M, M, M)).astype(np.float32)
     g, loss = small.learned_gradient(u, t)
     l_ref, g_ref = LR.backward(p.astype(np.float64), u.astype(np.float64), t.astype(np.float64))
@@ -71,7 +71,7 @@ def test_adam_step(small):
     p = LR.init_params(11).astype(np.float32)
     small.learned_init(5)
     small.learned_params(p)     # fresh moments come with init
-    u = rng.standard_normal((M, M, M)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
     t = rng.standard_normal((M, M, M)).astype(np.float32)
     g, _ = small.learned_gradient(u, t)
     small.learned_adam(1e-3)
@@ -101,11 +101,12 @@ def post_reference(O, params, x, f):
     lat = lattice_map(fine, M)
     ul = np.zeros(M ** 3)
     ul[lat] = u * sq / nrm
-    out = LR.forward(params.astype(np.float64), ul.reshape(M, M, M)).ravel()
+    A_lat, _ = lattice_operator(O, M)
+    out = LR.forward(params.astype(np.float64), ul.reshape(M, M, M), feat=LR.features(A_lat, M)).ravel()
     c = np.where(np.arange(N) < na, out[lat] * nrm / sq, 0.0)
     Ac = fine.matvec(E, c)
-    alpha = max(0.0, float(d @ Ac) / float(Ac @ Ac))
-    return x + alpha * c, d
+    alpha = float(d @ Ac) / float(Ac @ Ac)
+    return x + max(0.0, alpha) * c, d, alpha
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -117,17 +118,29 @@ def test_post_step_vs_oracle(ctx, order):
     O = orc.System(1, 3, order)
     O.assemble(prm)
     p = LR.init_params(21 + order).astype(np.float32)
-    S.learned_params(p)
     fine = O.levels[-1]
     rng = np.random.default_rng(order)
     x = rng.standard_normal(fine.N)
     f = O.rhs()
     x[fine.active:] = f[fine.active:]
-    ref, d0 = post_reference(O, p, x, f)
-    got = S.learned_post(x, f)
-    assert nrel(got - x, ref - x) < 3e-2, nrel(got - x, ref - x)
-    d1, _ = fine.defect(O.entries(), got, f)
-    assert np.linalg.norm(d1) <= np.linalg.norm(d0) * (1 + 1e-6)
+    # the parameters and their copy with the output layer negated give
+    # corrections of opposite sign: one takes a step, the other is clamped
+    n3 = 27 * 16 + 1
+    alphas = []
+    for sign in (1.0, -1.0):
+        q = p.copy()
+        q[-n3:] *= sign
+        S.learned_params(q)
+        ref, d0, alpha = post_reference(O, q, x, f)
+        got = S.learned_post(x, f)
+        alphas.append(alpha)
+        if alpha > 0:
+            assert nrel(got - x, ref - x) < 3e-2, nrel(got - x, ref - x)
+        else:
+            assert np.max(np.abs(got - x)) <= 1e-6 * np.max(np.abs(x))  # only FP32 rounding of x
+        d1, _ = fine.defect(O.entries(), got, f)
+        assert np.linalg.norm(d1) <= np.linalg.norm(d0) * (1 + 1e-6)
+    assert alphas[0] * alphas[1] < 0
     S.close()
 
 
@@ -161,7 +174,7 @@ def test_residual_gradient_vs_oracle(ctx, order):
     u = np.random.default_rng(40 + order).standard_normal((M, M, M)).astype(np.float32)
     g, loss = S.learned_gradient_level(u)
     l_ref, g_ref = LR.backward(p.astype(np.float64), u.astype(np.float64),
-                               head=LR.residual_head(u.astype(np.float64), A, dinv))
+                               head=LR.residual_head(u.astype(np.float64), A, dinv), feat=LR.features(A, M))
     assert abs(loss - l_ref) < 1e-3 * l_ref, (loss, l_ref)
     offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
     for l in range(4):

@@ -15,7 +15,7 @@ import learned_ref as LR  # noqa: E402
 
 
 def test_nparams_layout():
-    assert LR.nparams() == (16 * 27 * 1 + 16) + 2 * (16 * 27 * 16 + 16) + (1 * 27 * 16 + 1) == 14737
+    assert LR.nparams() == (16 * 27 * 8 + 16) + 2 * (16 * 27 * 16 + 16) + (1 * 27 * 16 + 1) == 17761
     p = LR.init_params(3)
     assert np.array_equal(LR.pack(LR.unpack(p)), p)
 
@@ -55,7 +55,7 @@ def test_backward_vs_finite_differences():
     rng = np.random.default_rng(3)
     M = 5
     p = LR.init_params(4, scale_last=1.0)
-    u = rng.standard_normal((M, M, M))
+    u = rng.standard_normal((M, M, M, 8))
     t = rng.standard_normal((M, M, M))
     l0, g = LR.backward(p, u, t, bf16_emulation=False)
     assert abs(l0 - LR.loss(LR.forward(p, u, False), t)) < 1e-14 * l0
@@ -106,8 +106,9 @@ def test_residual_head_vs_finite_differences():
     dinv = np.where(m, 1.0 / A.diagonal(), 0.0)
     p = LR.init_params(10, scale_last=1.0)
     u = rng.standard_normal((M, M, M))
+    feat = rng.standard_normal((M, M, M, 7))
     head = LR.residual_head(u, A, dinv)
-    l0, g = LR.backward(p, u, bf16_emulation=False, head=head)
+    l0, g = LR.backward(p, u, bf16_emulation=False, head=head, feat=feat)
     offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
     idx = np.concatenate([rng.choice(np.arange(offs[l], offs[l + 1]), 8, replace=False) for l in range(4)])
     h = 1e-5
@@ -115,7 +116,7 @@ def test_residual_head_vs_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (head(LR.forward(pp, u, False))[0] - head(LR.forward(pm, u, False))[0]) / (2 * h)
+        fd = (head(LR.forward(pp, u, False, feat=feat))[0] - head(LR.forward(pm, u, False, feat=feat))[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(g))), (i, fd, g[i])
     # with a zero correction the loss is the scaled defect itself
     pz = p.copy()
@@ -125,5 +126,28 @@ def test_residual_head_vs_finite_differences():
         if l == 3:
             pz[k:k + n] = 0.0
         k += n
-    lz, _ = LR.backward(pz, u, bf16_emulation=False, head=head)
+    lz, _ = LR.backward(pz, u, bf16_emulation=False, head=head, feat=feat)
     assert abs(lz - float((u[1:-1, 1:-1, 1:-1] ** 2).sum()) / (M - 2) ** 3) < 1e-12
+
+
+def test_features_of_a_stencil():
+    """features() reads the six nearest couplings: check on a 7-point operator
+    with distinct known coefficients."""
+    import scipy.sparse as sp
+    M = 6
+    N = M ** 3
+    coef = {1: -1.5, -1: -0.5, M: -2.0, -M: -0.25, M * M: -3.0, -M * M: -0.75}
+    rows, cols, vals = [], [], []
+    for i in range(N):
+        rows.append(i); cols.append(i); vals.append(8.0)
+        for off, c in coef.items():
+            j = i + off
+            if 0 <= j < N:
+                rows.append(i); cols.append(j); vals.append(c)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    f = LR.features(A, M)
+    m = LR.interior_mask(M)
+    for q, off in enumerate((1, -1, M, -M, M * M, -M * M)):
+        assert np.allclose(f[..., q][m], coef[off] / 8.0)
+        assert np.all(f[..., q][~m] == 0)
+    assert np.all(f[..., 6] == 0)
