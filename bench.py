@@ -16,6 +16,7 @@ with the plain FP64 time, assembly DOF/s and the SpMV roofline alongside.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nref 6] [--no-cpu]
     python bench.py --profile-kernels     # SpMV + a 1 GiB copy only (rocprofv3 --pmc runs)
+    python bench.py --transient 200       # TCD3D 128^3 Q1, Crank-Nicolson, 200 steps
 """
 import argparse
 import json
@@ -80,6 +81,78 @@ def profile_kernels(S, nref):
           file=sys.stderr, flush=True)
 
 
+def transient(args):
+    """Transient workload: TCD3D on 128^3 Q1 (2,146,689 DOFs), eps = 1e-4,
+    b = (-(y - 1/2), x - 1/2, 0.2), a seeded Gaussian blob as initial value,
+    Crank-Nicolson with tau = 1e-3; the stiffness matrix and rhs are
+    re-assembled on the device every step and every step is solved to FP64
+    relative residual 1e-10 (5 GMG levels 8^3..128^3, V(2,2) SSOR, FGMRES(20)).
+    Prints one JSON line with the per-form totals."""
+    import parmoon_amd as pa
+    from parmoon_amd import prm as P
+    ctx = pa.Context(0)
+    S = pa.System(ctx, 3, 5, 1)
+    N = int(S.dims(-1)[0])
+    prm = P.rotating(seed=1)
+    tau, steps = 1e-3, args.transient
+    td = [tau, .5, .5, .5, .5]
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    prec_of = {"fp64": 0, "fp32": 1, "learned": 2}
+    base = dict(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=0.0, smoother=3, omega=1.0,
+                pre=2, post=2, rel_target=1e-10)
+    res = {}
+    train = None
+    for form in forms:
+        S.tcd_init(prm, None)
+        if form == "learned" and train is None:
+            # the system matrix of the first step, then the networks
+            p1 = prm.copy()
+            p1[P.TIME] = tau
+            S.tcd_step(p1, td, P.solver_opt(**dict(base, precision=1)))
+            tinfo = S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=16, steps=args.train_steps,
+                                    lr=2e-3, seed=1, levels=args.learned_levels)
+            train = {"seconds": round(float(tinfo[2]), 3), "loss_first": round(float(tinfo[0]), 4),
+                     "loss_last": round(float(tinfo[1]), 4), "levels": args.learned_levels}
+            S.tcd_init(prm, None)
+        opt = P.solver_opt(**dict(base, precision=prec_of[form], learned_levels=args.learned_levels))
+        t, its, fb, t_solve = 0.0, 0, 0, 0.0
+        t0 = time.perf_counter()
+        for m in range(steps):
+            t += tau
+            p = prm.copy()
+            p[P.TIME] = t
+            info, rc = S.tcd_step(p, td, opt, check=False)
+            if rc != 0:
+                print(f"[bench] transient {form} step {m} rc={rc} info={info}", file=sys.stderr, flush=True)
+                break
+            its += int(info[0])
+            fb += int(info[5])
+            t_solve += info[4]
+            if (m + 1) % 50 == 0:
+                print(f"[bench] transient {form}: step {m + 1}/{steps}", file=sys.stderr, flush=True)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        sol = S.solution()
+        res[form] = {"total_s": round(wall, 4), "solve_s": round(t_solve, 4), "per_step_ms": round(wall / steps * 1e3, 3),
+                     "iterations": its, "fallbacks": fb, "rc": int(rc), "final_max": float(np.max(sol))}
+        print(f"[bench] transient {form}: {res[form]}", file=sys.stderr, flush=True)
+    main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
+    t64 = res.get("fp64", {}).get("total_s")
+    out = {"metric": "TCD3D transient: total time of the time steps (s), learned+FP32 vs plain FP64",
+           "value": res[main_form]["total_s"], "unit": "s", "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": res[main_form]["per_step_ms"], "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": {"learned": "f32-in-f64 + bf16 net", "fp32": "f32-in-f64", "fp64": "f64"}[main_form],
+           "data": "synthetic (seeded Gaussian blob)",
+           "config": {"workload": f"TCD3D Q1 128^3, eps 1e-4, rotating b, Crank-Nicolson tau 1e-3, {steps} steps, "
+                                  f"re-assembly every step, FP64 rel. residual 1e-10 per step", "dofs": N,
+                      "form": main_form, "parallelism": "1 GPU"},
+           "forms": res, "speedup_vs_plain_fp64": round(t64 / res[main_form]["total_s"], 3) if t64 else None,
+           "learned_training": train}
+    print(json.dumps(out))
+    S.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,11 +164,16 @@ def main():
     ap.add_argument("--learned-levels", type=int, default=1)
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--profile-kernels", action="store_true")
+    ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
+                    help="run the TCD3D 128^3 Q1 transient workload for STEPS time steps instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.transient > 0:
+        transient(args)
+        return
 
     import parmoon_amd as pa
     from parmoon_amd import prm as P
