@@ -237,8 +237,7 @@ __global__ void k_make_bf16_weights(const float *__restrict__ W, int cin, int co
 // ------------------------------------------------------------------ backward
 // loss = sum over interior nodes of (y - t)^2 / count;  gy = 2 (y - t) / count
 __global__ __launch_bounds__(256) void k_loss(int M, const float *__restrict__ y, const float *__restrict__ t,
-                                              float inv_count, uint16_t *__restrict__ gy, float *__restrict__ gyf,
-                                              double *__restrict__ part) {
+                                              float inv_count, uint16_t *__restrict__ gy, double *__restrict__ part) {
   const int64_t N = (int64_t)M * M * M;
   double s = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
@@ -248,7 +247,6 @@ __global__ __launch_bounds__(256) void k_loss(int M, const float *__restrict__ y
     s += (double)d * d;
     const float g = 2.f * d * inv_count;
     gy[i] = f2bf(g);
-    gyf[i] = g;
   }
   __shared__ double sh[4];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
@@ -258,12 +256,10 @@ __global__ __launch_bounds__(256) void k_loss(int M, const float *__restrict__ y
 }
 // gradient through GELU: gpre = gpost * gelu'(pre) (BF16 out, [N][16])
 __global__ void k_gelu_back(const uint16_t *__restrict__ gpost, const float *__restrict__ pre,
-                            uint16_t *__restrict__ gpre, float *__restrict__ gpref, int64_t n) {
+                            uint16_t *__restrict__ gpre, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float g = bf2f(gpost[i]) * gelu_d(pre[i]);
-  gpre[i] = f2bf(g);
-  gpref[i] = g;
+  gpre[i] = f2bf(bf2f(gpost[i]) * gelu_d(pre[i]));
 }
 __global__ void k_sum_parts(const double *__restrict__ p, int n, float *__restrict__ o) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -272,66 +268,113 @@ __global__ void k_sum_parts(const double *__restrict__ p, int n, float *__restri
     o[0] = (float)s;
   }
 }
-// weight and bias gradients of one layer: part[block][e] for e < cout*27*cin
-// (then cout bias entries), summed over the nodes of the block's tiles;
-// gp: FP32 gradient w.r.t. the layer output [N][cout]; a: BF16 layer input [N][cin]
+// Weight and bias gradients on MFMA: for every tap t,
+//   dW[o][t][c] = sum over nodes n of gp[n][o] a[n + off_t][c]
+// is a 16 x 16 product with the nodes as the K dimension (32 per instruction).
+// A persistent workgroup walks over 16 x 4 x 4-node bricks; per brick it
+// stages gp channel-major ([o][z][y][x]) and the input a with its halo as
+// three copies shifted by dx = -1, 0, 1 ([dx][c][z][y][x]), so that every A
+// and B fragment is one aligned 16-byte LDS read.  Wavefront w owns the taps
+// t = w (mod 4); the bias gradient is the extra "tap" 27 with B = 1.
+// gp: BF16 [N][COUT]; a: BF16 [N][CIN]; part[block][e] as in k_reduce_parts.
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void k_wgrad(int M, const float *__restrict__ gp, const uint16_t *__restrict__ a,
-                                               float *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_wgrad_mfma(int M, const uint16_t *__restrict__ gp,
+                                                    const uint16_t *__restrict__ a, float *__restrict__ part) {
   constexpr int NE = COUT * kTaps * CIN + COUT;
-  constexpr int TX = 16, TY = 4, TZ = 4;
-  __shared__ float sg[TX * TY * TZ * COUT];
-  __shared__ float sa[(TX + 2) * (TY + 2) * (TZ + 2) * CIN];
-  const int ntx = (M + TX - 1) / TX, nty = (M + TY - 1) / TY, ntz = (M + TZ - 1) / TZ;
-  const int ntile = ntx * nty * ntz;
-  constexpr int EPT = (NE + 255) / 256;
-  float acc[EPT];
+  constexpr int NT = 7;  // taps per wavefront (27 taps + bias over 4 wavefronts)
+  __shared__ __attribute__((aligned(16))) uint16_t sa[3 * 16 * kHZ * kHY * kBX];
+  __shared__ __attribute__((aligned(16))) uint16_t sg[16 * kBZ * kBY * kBX];
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, row = l & 15, g = l >> 4;
+  for (int i = tid; i < 3 * 16 * kHZ * kHY * kBX; i += 256) sa[i] = 0;
+  for (int i = tid; i < 16 * kBZ * kBY * kBX; i += 256) sg[i] = 0;
+  floatx4 acc[NT];
 #pragma unroll
-  for (int k = 0; k < EPT; k++) acc[k] = 0.f;
-  for (int tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    const int X0 = tx * TX, Y0 = ty * TY, Z0 = tz * TZ;
+  for (int e = 0; e < NT; e++) acc[e] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nbx = (M + kBX - 1) / kBX, nby = (M + kBY - 1) / kBY, nbz = (M + kBZ - 1) / kBZ;
+  const int nbricks = nbx * nby * nbz;
+  for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+    const int X0 = (b % nbx) * kBX, Y0 = ((b / nbx) % nby) * kBY, Z0 = (b / (nbx * nby)) * kBZ;
     __syncthreads();
-    for (int q = threadIdx.x; q < TX * TY * TZ * COUT; q += 256) {
-      const int c = q % COUT, n = q / COUT;
-      const int xx = X0 + n % TX, yy = Y0 + (n / TX) % TY, zz = Z0 + n / (TX * TY);
-      sg[q] = (xx < M && yy < M && zz < M) ? gp[(xx + (int64_t)M * (yy + (int64_t)M * zz)) * COUT + c] : 0.f;
-    }
-    constexpr int HX = TX + 2, HY = TY + 2, HZ = TZ + 2;
-    for (int q = threadIdx.x; q < HX * HY * HZ * CIN; q += 256) {
-      const int c = q % CIN, n = q / CIN;
-      const int xx = X0 - 1 + n % HX, yy = Y0 - 1 + (n / HX) % HY, zz = Z0 - 1 + n / (HX * HY);
-      sa[q] = (xx >= 0 && yy >= 0 && zz >= 0 && xx < M && yy < M && zz < M)
-                  ? bf2f(a[(xx + (int64_t)M * (yy + (int64_t)M * zz)) * CIN + c])
-                  : 0.f;
-    }
-    __syncthreads();
+    // input brick with halo, three x-shifted copies
+    for (int q = tid; q < kHalo * (CIN / 8); q += 256) {
+      const int n = q / (CIN / 8), cb = 8 * (q % (CIN / 8));
+      const int hx = n % kHX, hy = (n / kHX) % kHY, hz = n / (kHX * kHY);
+      const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
+      u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M)
+        v = *reinterpret_cast<const u16x8 *>(a + (x + (int64_t)M * (y + (int64_t)M * z)) * CIN + cb);
 #pragma unroll
-    for (int k = 0; k < EPT; k++) {
-      const int e = threadIdx.x + 256 * k;
-      if (e >= NE) continue;
-      float s = 0.f;
-      if (e < COUT * kTaps * CIN) {
-        const int c = e % CIN, t = (e / CIN) % kTaps, o = e / (CIN * kTaps);
-        int dx, dy, dz;
-        tap_off(t, dx, dy, dz);
-        for (int n = 0; n < TX * TY * TZ; n++) {
-          const int nx = n % TX, ny = (n / TX) % TY, nz = n / (TX * TY);
-          s += sg[n * COUT + o] * sa[((nx + 1 + dx) + HX * ((ny + 1 + dy) + HY * (nz + 1 + dz))) * CIN + c];
-        }
-      } else {
-        const int o = e - COUT * kTaps * CIN;
-        for (int n = 0; n < TX * TY * TZ; n++) s += sg[n * COUT + o];
+      for (int d = 0; d < 3; d++) {
+        const int xx = hx - d;  // copy d holds a(X0 + xx + d - 1, ...)
+        if (xx < 0 || xx >= kBX) continue;
+#pragma unroll
+        for (int j = 0; j < 8; j++) sa[(((d * 16 + cb + j) * kHZ + hz) * kHY + hy) * kBX + xx] = v[j];
       }
-      acc[k] += s;
+    }
+    // output gradient of the brick, channel-major
+    if (COUT == 16) {
+      for (int q = tid; q < kBX * kBY * kBZ * 2; q += 256) {
+        const int n = q >> 1, ob = 8 * (q & 1);
+        const int xx = n % kBX, yy = (n / kBX) % kBY, zz = n / (kBX * kBY);
+        const int x = X0 + xx, y = Y0 + yy, z = Z0 + zz;
+        u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (x < M && y < M && z < M)
+          v = *reinterpret_cast<const u16x8 *>(gp + (x + (int64_t)M * (y + (int64_t)M * z)) * 16 + ob);
+#pragma unroll
+        for (int j = 0; j < 8; j++) sg[(((ob + j) * kBZ + zz) * kBY + yy) * kBX + xx] = v[j];
+      }
+    } else {
+      for (int n = tid; n < kBX * kBY * kBZ; n += 256) {
+        const int xx = n % kBX, yy = (n / kBX) % kBY, zz = n / (kBX * kBY);
+        const int x = X0 + xx, y = Y0 + yy, z = Z0 + zz;
+        sg[(zz * kBY + yy) * kBX + xx] =
+            (x < M && y < M && z < M) ? gp[x + (int64_t)M * (y + (int64_t)M * z)] : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    // K chunks of 32 nodes: chunk ch = two x-lines (z = ch / 2, y = 2 (ch % 2) + g / 2), 8 nodes per lane
+#pragma unroll 2
+    for (int ch = 0; ch < (kBY * kBZ) / 2; ch++) {
+      const int zz = ch >> 1, yy = 2 * (ch & 1) + (g >> 1), x0 = 8 * (g & 1);
+      const u16x8 af = *reinterpret_cast<const u16x8 *>(sg + ((row * kBZ + zz) * kBY + yy) * kBX + x0);
+#pragma unroll
+      for (int e = 0; e < NT; e++) {
+        const int t = wave + 4 * e;
+        if (t > kTaps) continue;
+        u16x8 bf;
+        if (t == kTaps) {
+          bf = u16x8{0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80};  // 1.0
+        } else {
+          int dx, dy, dz;
+          tap_off(t, dx, dy, dz);
+          bf = *reinterpret_cast<const u16x8 *>(
+              sa + ((((dx + 1) * 16 + row) * kHZ + (zz + 1 + dz)) * kHY + (yy + 1 + dy)) * kBX + x0);
+        }
+        acc[e] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, af), __builtin_bit_cast(bf16x8, bf),
+                                                         acc[e], 0, 0, 0);
+      }
     }
   }
+  // C fragment: row o = 4 g + i, column c = l & 15
+  float *pb = part + (int64_t)blockIdx.x * NE;
+  const int c = l & 15;
 #pragma unroll
-  for (int k = 0; k < EPT; k++) {
-    const int e = threadIdx.x + 256 * k;
-    if (e < NE) part[(int64_t)blockIdx.x * NE + e] = acc[k];
+  for (int e = 0; e < NT; e++) {
+    const int t = wave + 4 * e;
+    if (t > kTaps) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int o = 4 * g + i;
+      if (o >= COUT) continue;
+      if (t < kTaps) {
+        if (c < CIN) pb[(o * kTaps + t) * CIN + c] = acc[e][i];
+      } else if (c == 0) {
+        pb[COUT * kTaps * CIN + o] = acc[e][i];
+      }
+    }
   }
 }
+
 // grads[e] += sum_b part[b][e] (fixed order)
 __global__ void k_reduce_parts(const float *__restrict__ part, int nb, int ne, float *__restrict__ grads) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,7 +417,7 @@ void LearnedNet::release() {
   fr((void *&)wbf); fr((void *&)wtbf); fr((void *&)part); fr((void *&)out);
   for (int l = 0; l < kNetLayers; l++) { fr((void *&)act[l]); fr((void *&)pre[l]); }
   fr((void *&)gbuf[0]); fr((void *&)gbuf[1]);
-  fr((void *&)gf); fr((void *&)lpart);
+  fr((void *&)lpart);
   cap = 0;
   part_cap = 0;
   ready = false;
@@ -387,7 +430,7 @@ void LearnedData::release() {
 }
 
 static const int kWStride = 16 * kTaps * 16;  // BF16 weight slots per layer (padded)
-static const int kWgradBlocks = 240;
+static const int kWgradBlocks = 512;
 static const int kLossBlocks = 1024;
 
 void net_init(LearnedNet &net, uint32_t seed, hipStream_t st) {
@@ -438,15 +481,12 @@ bool net_reserve(LearnedNet &net, int64_t N, bool training) {
       if (g) (void)hipFree(g);
       g = nullptr;
     }
-    if (net.gf) (void)hipFree(net.gf);
-    net.gf = nullptr;
     net.cap = N;
   }
   if (training && !net.pre[0]) {
     changed = true;
     for (int l = 0; l < kNetLayers; l++) check(hipMalloc(&net.pre[l], sizeof(float) * net.cap * kNetCh), "net pre");
     for (auto &g : net.gbuf) check(hipMalloc(&g, sizeof(uint16_t) * net.cap * kNetCh), "net grad buf");
-    check(hipMalloc(&net.gf, sizeof(float) * net.cap * kNetCh), "net grad scratch");
     if (!net.lpart) check(hipMalloc(&net.lpart, sizeof(double) * kLossBlocks), "net loss partials");
   }
   const int64_t pc = (int64_t)kWgradBlocks * (16 * kTaps * 16 + 16);
@@ -486,9 +526,9 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
 void net_zero_grads(LearnedNet &net, hipStream_t st) { fill<float>(net.grads, 0.f, net_total_params(), st); }
 
 template <int CIN, int COUT>
-static void wgrad(LearnedNet &net, int M, const float *gp, const uint16_t *a, int layer, hipStream_t st) {
+static void wgrad(LearnedNet &net, int M, const uint16_t *gp, const uint16_t *a, int layer, hipStream_t st) {
   constexpr int NE = COUT * kTaps * CIN + COUT;
-  hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(kWgradBlocks), dim3(256), 0, st, M, gp, a, net.part);
+  hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3(kWgradBlocks), dim3(256), 0, st, M, gp, a, net.part);
   hipLaunchKernelGGL(k_reduce_parts, dim3((NE + 255) / 256), dim3(256), 0, st, net.part, kWgradBlocks, NE,
                      net.grads + net_param_offset(layer));
 }
@@ -498,36 +538,31 @@ static void backward_layers(LearnedNet &net, int M, hipStream_t st);
 void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st) {
   const int64_t N = (int64_t)M * M * M;
   const double inner = (double)(M - 2) * (M - 2) * (M - 2);
-  float *gf = net.gf;
   double *lpart = net.lpart;
   uint16_t *g0 = net.gbuf[0];
-  // output layer: gy (BF16 in g0[:N], FP32 in gf[:N])
-  hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, M, net.out, t, (float)(1.0 / inner), g0, gf, lpart);
+  // output layer: gy (BF16 in g0[:N])
+  hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, M, net.out, t, (float)(1.0 / inner), g0, lpart);
   backward_layers(net, M, st);
   if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, lpart, kLossBlocks, loss_dev);
 }
 
-__global__ void k_split_grad(const float *__restrict__ gy, uint16_t *__restrict__ gb, float *__restrict__ gf,
-                             int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+__global__ void k_split_grad(const float *__restrict__ gy, uint16_t *__restrict__ gb, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     gb[i] = f2bf(gy[i]);
-    gf[i] = gy[i];
-  }
 }
 
 void net_backward_from(LearnedNet &net, int M, const float *gy, hipStream_t st) {
   const int64_t N = (int64_t)M * M * M;
   hipLaunchKernelGGL(k_split_grad, dim3((unsigned)std::min<int64_t>((N + 255) / 256, 4096)), dim3(256), 0, st, gy,
-                     net.gbuf[0], net.gf, N);
+                     net.gbuf[0], N);
   backward_layers(net, M, st);
 }
 
-// the layers' backward pass, from gy in gbuf[0] (BF16) and gf (FP32)
+// the layers' backward pass, from gy in gbuf[0] (BF16)
 static void backward_layers(LearnedNet &net, int M, hipStream_t st) {
   const int64_t N = (int64_t)M * M * M;
-  float *gf = net.gf;
   uint16_t *g0 = net.gbuf[0], *g1 = net.gbuf[1];
-  wgrad<16, 1>(net, M, gf, net.act[3], 3, st);
+  wgrad<16, 1>(net, M, g0, net.act[3], 3, st);
   const unsigned g = conv_grid(M);
   // input gradient of layer 3's output: conv of gy with transposed weights (1 -> 16)
   hipLaunchKernelGGL((k_conv<1, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + 3 * kWStride, nullptr, nullptr,
@@ -535,9 +570,9 @@ static void backward_layers(LearnedNet &net, int M, hipStream_t st) {
   for (int l = 2; l >= 0; l--) {
     // through GELU of layer l
     hipLaunchKernelGGL(k_gelu_back, dim3((unsigned)((N * kNetCh + 255) / 256)), dim3(256), 0, st, g1, net.pre[l], g0,
-                       gf, N * kNetCh);
-    if (l == 0) wgrad<kNetIn, 16>(net, M, gf, net.act[0], 0, st);
-    else wgrad<16, 16>(net, M, gf, net.act[l], l, st);
+                       N * kNetCh);
+    if (l == 0) wgrad<kNetIn, 16>(net, M, g0, net.act[0], 0, st);
+    else wgrad<16, 16>(net, M, g0, net.act[l], l, st);
     if (l > 0)
       hipLaunchKernelGGL((k_conv<16, 16, 2>), dim3(g), dim3(256), 0, st, M, g0, net.wtbf + l * kWStride, nullptr,
                          nullptr, g1, nullptr);

@@ -45,7 +45,6 @@ struct LearnedNet {
   float *part = nullptr;                  // weight-gradient partial sums
   int64_t part_cap = 0;
   float *out = nullptr;                   // network output [N]
-  float *gf = nullptr;                    // FP32 gradient of a layer output [N][16]
   double *lpart = nullptr;                // loss partial sums
   void release();
 };
