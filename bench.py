@@ -109,8 +109,8 @@ def transient(args):
             p1 = prm.copy()
             p1[P.TIME] = tau
             S.tcd_step(p1, td, P.solver_opt(**dict(base, precision=1)))
-            tinfo = S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=16, steps=args.train_steps,
-                                    lr=2e-3, seed=1, levels=args.learned_levels)
+            tinfo = S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=32, steps=args.train_steps,
+                                    lr=2e-3, seed=1, levels=args.learned_levels, data=1)
             train = {"seconds": round(float(tinfo[2]), 3), "loss_first": round(float(tinfo[0]), 4),
                      "loss_last": round(float(tinfo[1]), 4), "levels": args.learned_levels}
             S.tcd_init(prm, None)
@@ -162,7 +162,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="learned,fp32,fp64")
     ap.add_argument("--learned-levels", type=int, default=1)
-    ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--train-steps", type=int, default=1200)
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
                     help="run the TCD3D 128^3 Q1 transient workload for STEPS time steps instead")
@@ -201,12 +201,13 @@ def main():
     train = None
     if "learned" in forms:
         t0 = time.perf_counter()
-        tinfo = S.learned_train(opt(1), samples=16, steps=args.train_steps, lr=2e-3, seed=1,
-                                levels=args.learned_levels)
+        tinfo = S.learned_train(opt(1), samples=32, steps=args.train_steps, lr=2e-3, seed=1,
+                                levels=args.learned_levels, data=1)
         train = {"seconds": round(float(tinfo[2]), 3), "wall_s": round(time.perf_counter() - t0, 3),
                  "adam_steps": int(tinfo[3]), "levels": args.learned_levels,
                  "loss_first": round(float(tinfo[0]), 4), "loss_last": round(float(tinfo[1]), 4),
-                 "samples_per_level": 16, "loss": "scaled defect left after the correction"}
+                 "samples_per_level": 32, "samples": "zero start, random right-hand side",
+                 "loss": "scaled defect left after the correction"}
         print(f"[bench] learned smoother trained: {train}", file=sys.stderr, flush=True)
 
     def step(form):

@@ -187,10 +187,12 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
 int pmx_learned_adam(pmx_system *sys, double lr);
 /* train on pairs from plain cycles (opt: the cycle options of pmx_system_solve).
- * topt: [0] samples per level, [1] Adam steps per level (levels interleaved),
+ * topt (8 doubles, unused entries 0): [0] samples per level, [1] Adam steps per level (levels interleaved),
  * [2] learning rate, [3] seed, [4] finest levels that use the network (0: all
  * but the coarsest), [5] loss: 0 squared error against the cycle's error,
- * 1 the scaled defect left after the correction (as pmx_learned_gradient_level).
+ * 1 the scaled defect left after the correction (as pmx_learned_gradient_level),
+ * [6] samples: 0 a random start of the homogeneous problem, 1 a zero start
+ * with a random right-hand side, 2 half each.
  * tinfo (16 doubles): [0] first / [1] last mean loss of the finest network,
  * [2] s, [3] total steps, [4 + 2j] / [5 + 2j] first / last mean loss of the
  * network of level L - j. */

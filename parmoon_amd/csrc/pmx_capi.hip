@@ -110,7 +110,7 @@ struct System {
   struct Graph {
     bool valid = false;
     int prec = 0;
-    double key[10] = {0};
+    double key[11] = {0};
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[2];
@@ -422,9 +422,11 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(v.f.p, vin, lv.L.N, st);
   const int pi = std::is_same<T, double>::value ? 0 : 1;
   System::Graph &g = Y.graphs[pi];
-  double key[10];
+  double key[11];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
-  key[9] = (pi == 1 && Y.learn_active) ? Y.learned_levels : 0;
+  const bool learned = pi == 1 && Y.learn_active;
+  key[9] = learned ? Y.learned_levels : 0;
+  key[10] = learned ? std::max(1.0, Y.opt[20]) : 0;
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -682,7 +684,8 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
 // topt: [0] samples per level, [1] Adam steps per level (levels interleaved),
 // [2] learning rate, [3] seed, [4] finest levels using the network (default:
 // all but the coarsest), [5] loss: 0 squared error against -x, 1 the scaled
-// defect left after the correction (learned_res_step).  tinfo: [0] mean loss
+// defect left after the correction (learned_res_step), [6] samples: 0 random
+// start and f = 0, 1 zero start and a random f, 2 half each.  tinfo: [0] mean loss
 // of the first 10 steps per level, [1] of the last ones, [2] seconds, [3] steps.
 static void learned_train(System &Y, const double *to, double *ti) {
   hipStream_t st = Y.ctx->st;
@@ -693,6 +696,7 @@ static void learned_train(System &Y, const double *to, double *ti) {
   const int L = (int)Y.lev.size() - 1;
   Y.learned_levels = to[4] > 0 ? std::min((int)to[4], L) : L;
   const bool residual_loss = to[5] != 0.0;
+  const int data_mode = (int)to[6];  // 0 random error, f = 0; 1 x = 0, random f; 2 both
   const int nl = Y.learned_levels;
   Y.trained_levels = nl;
   Y.learn_active = false;
@@ -723,8 +727,17 @@ static void learned_train(System &Y, const double *to, double *ti) {
     const int M = lv.L.M;
     const double sq = std::sqrt((double)lv.L.nactive);
     for (int k = 0; k < ns; k++) {
-      learned_random(v.x.p, M, seed * 7919u + 104729u * (uint32_t)j + (uint32_t)k, st);
-      fill<float>(v.f.p, 0.0f, N, st);
+      const uint32_t sk = seed * 7919u + 104729u * (uint32_t)j + (uint32_t)k;
+      const bool zero_start = data_mode == 1 || (data_mode == 2 && (k & 1));
+      if (zero_start) {
+        // as inside a preconditioner application: x = 0, a random right-hand side
+        fill<float>(v.x.p, 0.0f, N, st);
+        learned_random(v.f.p, M, sk, st);
+      } else {
+        // a random error of the homogeneous problem
+        learned_random(v.x.p, M, sk, st);
+        fill<float>(v.f.p, 0.0f, N, st);
+      }
       mg_cycle<float>(Y, L - j, false);
       css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
       boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
