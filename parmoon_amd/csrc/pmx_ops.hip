@@ -210,11 +210,18 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
 // TCD3D.C:37-322), and adds it into its own CSS row.  No two threads write
 // the same row, so no atomics and no colouring of cells are needed, and the
 // writes of a wavefront are coalesced.
-template <int ORDER, int EX, int EY, int EZ>
+// The incident cells per direction are compile-time (a Q2 node with an odd
+// coordinate lies in one cell along it, every other node in two), so the
+// loops are uniform across the wavefront and the basis tables are read with
+// scalar loads; TWO selects the second matrix of the SUPG TCD form.
+template <int ORDER, int EX, int EY, int EZ, bool TWO>
 __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M, int n, int nq, int form,
                                              const double *__restrict__ qp, const double *__restrict__ tab,
                                              double *__restrict__ val, double *__restrict__ val2,
                                              double *__restrict__ rhs) {
+  constexpr int NCX = (ORDER == 2 && EX == 1) ? 1 : 2;
+  constexpr int NCY = (ORDER == 2 && EY == 1) ? 1 : 2;
+  constexpr int NCZ = (ORDER == 2 && EZ == 1) ? 1 : 2;
   constexpr int K = ORDER + 1, NB = K * K * K;
   const int i = idx % s.cx;
   const int t = idx / s.cx;
@@ -223,17 +230,16 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   const int p = px + M * (py + M * pz);
   const double g = 2.0 * n;  // reference -> physical derivative factor 2/h
   double *v = val + vbase(s, idx);
-  double *v2 = val2 ? val2 + vbase(s, idx) : nullptr;
+  double *v2 = TWO ? val2 + vbase(s, idx) : nullptr;
   constexpr int cnt = kSlice;
   double frhs = 0;
   // incident cells per direction: lattice coordinate x lies in cells
   // floor((x-1)/ORDER) .. floor(x/ORDER) (two cells at a vertex plane)
-  const int cx0 = (px - 1) / ORDER, cx1 = px / ORDER;
-  const int cy0 = (py - 1) / ORDER, cy1 = py / ORDER;
-  const int cz0 = (pz - 1) / ORDER, cz1 = pz / ORDER;
-  for (int cz = cz0; cz <= cz1; cz++)
-    for (int cy = cy0; cy <= cy1; cy++)
-      for (int cx = cx0; cx <= cx1; cx++) {
+  const int cx0 = (px - 1) / ORDER, cy0 = (py - 1) / ORDER, cz0 = (pz - 1) / ORDER;
+  for (int iz = 0; iz < NCZ; iz++)
+    for (int iy = 0; iy < NCY; iy++)
+      for (int ix = 0; ix < NCX; ix++) {
+        const int cx = cx0 + ix, cy = cy0 + iy, cz = cz0 + iz;
         const int ax = px - ORDER * cx, ay = py - ORDER * cy, az = pz - ORDER * cz;
         const int a = ax + K * (ay + K * az);
         const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
@@ -266,7 +272,7 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
           for (int b = 0; b < NB; b++) {
             const double *Tb = Tq + 4 * b;
             acc[b] += u0 * Tb[0] + ux * Tb[1] + uy * Tb[2] + uz * Tb[3];
-            acc2[b] += k0 * Tb[0];
+            if (TWO) acc2[b] += k0 * Tb[0];
           }
         }
         // add the row into the CSS slots of node p
@@ -276,13 +282,13 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
           const int dx = ORDER * cx + bx - px, dy = ORDER * cy + by - py, dz = ORDER * cz + bz - pz;
           const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
           v[slot * cnt] += acc[b];
-          if (v2) v2[slot * cnt] += acc2[b];
+          if (TWO) v2[slot * cnt] += acc2[b];
         }
       }
   rhs[p] = frhs;
 }
 
-template <int ORDER>
+template <int ORDER, bool TWO>
 __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
                                                      const DevSegment *__restrict__ segs, int M, int n, int nq,
                                                      const double *__restrict__ prm, const double *__restrict__ qp,
@@ -294,19 +300,19 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__r
   if (idx >= s.count) return;
   const int form = (int)prm[PMX_PRM_FORM];
   if (ORDER == 1) {
-    assemble_row<1, 1, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs);
+    assemble_row<1, 1, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs);
     return;
   }
   const int code = s.ex * 100 + s.ey * 10 + s.ez;
   switch (code) {
-    case 111: assemble_row<2, 1, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 211: assemble_row<2, 2, 1, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 121: assemble_row<2, 1, 2, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 221: assemble_row<2, 2, 2, 1>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 112: assemble_row<2, 1, 1, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 212: assemble_row<2, 2, 1, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 122: assemble_row<2, 1, 2, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    default: assemble_row<2, 2, 2, 2>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 111: assemble_row<2, 1, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 211: assemble_row<2, 2, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 121: assemble_row<2, 1, 2, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 221: assemble_row<2, 2, 2, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 112: assemble_row<2, 1, 1, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 212: assemble_row<2, 2, 1, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 122: assemble_row<2, 1, 2, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    default: assemble_row<2, 2, 2, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
   }
 }
 // Dirichlet rhs: RHS[dof] = g(node) (AssembleMat3D.C:1031-1096)
@@ -642,12 +648,17 @@ void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int
                      nq, D.n, qpbuf);
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
-  if (D.order == 1)
-    hipLaunchKernelGGL(k_assemble<1>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, D.n,
-                       nq, prm_dev, qpbuf, tab, val, val2, rhs);
-  else
-    hipLaunchKernelGGL(k_assemble<2>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, D.n,
-                       nq, prm_dev, qpbuf, tab, val, val2, rhs);
+#define PMX_ASM(O, TW)                                                                                      \
+  hipLaunchKernelGGL((k_assemble<O, TW>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, \
+                     D.M, D.n, nq, prm_dev, qpbuf, tab, val, val2, rhs)
+  if (D.order == 1) {
+    if (val2) PMX_ASM(1, true);
+    else PMX_ASM(1, false);
+  } else {
+    if (val2) PMX_ASM(2, true);
+    else PMX_ASM(2, false);
+  }
+#undef PMX_ASM
   hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
                      D.nboundary, D.M, prm_dev, rhs);
 }
