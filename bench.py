@@ -17,6 +17,7 @@ with the plain FP64 time, assembly DOF/s and the SpMV roofline alongside.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nref 6] [--no-cpu]
     python bench.py --profile-kernels     # SpMV + a 1 GiB copy only (rocprofv3 --pmc runs)
     python bench.py --transient 200       # TCD3D 128^3 Q1, Crank-Nicolson, 200 steps
+    python bench.py --train-set 256 --train-steps 16   # networks across 256 seeded 64^3 Q1 fields
 """
 import argparse
 import json
@@ -153,6 +154,65 @@ def transient(args):
     ctx.close()
 
 
+def train_set(args):
+    """Training-set workload: learned-smoother networks trained across seeded
+    coefficient fields on 64^3 Q1 (274,625 DOFs): eps = exp(G) with G a
+    Gaussian random field (sigma 1, correlation length 0.1), b = (1,2,3)/sqrt(14)
+    scaled to a cell Peclet number log-uniform in [10, 1000], SUPG; pairs from
+    the three finest levels of the plain cycle; Adam.  Reports training
+    samples/s and convolution GFLOP/s, then the FGMRES solve of held-out
+    fields with and without the networks.  Prints one JSON line."""
+    import parmoon_amd as pa
+    from parmoon_amd import prm as P
+    ctx = pa.Context(0)
+    S = pa.System(ctx, 2, 5, 1)
+    nodes = [int(S.dims(l)[0]) for l in range(5)]
+    nseeds, per_seed = args.train_set, args.train_steps
+    rng = np.random.default_rng(0)
+    pe = 10 ** rng.uniform(1, 3, nseeds + args.eval_seeds)
+    base = dict(solver=16, restart=20, maxit=400, res_norm_min=0.0, red_factor=0.0, smoother=3, omega=1.0,
+                pre=2, post=2, rel_target=1e-10)
+    S.learned_init(1)
+    t_train, steps, flops = 0.0, 0, 0.0
+    fwd = [2.0 * 27 * n * (8 * 16 + 16 * 16 + 16 * 16 + 16) for n in nodes]  # forward flops per sample
+    for k in range(nseeds):
+        S.assemble(P.lognormal(seed=k, nref=6, peclet=pe[k]))
+        info = S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=8, steps=per_seed, lr=1e-3,
+                               seed=k, levels=3, data=1)
+        t_train += float(info[2])
+        steps += int(info[3])
+        flops += per_seed * 3.0 * (fwd[4] + fwd[3] + fwd[2])  # forward + backward (about twice the forward)
+        if (k + 1) % 16 == 0:
+            print(f"[bench] train-set: {k + 1}/{nseeds} fields, {t_train:.1f}s", file=sys.stderr, flush=True)
+    evals = []
+    for k in range(nseeds, nseeds + args.eval_seeds):
+        S.assemble(P.lognormal(seed=k, nref=6, peclet=pe[k]))
+        row = {"peclet": round(float(pe[k]), 1)}
+        for name, kw in (("fp64", dict(precision=0)), ("fp32", dict(precision=1)),
+                         ("learned1", dict(precision=2, learned_levels=1)),
+                         ("learned3", dict(precision=2, learned_levels=3))):
+            info, rc = S.solve_resident(P.solver_opt(**dict(base, **kw)), check=False)
+            row[name] = {"rc": int(rc), "iters": int(info[0]), "s": round(float(info[4]), 4)}
+        evals.append(row)
+        print(f"[bench] held-out field: {row}", file=sys.stderr, flush=True)
+    gm = lambda key: float(np.exp(np.mean([np.log(r["fp64"]["s"] / r[key]["s"]) for r in evals])))
+    out = {"metric": "learned-smoother training throughput (samples/s) on 64^3 Q1 seeded fields",
+           "value": round(steps / t_train, 1), "unit": "samples/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+           "ms_per_step": round(t_train / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic (seeded Gaussian random fields)",
+           "config": {"workload": f"{nseeds} lognormal fields 64^3 Q1 SUPG, Pe log-uniform 10..1000, "
+                                  f"{per_seed} Adam steps per field and level, levels 64^3/32^3/16^3, batch 1",
+                      "parallelism": "1 GPU"},
+           "conv_GFLOPs": round(flops / t_train / 1e9, 1),
+           "train_seconds": round(t_train, 3),
+           "note": "seconds include generating the samples with the plain cycle",
+           "held_out": evals,
+           "held_out_speedup_vs_fp64_geomean": {k: round(gm(k), 3) for k in ("fp32", "learned1", "learned3")}}
+    print(json.dumps(out))
+    S.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +226,9 @@ def main():
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
                     help="run the TCD3D 128^3 Q1 transient workload for STEPS time steps instead")
+    ap.add_argument("--train-set", type=int, default=0, metavar="FIELDS",
+                    help="train the learned smoother across FIELDS seeded 64^3 Q1 fields instead")
+    ap.add_argument("--eval-seeds", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -173,6 +236,9 @@ def main():
         sys.exit(2)
     if args.transient > 0:
         transient(args)
+        return
+    if args.train_set > 0:
+        train_set(args)
         return
 
     import parmoon_amd as pa
