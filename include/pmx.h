@@ -160,11 +160,12 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
 /* ---- learned post-smoother (no counterpart in the reference; it replaces
  * the post-sweeps of TMGLevel3D::SOR/SSOR, src/FE/MGLevel3D.C:280-467, on the
  * finest levels of FP32 cycles).  Network: 4 conv layers 3x3x3 with
- * 8 -> 16 -> 16 -> 16 -> 1 channels and GELU, BF16 operands with FP32
+ * 16 -> 16 -> 16 -> 16 -> 1 channels and GELU, BF16 operands with FP32
  * accumulation.  Parameters are FP32, per layer W[cout][27][cin] then b[cout],
  * tap t = (dx+1) + 3 (dy+1) + 9 (dz+1).  Lattice arrays are x-fastest M^3.
- * Network input per node (8 channels): the scaled defect, then
- * a(p,p+e_x)/a(p,p), a(p,p-e_x)/a(p,p), the same for y and z, and 0. */
+ * Network input per node (16 channels): the scaled defect, then
+ * a(p,p+e)/a(p,p) for e = +x, -x, +y, -y, +z, -z, then a(p,p+2e)/a(p,p) for
+ * the same six e (0 where the stencil has no such entry), then three zeros. */
 int pmx_learned_nparams(int *n);
 int pmx_learned_init(pmx_system *sys, unsigned seed);
 /* Each level above the coarsest has its own network.  init: fresh draws for
@@ -173,10 +174,10 @@ int pmx_learned_init(pmx_system *sys, unsigned seed);
  * finest level's network. */
 int pmx_learned_params(pmx_system *sys, int set, float *params);
 int pmx_learned_params_level(pmx_system *sys, int level, int set, float *params);
-/* network output on an M^3 lattice; in: [M^3][8] input channels */
+/* network output on an M^3 lattice; in: [M^3][16] input channels */
 int pmx_learned_forward(pmx_system *sys, int M, const float *in, float *out);
 /* loss = sum over interior nodes (out - target)^2 / (M-2)^3 and its gradient
- * with respect to the parameters (fresh, not accumulated); in: [M^3][8] */
+ * with respect to the parameters (fresh, not accumulated); in: [M^3][16] */
 int pmx_learned_gradient(pmx_system *sys, int M, const float *in, const float *target, float *grads,
                          double *loss);
 /* residual loss on a level: c = network output on the level's lattice (0 on

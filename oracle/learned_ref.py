@@ -11,11 +11,13 @@ algorithm to follow; the oracle states the network itself:
 * tap t = (dx+1) + 3 (dy+1) + 9 (dz+1); nodes outside the lattice read as 0;
 * layer l: pre = sum_t sum_c W[o, t, c] a[z+dz, y+dy, x+dx, c] + b[o];
   hidden layers a_{l+1} = GELU(pre) (exact erf form), the last layer is linear;
-* channels 8 -> 16 -> 16 -> 16 -> 1; parameters packed per layer as
+* channels 16 -> 16 -> 16 -> 16 -> 1; parameters packed per layer as
   W[cout][27][cin] followed by b[cout], FP32 on the device;
-* input channels per node: the scaled defect u, then the six nearest
-  couplings of the row divided by its diagonal, a(p,p+e)/a(p,p) for
-  e = +x, -x, +y, -y, +z, -z (0 on Dirichlet nodes), then 0 (``features``);
+* input channels per node: the scaled defect u, then the couplings of the
+  row along the axes divided by its diagonal, a(p,p+e)/a(p,p) for
+  e = +x, -x, +y, -y, +z, -z, then a(p,p+2e)/a(p,p) for the same e (0 where
+  the stencil has no such entry and on Dirichlet nodes), then three zeros
+  (``features``);
 * BF16 emulation (``bf16=True``): the input, every weight and every hidden
   activation is rounded to BF16 (round to nearest even) before it enters a
   convolution, the products are summed exactly (FP64 here, FP32 on MFMA);
@@ -31,7 +33,7 @@ import numpy as np
 from scipy.special import erf
 
 TAPS = 27
-CH = [8, 16, 16, 16, 1]
+CH = [16, 16, 16, 16, 1]
 
 
 def offsets():
@@ -131,33 +133,39 @@ def conv_weight_grad(g, a):
 
 
 def features(A, M):
-    """(M, M, M, 7) operator features of a lattice-ordered sparse matrix A:
-    a(p, p + e) / a(p, p) for e = +x, -x, +y, -y, +z, -z on interior nodes,
-    then a zero channel."""
+    """(M, M, M, 15) operator features of a lattice-ordered sparse matrix A:
+    a(p, p + e) / a(p, p) and a(p, p + 2e) / a(p, p) for e = +x, -x, +y, -y,
+    +z, -z on interior nodes (0 where A has no such entry), then three zero
+    channels."""
     A = A.tocsr()
     N = M ** 3
     d = A.diagonal()
     m = interior_mask(M).ravel()
-    out = np.zeros((N, 7))
+    out = np.zeros((N, 15))
     idx = np.nonzero(m)[0]
     for q, off in enumerate((1, -1, M, -M, M * M, -M * M)):
         out[idx, q] = np.asarray(A[idx, idx + off]).ravel() / d[idx]
-    return out.reshape(M, M, M, 7)
+        j = idx + 2 * off
+        ok = (j >= 0) & (j < N)
+        vals = np.zeros(idx.size)
+        vals[ok] = np.asarray(A[idx[ok], j[ok]]).ravel()
+        out[idx, 6 + q] = vals / d[idx]
+    return out.reshape(M, M, M, 15)
 
 
 def net_input(u, feat=None):
-    """Stack the scaled defect (M, M, M) and the features (M, M, M, 7) into the
-    (M, M, M, 8) network input (features 0 if not given)."""
+    """Stack the scaled defect (M, M, M) and the features (M, M, M, 15) into
+    the (M, M, M, 16) network input (features 0 if not given)."""
     u = np.asarray(u, np.float64)
     if u.ndim == 4:
         return u
-    f = np.zeros(u.shape + (7,)) if feat is None else feat
+    f = np.zeros(u.shape + (CH[0] - 1,)) if feat is None else feat
     return np.concatenate([u[..., None], f], axis=-1)
 
 
 def forward(p, u, bf16_emulation=True, keep=False, feat=None):
-    """Network output (M, M, M) for the input u: (M, M, M, 8), or the scaled
-    defect (M, M, M) with the features `feat` (M, M, M, 7) (0 if None)."""
+    """Network output (M, M, M) for the input u: (M, M, M, 16), or the scaled
+    defect (M, M, M) with the features `feat` (M, M, M, 15) (0 if None)."""
     r = bf16 if bf16_emulation else (lambda z: np.asarray(z, np.float64))
     layers = unpack(p)
     a = r(net_input(u, feat))

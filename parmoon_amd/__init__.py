@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpmx.so")
 
 PRM_LEN = 512
+LEARNED_IN = 16  # input channels of the learned smoother's network
 _lib = None
 
 
@@ -289,19 +290,19 @@ class System:
         return out
 
     def learned_forward(self, u):
-        """Network output (M, M, M) for the input u (M, M, M, 8) indexed [z, y, x, channel]."""
+        """Network output (M, M, M) for the input u (M, M, M, 16) indexed [z, y, x, channel]."""
         u = np.ascontiguousarray(u, np.float32)
-        if u.ndim != 4 or u.shape[3] != 8:
-            raise ValueError("input must be (M, M, M, 8)")
+        if u.ndim != 4 or u.shape[3] != LEARNED_IN:
+            raise ValueError(f"input must be (M, M, M, {LEARNED_IN})")
         out = np.zeros(u.shape[:3], np.float32)
         _ok(lib().pmx_learned_forward(self.h, u.shape[0], _p(u), _p(out)), "pmx_learned_forward")
         return out
 
     def learned_gradient(self, u, t):
-        """Squared-error gradient for the input u (M, M, M, 8) and target t (M, M, M)."""
+        """Squared-error gradient for the input u (M, M, M, 16) and target t (M, M, M)."""
         u = np.ascontiguousarray(u, np.float32)
-        if u.ndim != 4 or u.shape[3] != 8:
-            raise ValueError("input must be (M, M, M, 8)")
+        if u.ndim != 4 or u.shape[3] != LEARNED_IN:
+            raise ValueError(f"input must be (M, M, M, {LEARNED_IN})")
         t = np.ascontiguousarray(t, np.float32)
         g = np.zeros(self.learned_nparams(), np.float32)
         loss = np.zeros(1)

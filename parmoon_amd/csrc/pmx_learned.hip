@@ -615,9 +615,12 @@ __global__ void k_build_input(const float *__restrict__ u, const double *__restr
     sc = nv > 0 ? (float)(sq / nv) : 0.f;
   }
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    u16x8 v = *reinterpret_cast<const u16x8 *>(feat + i * kNetIn);
-    v[0] = f2bf(u[i] * sc);
-    *reinterpret_cast<u16x8 *>(act0 + i * kNetIn) = v;
+#pragma unroll
+    for (int piece = 0; piece < kNetIn / 8; piece++) {
+      u16x8 v = *reinterpret_cast<const u16x8 *>(feat + i * kNetIn + 8 * piece);
+      if (piece == 0) v[0] = f2bf(u[i] * sc);
+      *reinterpret_cast<u16x8 *>(act0 + i * kNetIn + 8 * piece) = v;
+    }
   }
 }
 // the six nearest couplings of every interior row, divided by its diagonal
@@ -634,11 +637,20 @@ __global__ void k_features(const BlockInfo *__restrict__ blocks, const DevSegmen
   auto slot = [&](int dx, int dy, int dz) { return (dx + s.ex) + wx * ((dy + s.ey) + wy * (dz + s.ez)); };
   const float dinv = 1.0f / val[vb + (int64_t)s.dslot * kSlice];
   const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-  u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int ext[3] = {s.ex, s.ey, s.ez};
+  uint16_t f[16] = {0};
 #pragma unroll
-  for (int q = 0; q < 6; q++)
-    v[1 + q] = f2bf(val[vb + (int64_t)slot(off[q][0], off[q][1], off[q][2]) * kSlice] * dinv);
-  *reinterpret_cast<u16x8 *>(feat + p * kNetIn) = v;
+  for (int q = 0; q < 6; q++) {
+    f[1 + q] = f2bf(val[vb + (int64_t)slot(off[q][0], off[q][1], off[q][2]) * kSlice] * dinv);
+    const int axis = q >> 1;
+    if (ext[axis] >= 2)
+      f[7 + q] = f2bf(val[vb + (int64_t)slot(2 * off[q][0], 2 * off[q][1], 2 * off[q][2]) * kSlice] * dinv);
+  }
+  u16x8 v0, v1;
+#pragma unroll
+  for (int j = 0; j < 8; j++) { v0[j] = f[j]; v1[j] = f[8 + j]; }
+  *reinterpret_cast<u16x8 *>(feat + p * kNetIn) = v0;
+  *reinterpret_cast<u16x8 *>(feat + p * kNetIn + 8) = v1;
 }
 
 // MODE 0: c = in * nrm / sq;  MODE 1: c = sign * in * sq / nrm  (interior only)

@@ -1,9 +1,10 @@
 // pmx_learned.hpp -- the learned post-smoother: a 4-layer 3x3x3 convolution
-// network (8 -> 16 -> 16 -> 16 -> 1 channels, GELU between layers) that maps
+// network (16 -> 16 -> 16 -> 16 -> 1 channels, GELU between layers) that maps
 // a diagonally scaled defect to a correction on the node lattice of a level.
-// Input channels per node: [u, a(p,p+ex)/a(p,p), a(p,p-ex)/a(p,p), (same for
-// y and z), 0] -- the scaled defect and the six nearest couplings of the row,
-// which tell the network the local convection direction and strength.
+// Input channels per node: [u, a(p,p+e)/a(p,p) for e = +x,-x,+y,-y,+z,-z,
+// a(p,p+2e)/a(p,p) for the same six directions (0 where the row has no such
+// coupling), 0, 0, 0] -- the scaled defect and the couplings of the row along
+// the axes, which tell the network the local convection direction and strength.
 // BF16 weights and activations, FP32 accumulation on MFMA
 // (v_mfma_f32_16x16x32_bf16), hand-written forward, backward and Adam kernels.
 // No counterpart in the reference (its smoothers are MGLevel3D.C:280-563).
@@ -17,7 +18,7 @@ namespace pmx {
 
 constexpr int kNetLayers = 4;
 constexpr int kNetCh = 16;   // hidden channels
-constexpr int kNetIn = 8;    // input channels: scaled defect + 6 operator features + 0
+constexpr int kNetIn = 16;   // input channels: scaled defect + 12 operator features + 3 zero
 constexpr int kTaps = 27;    // 3x3x3 stencil of a layer
 
 // layer l maps cin(l) -> cout(l) channels
@@ -84,8 +85,10 @@ void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipSt
 // FP64 2-norm of u; nrm == nullptr: no scaling)
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
                          int64_t n, hipStream_t st);
-// operator features of a level: feat[p][1..6] = bf16(a(p, p +- e_k) / a(p,p)) on
-// interior nodes (x+, x-, y+, y-, z+, z-), channels 0 and 7 and boundary nodes 0
+// operator features of a level: feat[p][1..6] = bf16(a(p, p +- e_k) / a(p,p)),
+// feat[p][7..12] = bf16(a(p, p +- 2 e_k) / a(p,p)) on interior nodes (order x+,
+// x-, y+, y-, z+, z-; 0 where the stencil has no such entry); channel 0, 13..15
+// and boundary nodes 0
 void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipStream_t st);
 // c = net.out * (*nrm) / sqrt_count on interior nodes of an M^3 lattice, 0 on its boundary
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);

@@ -40,7 +40,7 @@ def test_forward_vs_oracle(small, M):
     rng = np.random.default_rng(M)
     p = LR.init_params(M, scale_last=1.0).astype(np.float32)
     small.learned_params(p)
-    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 16)).astype(np.float32)
     out = small.learned_forward(u)
     ref = LR.forward(p.astype(np.float64), u.astype(np.float64))
     assert nrel(out, ref) < 5e-3, nrel(out, ref)
@@ -52,7 +52,7 @@ def test_gradient_vs_oracle(small, M):
     rng = np.random.default_rng(100 + M)
     p = LR.init_params(7 + M, scale_last=1.0).astype(np.float32)
     small.learned_params(p)
-    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 16)).astype(np.float32)
     t = rng.standard_normal((M, M, M)).astype(np.float32)
     g, loss = small.learned_gradient(u, t)
     l_ref, g_ref = LR.backward(p.astype(np.float64), u.astype(np.float64), t.astype(np.float64))
@@ -71,7 +71,7 @@ def test_adam_step(small):
     p = LR.init_params(11).astype(np.float32)
     small.learned_init(5)
     small.learned_params(p)     # fresh moments come with init
-    u = rng.standard_normal((M, M, M, 8)).astype(np.float32)
+    u = rng.standard_normal((M, M, M, 16)).astype(np.float32)
     t = rng.standard_normal((M, M, M)).astype(np.float32)
     g, _ = small.learned_gradient(u, t)
     small.learned_adam(1e-3)

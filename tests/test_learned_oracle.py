@@ -15,7 +15,7 @@ import learned_ref as LR  # noqa: E402
 
 
 def test_nparams_layout():
-    assert LR.nparams() == (16 * 27 * 8 + 16) + 2 * (16 * 27 * 16 + 16) + (1 * 27 * 16 + 1) == 17761
+    assert LR.nparams() == 3 * (16 * 27 * 16 + 16) + (1 * 27 * 16 + 1) == 21217
     p = LR.init_params(3)
     assert np.array_equal(LR.pack(LR.unpack(p)), p)
 
@@ -55,7 +55,7 @@ def test_backward_vs_finite_differences():
     rng = np.random.default_rng(3)
     M = 5
     p = LR.init_params(4, scale_last=1.0)
-    u = rng.standard_normal((M, M, M, 8))
+    u = rng.standard_normal((M, M, M, 16))
     t = rng.standard_normal((M, M, M))
     l0, g = LR.backward(p, u, t, bf16_emulation=False)
     assert abs(l0 - LR.loss(LR.forward(p, u, False), t)) < 1e-14 * l0
@@ -106,7 +106,7 @@ def test_residual_head_vs_finite_differences():
     dinv = np.where(m, 1.0 / A.diagonal(), 0.0)
     p = LR.init_params(10, scale_last=1.0)
     u = rng.standard_normal((M, M, M))
-    feat = rng.standard_normal((M, M, M, 7))
+    feat = rng.standard_normal((M, M, M, 15))
     head = LR.residual_head(u, A, dinv)
     l0, g = LR.backward(p, u, bf16_emulation=False, head=head, feat=feat)
     offs = np.cumsum([0] + [LR.CH[l + 1] * 27 * LR.CH[l] + LR.CH[l + 1] for l in range(4)])
@@ -131,12 +131,13 @@ def test_residual_head_vs_finite_differences():
 
 
 def test_features_of_a_stencil():
-    """features() reads the six nearest couplings: check on a 7-point operator
-    with distinct known coefficients."""
+    """features() reads the couplings along the axes: check on a 13-point
+    operator (first and second neighbours) with distinct known coefficients."""
     import scipy.sparse as sp
-    M = 6
+    M = 7
     N = M ** 3
-    coef = {1: -1.5, -1: -0.5, M: -2.0, -M: -0.25, M * M: -3.0, -M * M: -0.75}
+    coef = {1: -1.5, -1: -0.5, M: -2.0, -M: -0.25, M * M: -3.0, -M * M: -0.75,
+            2: 0.1, -2: 0.2, 2 * M: 0.3, -2 * M: 0.4, 2 * M * M: 0.5, -2 * M * M: 0.6}
     rows, cols, vals = [], [], []
     for i in range(N):
         rows.append(i); cols.append(i); vals.append(8.0)
@@ -147,7 +148,10 @@ def test_features_of_a_stencil():
     A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
     f = LR.features(A, M)
     m = LR.interior_mask(M)
+    m2 = np.zeros((M, M, M), bool)  # nodes whose second neighbours lie in the lattice
+    m2[2:-2, 2:-2, 2:-2] = True
     for q, off in enumerate((1, -1, M, -M, M * M, -M * M)):
         assert np.allclose(f[..., q][m], coef[off] / 8.0)
-        assert np.all(f[..., q][~m] == 0)
-    assert np.all(f[..., 6] == 0)
+        assert np.allclose(f[..., 6 + q][m2], coef[2 * off] / 8.0)
+        assert np.all(f[..., q][~m] == 0) and np.all(f[..., 6 + q][~m] == 0)
+    assert np.all(f[..., 12:] == 0)
