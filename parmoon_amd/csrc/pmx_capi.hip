@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pmx.h"
@@ -71,7 +72,7 @@ struct Level {
   DevLayout D;
   DevBuf<int> boundary, segidx;
   DevBuf<DevSegment> segs;
-  DevBuf<BlockInfo> blocks_all, blocks_colour;
+  DevBuf<BlockInfo> blocks_all, blocks_colour, blocks_asm;
   DevBuf<int> colour_off;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
@@ -82,6 +83,8 @@ struct Level {
   bool have_val32 = false;
   DevBuf<double> ainv;                // coarsest level: A_II^-1 [I, -A_IB] (n_I x N)
   DevBuf<int> cidx, iidx;             // lattice node <-> interior row of ainv
+  DevBuf<double> aug;                 // Gauss-Jordan workspace
+  DevBuf<int> piv;
   int ainv_n = 0;
   // learned post-smoother: 1/diag, correction and its image under A (FP32)
   DevBuf<float> dinv, lc, lac;
@@ -97,6 +100,8 @@ struct System {
   bool two_matrices = false;
   DevBuf<double> prm, qref, tab, qpbuf;
   int nq = 0;
+  int tab_key = -1;                   // rule of the uploaded tables
+  int example = 0;                    // prm[PMX_PRM_EXAMPLE] of the last assembly
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
   DevBuf<double> oldrhs, oldsol, tmp1, tmp2;  // time stepping (finest level)
@@ -161,6 +166,20 @@ static void setup_level(System &Y, Level &lv, int nref) {
   }
   lv.segidx.upload(sidx.data(), sidx.size(), st);
   lv.blocks_all.upload(L.blocks_all.data(), L.blocks_all.size(), st);
+  {
+    // assembly order: blocks of all residue classes sorted by the position of
+    // their first row (z, then y), so that the rows sharing a cell are
+    // assembled close in time and its quadrature data is re-read from cache
+    std::vector<BlockInfo> ba(L.blocks_all);
+    auto key = [&](const BlockInfo &b) {
+      const Segment &sg = L.segs[b.seg];
+      const int64_t t = b.base / sg.cx;
+      const int64_t y = sg.y0 + (int64_t)L.S * (t % sg.cy), z = sg.z0 + (int64_t)L.S * (t / sg.cy);
+      return std::make_tuple(z, y, b.seg, b.base);
+    };
+    std::stable_sort(ba.begin(), ba.end(), [&](const BlockInfo &a, const BlockInfo &b) { return key(a) < key(b); });
+    lv.blocks_asm.upload(ba.data(), ba.size(), st);
+  }
   std::vector<BlockInfo> bc;
   D.colour_off.assign(L.ncolours + 1, 0);
   for (int c = 0; c < L.ncolours; c++) {
@@ -176,6 +195,7 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.boundary = lv.boundary.p; D.nboundary = (int)L.boundary.size();
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
+  D.blocks_asm = lv.blocks_asm.p;
   D.blocks_colour = lv.blocks_colour.p;
   D.colour_off_dev = lv.colour_off.p;
 
@@ -236,6 +256,21 @@ static void make_tables(int order, bool linear, std::vector<double> &qref, std::
 // explicit FP64 inverse of the coarsest operator (device Gauss-Jordan); used
 // as the exact coarse solve (SC_COARSE_SMOOTHER_SCALAR 17, MGLevel3D.C:865-905,
 // which refactors a dense copy on every call; here it is formed once per assembly)
+// reference tables and quadrature of the element for the rule in use
+// (uploaded again only when the rule changes)
+static void ensure_tables(System &Y, bool linear) {
+  const int key = 2 * Y.order + (linear ? 1 : 0);
+  if (Y.tab_key == key) return;
+  std::vector<double> qref, tab;
+  int nq;
+  make_tables(Y.order, linear, qref, tab, nq);
+  Y.qref.upload(qref.data(), qref.size(), Y.ctx->st);
+  Y.tab.upload(tab.data(), tab.size(), Y.ctx->st);
+  Y.nq = nq;
+  Y.qpbuf.alloc((size_t)Y.lev.back()->Nb.ncells * nq * 8);
+  Y.tab_key = key;
+}
+
 static void coarse_inverse(System &Y, Level &lv) {
   const int N = (int)lv.L.N;
   const int nI = (int)lv.L.nactive;
@@ -254,12 +289,10 @@ static void coarse_inverse(System &Y, Level &lv) {
     lv.cidx.upload(c.data(), c.size(), st);
     lv.iidx.upload(ii.data(), ii.size(), st);
   }
-  DevBuf<double> aug;
-  DevBuf<int> piv;
-  aug.alloc((size_t)nI * (nI + N));
-  piv.alloc(nI);
+  lv.aug.alloc((size_t)nI * (nI + N));
+  lv.piv.alloc(nI);
   lv.ainv.alloc((size_t)nI * N);
-  coarse_inverse_dense(lv.D, lv.v64.val.p, lv.cidx.p, nI, aug.p, piv.p, lv.ainv.p, st);
+  coarse_inverse_dense(lv.D, lv.v64.val.p, lv.cidx.p, nI, lv.aug.p, lv.piv.p, lv.ainv.p, st);
   check(hipStreamSynchronize(st), "coarse inverse");
   lv.ainv_n = nI;
 }
@@ -1074,19 +1107,14 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
     const int form = (int)prm[PMX_PRM_FORM];
     if (form < PMX_FORM_GALERKIN || form > PMX_FORM_TCD_SUPG_AK) throw std::invalid_argument("unknown form");
     Y.prm.upload(prm, PMX_PRM_LEN, st);
-    std::vector<double> qref, tab;
-    int nq;
-    make_tables(Y.order, prm[PMX_PRM_LINEAR] != 0, qref, tab, nq);
-    Y.qref.upload(qref.data(), qref.size(), st);
-    Y.tab.upload(tab.data(), tab.size(), st);
-    Y.nq = nq;
-    const int64_t nc = Y.lev.back()->Nb.ncells;
-    Y.qpbuf.alloc((size_t)nc * nq * 8);
+    Y.example = (int)prm[PMX_PRM_EXAMPLE];
+    ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
+    const int nq = Y.nq;
     Y.two_matrices = form == PMX_FORM_TCD_SUPG_AK;
     for (auto &lp : Y.lev) {
       Level &lv = *lp;
       if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
-      assemble(lv.D, Y.prm.p, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr,
+      assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr,
                lv.rhs.p, st);
       lv.have_val32 = false;
       lv.have_dinv = false;
@@ -1104,17 +1132,13 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
 static void assemble_into(System &Y, const double *prm, bool mass) {
   hipStream_t st = Y.ctx->st;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
-  std::vector<double> qref, tab;
-  int nq;
-  make_tables(Y.order, prm[PMX_PRM_LINEAR] != 0, qref, tab, nq);
-  Y.qref.upload(qref.data(), qref.size(), st);
-  Y.tab.upload(tab.data(), tab.size(), st);
-  Y.nq = nq;
-  Y.qpbuf.alloc((size_t)Y.lev.back()->Nb.ncells * nq * 8);
+  Y.example = (int)prm[PMX_PRM_EXAMPLE];
+  ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
+  const int nq = Y.nq;
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (mass) lv.mass.alloc(lv.L.nvals);
-    assemble(lv.D, Y.prm.p, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
+    assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
   }
 }
 
@@ -1164,7 +1188,6 @@ int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const dou
       l.have_dinv = false;
     }
     check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
-    check(hipStreamSynchronize(st), "tcd assemble");
     coarse_inverse(Y, *Y.lev.front());
     Y.assembled = true;
     std::memcpy(Y.opt, opt, sizeof(Y.opt));
@@ -1341,7 +1364,7 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 1: css_apply<double>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
           case 2: level_smooth<double>(Y, lv, 1, 0.8, 1); break;
           case 3: level_smooth<double>(Y, lv, 3, 1.0, 1); break;
-          case 4: assemble(lv.D, Y.prm.p, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
+          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
           case 5: if (li > 0) prolongate<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.f.p, v.d.p, st); break;
           case 7: mg_prec<double>(Y, v.aux.p, v.d.p); break;

@@ -171,16 +171,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T
 }
 
 // ============================================================ assembly
-// Stage 1: coefficients at every quadrature point of every cell, in the
-// canonical (axis-aligned) frame of the lattice: {w|detJ|, eps, b0, b1, b2, c, f, delta}.
-__global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ prm, const double *__restrict__ qref,
-                                                    int nq, int n, double *__restrict__ out) {
-  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  const int64_t ncell = (int64_t)n * n * n;
-  if (id >= ncell * nq) return;
-  const int q = id % nq;
-  const int64_t cell = id / nq;
-  const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
+// Coefficients at quadrature point q of cell (cx, cy, cz), in the canonical
+// (axis-aligned) frame of the lattice: {w|detJ|, eps, b0, b1, b2, c, f, delta}.
+__device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const double *__restrict__ qref, int n,
+                                          int cx, int cy, int cz, int q, int form, double *o) {
   const double h = 1.0 / n;
   const double xi = qref[4 * q], eta = qref[4 * q + 1], zeta = qref[4 * q + 2], w = qref[4 * q + 3];
   // THexaAffin::GetOrigFromRef: X = xc0 + xc1 xi + ... with xc0 the cell centre
@@ -188,7 +182,6 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
   const QpCoeffs c = example_coeffs(prm, X, Y, Z);
-  const int form = (int)prm[PMX_PRM_FORM];
   const double hK = sqrt(3.0) * h;  // THexahedron::GetDiameter of a cube
   double delta = 0;
   if (form == PMX_FORM_SDFEM) {
@@ -197,9 +190,25 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
     const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
     delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
   }
-  double *o = out + id * 8;
   o[0] = w * (h * h * h / 8.0);
   o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = delta;
+}
+// Stage 1 (coefficient functions that are expensive to evaluate): the
+// coefficients of every quadrature point of every cell, once.
+__global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ prm, const double *__restrict__ qref,
+                                                    int nq, int n, double *__restrict__ out) {
+  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  const int64_t ncell = (int64_t)n * n * n;
+  if (id >= ncell * nq) return;
+  const int q = id % nq;
+  const int64_t cell = id / nq;
+  const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
+  qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], out + id * 8);
+}
+// examples whose coefficients are cheaper to evaluate again than to read back
+__host__ __device__ inline bool coeffs_inline(int example) {
+  return example == PMX_EX_LAPLACE || example == PMX_EX_SINE_CD || example == PMX_EX_HEAT_AMC ||
+         example == PMX_EX_ROTATING;
 }
 
 // Stage 2: row gather.  The thread of active node p visits the cells that
@@ -214,8 +223,9 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
 // coordinate lies in one cell along it, every other node in two), so the
 // loops are uniform across the wavefront and the basis tables are read with
 // scalar loads; TWO selects the second matrix of the SUPG TCD form.
-template <int ORDER, int EX, int EY, int EZ, bool TWO>
+template <int ORDER, int EX, int EY, int EZ, bool TWO, bool INL>
 __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M, int n, int nq, int form,
+                                             const double *__restrict__ prm, const double *__restrict__ qref,
                                              const double *__restrict__ qp, const double *__restrict__ tab,
                                              double *__restrict__ val, double *__restrict__ val2,
                                              double *__restrict__ rhs) {
@@ -248,8 +258,15 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[b] = acc2[b] = 0;
         for (int q = 0; q < nq; q++) {
-          const double w = Q[8 * q], eps = Q[8 * q + 1], b0 = Q[8 * q + 2], b1 = Q[8 * q + 3], b2 = Q[8 * q + 4];
-          const double cc = Q[8 * q + 5], ff = Q[8 * q + 6], dl = Q[8 * q + 7];
+          double Qq[8];
+          if (INL) {
+            qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) Qq[e] = Q[8 * q + e];
+          }
+          const double w = Qq[0], eps = Qq[1], b0 = Qq[2], b1 = Qq[3], b2 = Qq[4];
+          const double cc = Qq[5], ff = Qq[6], dl = Qq[7];
           const double *Ta = tab + (q * NB + a) * 4;
           const double fa = Ta[0], gax = g * Ta[1], gay = g * Ta[2], gaz = g * Ta[3];
           const double bga = b0 * gax + b1 * gay + b2 * gaz;
@@ -288,32 +305,36 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   rhs[p] = frhs;
 }
 
-template <int ORDER, bool TWO>
+template <int ORDER, bool TWO, bool INL>
 __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
                                                      const DevSegment *__restrict__ segs, int M, int n, int nq,
-                                                     const double *__restrict__ prm, const double *__restrict__ qp,
-                                                     const double *__restrict__ tab, double *__restrict__ val,
-                                                     double *__restrict__ val2, double *__restrict__ rhs) {
+                                                     const double *__restrict__ prm, const double *__restrict__ qref,
+                                                     const double *__restrict__ qp, const double *__restrict__ tab,
+                                                     double *__restrict__ val, double *__restrict__ val2,
+                                                     double *__restrict__ rhs) {
   const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
   const DevSegment s = segs[bi.seg];
   const int idx = bi.base + threadIdx.x;
   if (idx >= s.count) return;
   const int form = (int)prm[PMX_PRM_FORM];
+#define PMX_AROW(O, EX, EY, EZ) \
+  assemble_row<O, EX, EY, EZ, TWO, INL>(s, idx, M, n, nq, form, prm, qref, qp, tab, val, val2, rhs)
   if (ORDER == 1) {
-    assemble_row<1, 1, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs);
+    PMX_AROW(1, 1, 1, 1);
     return;
   }
   const int code = s.ex * 100 + s.ey * 10 + s.ez;
   switch (code) {
-    case 111: assemble_row<2, 1, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 211: assemble_row<2, 2, 1, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 121: assemble_row<2, 1, 2, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 221: assemble_row<2, 2, 2, 1, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 112: assemble_row<2, 1, 1, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 212: assemble_row<2, 2, 1, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    case 122: assemble_row<2, 1, 2, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
-    default: assemble_row<2, 2, 2, 2, TWO>(s, idx, M, n, nq, form, qp, tab, val, val2, rhs); break;
+    case 111: PMX_AROW(2, 1, 1, 1); break;
+    case 211: PMX_AROW(2, 2, 1, 1); break;
+    case 121: PMX_AROW(2, 1, 2, 1); break;
+    case 221: PMX_AROW(2, 2, 2, 1); break;
+    case 112: PMX_AROW(2, 1, 1, 2); break;
+    case 212: PMX_AROW(2, 2, 1, 2); break;
+    case 122: PMX_AROW(2, 1, 2, 2); break;
+    default: PMX_AROW(2, 2, 2, 2); break;
   }
+#undef PMX_AROW
 }
 // Dirichlet rhs: RHS[dof] = g(node) (AssembleMat3D.C:1031-1096)
 __global__ void k_boundary_rhs(const int *__restrict__ bnd, int nbd, int M, const double *__restrict__ prm,
@@ -641,23 +662,31 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
                      kcol, dof2lat, D.N, active, D.M, D.S, D.segidx, D.segs, val, out);
 }
 
-void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int nq, const double *tab,
+void assemble(const DevLayout &D, const double *prm_dev, int example, const double *qref, int nq, const double *tab,
               double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st) {
   const int64_t ncq = (int64_t)D.n * D.n * D.n * nq;
-  hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
-                     nq, D.n, qpbuf);
+  const bool inl = coeffs_inline(example);
+  if (!inl)
+    hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
+                       nq, D.n, qpbuf);
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
-#define PMX_ASM(O, TW)                                                                                      \
-  hipLaunchKernelGGL((k_assemble<O, TW>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, \
-                     D.M, D.n, nq, prm_dev, qpbuf, tab, val, val2, rhs)
+#define PMX_ASM(O, TW, IN)                                                                                   \
+  hipLaunchKernelGGL((k_assemble<O, TW, IN>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_asm,      \
+                     D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, val, val2, rhs)
+#define PMX_ASM2(O, TW) \
+  do {                   \
+    if (inl) PMX_ASM(O, TW, true); \
+    else PMX_ASM(O, TW, false);    \
+  } while (0)
   if (D.order == 1) {
-    if (val2) PMX_ASM(1, true);
-    else PMX_ASM(1, false);
+    if (val2) PMX_ASM2(1, true);
+    else PMX_ASM2(1, false);
   } else {
-    if (val2) PMX_ASM(2, true);
-    else PMX_ASM(2, false);
+    if (val2) PMX_ASM2(2, true);
+    else PMX_ASM2(2, false);
   }
+#undef PMX_ASM2
 #undef PMX_ASM
   hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
                      D.nboundary, D.M, prm_dev, rhs);

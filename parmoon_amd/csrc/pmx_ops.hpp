@@ -20,6 +20,7 @@ struct DevLayout {
   int *segidx = nullptr;  // residue class (rx + S(ry + S rz)) -> segment, -1 if empty
   BlockInfo *blocks_all = nullptr;
   int nblocks_all = 0;
+  BlockInfo *blocks_asm = nullptr;  // the same blocks in spatial order (assembly)
   BlockInfo *blocks_colour = nullptr;
   std::vector<int> colour_off;  // host: block offsets per colour
   int *colour_off_dev = nullptr;  // the same on the device
@@ -61,7 +62,9 @@ void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *
 template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
                 const T *val, double *out, hipStream_t st);
-void assemble(const DevLayout &D, const double *prm_dev, const double *qref, int nq, const double *tab,
+// example: prm[PMX_PRM_EXAMPLE] (selects evaluating the coefficients inline or
+// through the quadrature-point buffer)
+void assemble(const DevLayout &D, const double *prm_dev, int example, const double *qref, int nq, const double *tab,
               double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st);
 // time stepping (TSystemTCD3D): nodal interpolation of the example's initial
 // function, val = mass + c val, and the Crank-Nicolson / theta right-hand side
