@@ -31,9 +31,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-# HBM bytes per FP32 SpMV launch at 64^3 Q2 from rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# scaled by the same counters of a copy of known size (None until measured)
-TRAFFIC_BYTES = None
+# HBM bytes per FP32 SpMV launch at 64^3 Q2: rocprofv3 FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000 (the factors that make the counters of a 1 GiB device
+# copy read 1 GiB each) = 535.1 MB + 9.4 MB per dispatch
+TRAFFIC_BYTES = 544_480_000
 
 
 def cpu_baseline(nref_cpu=4, seed=2024):

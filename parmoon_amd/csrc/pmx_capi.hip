@@ -72,7 +72,7 @@ struct Level {
   DevLayout D;
   DevBuf<int> boundary, segidx;
   DevBuf<DevSegment> segs;
-  DevBuf<BlockInfo> blocks_all, blocks_colour, blocks_asm;
+  DevBuf<BlockInfo> blocks_all, blocks_colour;
   DevBuf<int> colour_off;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
@@ -165,26 +165,34 @@ static void setup_level(System &Y, Level &lv, int nref) {
     sidx[s.rx + L.S * (s.ry + L.S * s.rz)] = (int)i;
   }
   lv.segidx.upload(sidx.data(), sidx.size(), st);
-  lv.blocks_all.upload(L.blocks_all.data(), L.blocks_all.size(), st);
+  // Launch order of the blocks: sorted by the lattice position of their first
+  // row (z, then y) across residue classes, in every list.  A launch's blocks
+  // are split into 8 contiguous chunks, one per XCD (xcd_remap), so each XCD
+  // works on one z-slab: the x values its rows gather (the slab plus two
+  // planes either side) stay in that XCD's L2 instead of every XCD streaming
+  // the whole vector once per class, and the rows sharing a cell are
+  // assembled close in time.  Rows of one list are independent, so the order
+  // changes no result.
+  auto key = [&](const BlockInfo &b) {
+    const Segment &sg = L.segs[b.seg];
+    const int64_t t = b.base / sg.cx;
+    const int64_t y = sg.y0 + (int64_t)L.S * (t % sg.cy), z = sg.z0 + (int64_t)L.S * (t / sg.cy);
+    return std::make_tuple(z, y, b.seg, b.base);
+  };
+  auto spatial = [&](std::vector<BlockInfo> v) {
+    std::stable_sort(v.begin(), v.end(), [&](const BlockInfo &a, const BlockInfo &b) { return key(a) < key(b); });
+    return v;
+  };
   {
-    // assembly order: blocks of all residue classes sorted by the position of
-    // their first row (z, then y), so that the rows sharing a cell are
-    // assembled close in time and its quadrature data is re-read from cache
-    std::vector<BlockInfo> ba(L.blocks_all);
-    auto key = [&](const BlockInfo &b) {
-      const Segment &sg = L.segs[b.seg];
-      const int64_t t = b.base / sg.cx;
-      const int64_t y = sg.y0 + (int64_t)L.S * (t % sg.cy), z = sg.z0 + (int64_t)L.S * (t / sg.cy);
-      return std::make_tuple(z, y, b.seg, b.base);
-    };
-    std::stable_sort(ba.begin(), ba.end(), [&](const BlockInfo &a, const BlockInfo &b) { return key(a) < key(b); });
-    lv.blocks_asm.upload(ba.data(), ba.size(), st);
+    const std::vector<BlockInfo> ba = spatial(L.blocks_all);
+    lv.blocks_all.upload(ba.data(), ba.size(), st);
   }
   std::vector<BlockInfo> bc;
   D.colour_off.assign(L.ncolours + 1, 0);
   for (int c = 0; c < L.ncolours; c++) {
     D.colour_off[c] = (int)bc.size();
-    bc.insert(bc.end(), L.blocks_colour[c].begin(), L.blocks_colour[c].end());
+    const std::vector<BlockInfo> bs = spatial(L.blocks_colour[c]);
+    bc.insert(bc.end(), bs.begin(), bs.end());
   }
   D.colour_off[L.ncolours] = (int)bc.size();
   lv.blocks_colour.upload(bc.data(), bc.size(), st);
@@ -195,7 +203,6 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.boundary = lv.boundary.p; D.nboundary = (int)L.boundary.size();
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
-  D.blocks_asm = lv.blocks_asm.p;
   D.blocks_colour = lv.blocks_colour.p;
   D.colour_off_dev = lv.colour_off.p;
 

@@ -30,13 +30,44 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
 // 3: x = w ((f - A_off x)/d - x) + x  (TMGLevel3D::SOR, MGLevel3D.C:280-361) in place, one colour
 template <typename T> struct Acc { using type = T; };
 
+// W consecutive lattice values starting at q (the x-run of one (dy, dz) group
+// of a stencil row) in as few loads as the element alignment allows:
+// one 16-byte load plus one element for W = 5 floats, 8 + 4 bytes for W = 3.
+template <typename T, int W>
+__device__ __forceinline__ void load_run(const T *q, T (&r)[W]) {
+  if constexpr (sizeof(T) == 4 && W == 5) {
+    float4 a;
+    __builtin_memcpy(&a, q, 16);
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w; r[4] = q[4];
+  } else if constexpr (sizeof(T) == 4 && W == 3) {
+    float2 a;
+    __builtin_memcpy(&a, q, 8);
+    r[0] = a.x; r[1] = a.y; r[2] = q[2];
+  } else if constexpr (sizeof(T) == 8 && W == 5) {
+    double2 a, b;
+    __builtin_memcpy(&a, q, 16);
+    __builtin_memcpy(&b, q + 2, 16);
+    r[0] = a.x; r[1] = a.y; r[2] = b.x; r[3] = b.y; r[4] = q[4];
+  } else if constexpr (sizeof(T) == 8 && W == 3) {
+    double2 a;
+    __builtin_memcpy(&a, q, 16);
+    r[0] = a.x; r[1] = a.y; r[2] = q[2];
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; k++) r[k] = q[k];
+  }
+}
+
+// The slots of a row are grouped by (dy, dz); the WX slots of a group read
+// WX consecutive x values, fetched as one run.  Lane `part` of the R lanes
+// takes groups part, part+R, ...
 template <typename T, int EX, int EY, int EZ, int R, int MODE>
 __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part, int M,
                                           const T *__restrict__ val, const T *x, const T *__restrict__ f, T *y,
                                           double omega) {
   using A = typename Acc<T>::type;
-  constexpr int WX = 2 * EX + 1, WY = 2 * EY + 1, WZ = 2 * EZ + 1, NS = WX * WY * WZ;
-  constexpr int DS = EX + WX * (EY + WY * EZ);
+  constexpr int WX = 2 * EX + 1, WY = 2 * EY + 1, WZ = 2 * EZ + 1, NG = WY * WZ;
+  constexpr int DS = EX + WX * (EY + WY * EZ), DG = DS / WX;
   const int i = idx % s.cx;
   const int t = idx / s.cx;
   const int j = t % s.cy, k = t / s.cy;
@@ -44,11 +75,17 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   const T *v = val + vbase(s, idx);
   A acc = 0;
 #pragma unroll
-  for (int m = 0; m < (NS + R - 1) / R; m++) {
-    const int sl = part + R * m;
-    if (sl < NS && (MODE < 2 || sl != DS)) {
-      const int dx = sl % WX - EX, dy = (sl / WX) % WY - EY, dz = sl / (WX * WY) - EZ;
-      acc += (A)v[sl * kSlice] * (A)x[p + dx + M * (dy + M * dz)];
+  for (int m = 0; m < (NG + R - 1) / R; m++) {
+    const int g = part + R * m;
+    if (g < NG) {
+      const int dy = g % WY - EY, dz = g / WY - EZ;
+      T xr[WX];
+      load_run<T, WX>(x + (p - EX + M * (dy + M * dz)), xr);
+#pragma unroll
+      for (int e = 0; e < WX; e++) {
+        if (MODE >= 2 && g == DG && e == EX) continue;
+        acc += (A)v[(g * WX + e) * kSlice] * (A)xr[e];
+      }
     }
   }
 #pragma unroll
@@ -672,7 +709,7 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, const doub
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
 #define PMX_ASM(O, TW, IN)                                                                                   \
-  hipLaunchKernelGGL((k_assemble<O, TW, IN>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_asm,      \
+  hipLaunchKernelGGL((k_assemble<O, TW, IN>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,      \
                      D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, val, val2, rhs)
 #define PMX_ASM2(O, TW) \
   do {                   \

@@ -18,9 +18,8 @@ struct DevLayout {
   int *boundary = nullptr;
   DevSegment *segs = nullptr;
   int *segidx = nullptr;  // residue class (rx + S(ry + S rz)) -> segment, -1 if empty
-  BlockInfo *blocks_all = nullptr;
+  BlockInfo *blocks_all = nullptr;  // all rows, spatial order (z, y of the first row)
   int nblocks_all = 0;
-  BlockInfo *blocks_asm = nullptr;  // the same blocks in spatial order (assembly)
   BlockInfo *blocks_colour = nullptr;
   std::vector<int> colour_off;  // host: block offsets per colour
   int *colour_off_dev = nullptr;  // the same on the device
