@@ -120,6 +120,9 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [19] precision 2: number of finest levels with the learned post-smoother
  *       (0 = the levels it was trained for)
  *  [20] precision 2: learned steps per post-smoothing (default 1)
+ *  [21] SOR/SSOR sweeps of levels whose largest colour has at most this many
+ *       rows run in one workgroup (same iterate, fewer launches); 0 = 512,
+ *       < 0 = never
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

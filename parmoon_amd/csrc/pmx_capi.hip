@@ -73,7 +73,7 @@ struct Level {
   DevBuf<int> boundary, segidx;
   DevBuf<DevSegment> segs;
   DevBuf<BlockInfo> blocks_all, blocks_colour;
-  DevBuf<int> colour_off;
+  DevBuf<int> colour_off, colour_seg;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
   DevBuf<double> mass;                // time stepping: mass matrix M
@@ -110,12 +110,12 @@ struct System {
   DevBuf<float> krylov32;
   DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
   std::vector<int> rec;
-  double opt[24];
+  double opt[24] = {0};
   // captured MG cycles (finest f -> finest x), keyed by precision and cycle options
   struct Graph {
     bool valid = false;
     int prec = 0;
-    double key[11] = {0};
+    double key[12] = {0};
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[2];
@@ -197,6 +197,22 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.colour_off[L.ncolours] = (int)bc.size();
   lv.blocks_colour.upload(bc.data(), bc.size(), st);
   lv.colour_off.upload(D.colour_off.data(), D.colour_off.size(), st);
+  {
+    // segments are stored colour-major: colour c owns segments cs[c] .. cs[c+1]-1
+    std::vector<int> cs(L.ncolours + 1, 0);
+    std::vector<int64_t> rows(L.ncolours, 0);
+    for (size_t i = 0; i < L.segs.size(); i++) {
+      const int c = L.segs[i].colour;
+      if (i > 0 && c < L.segs[i - 1].colour) throw std::logic_error("segments not colour-major");
+      cs[c + 1]++;
+      rows[c] += L.segs[i].count;
+    }
+    for (int c = 0; c < L.ncolours; c++) cs[c + 1] += cs[c];
+    lv.colour_seg.upload(cs.data(), cs.size(), st);
+    D.colour_seg_dev = lv.colour_seg.p;
+    D.max_colour_rows = *std::max_element(rows.begin(), rows.end());
+    D.nsegs = (int)L.segs.size();
+  }
   lv.boundary.upload(L.boundary.data(), L.boundary.size(), st);
   lv.lat2dof.upload(lv.Nb.lat2dof.data(), lv.Nb.lat2dof.size(), st);
   lv.dof2lat.upload(lv.Nb.dof2lat.data(), lv.Nb.dof2lat.size(), st);
@@ -352,11 +368,15 @@ template <typename T>
 static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
   hipStream_t st = Y.ctx->st;
   Vecs<T> &v = vecs<T>(lv);
+  // levels whose largest colour has at most this many rows sweep in one workgroup
+  const int64_t wg_limit = Y.opt[21] == 0 ? kSweepWgRows : (int64_t)Y.opt[21];
   for (int s = 0; s < sweeps; s++) {
     if (kind == 1) {
       copy<T>(v.aux.p, v.x.p, lv.L.N, st);
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
       css_apply<T>(lv.D, 2, v.val.p, v.aux.p, v.f.p, v.x.p, omega, -1, st);
+    } else if (lv.D.max_colour_rows <= wg_limit && sweep_wg_fits(lv.D, sizeof(T))) {
+      sweep_wg<T>(lv.D, v.val.p, v.x.p, v.f.p, omega, kind == 3 ? 1 : 0, st);
     } else {
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
       for (int c = 0; c < lv.L.ncolours; c++) css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
@@ -462,11 +482,12 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(v.f.p, vin, lv.L.N, st);
   const int pi = std::is_same<T, double>::value ? 0 : 1;
   System::Graph &g = Y.graphs[pi];
-  double key[11];
+  double key[12];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
   const bool learned = pi == 1 && Y.learn_active;
   key[9] = learned ? Y.learned_levels : 0;
   key[10] = learned ? std::max(1.0, Y.opt[20]) : 0;
+  key[11] = Y.opt[21];
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);

@@ -29,6 +29,13 @@ namespace pmx {
 constexpr int kBlock = 256;       // threads per workgroup of the operator kernels
 constexpr int kRowLanes = 4;      // lanes per matrix row in the row kernels
 constexpr int kRowsPerBlock = kBlock / kRowLanes;
+// workgroup of the single-workgroup sweep: FP64 rows need twice the
+// registers for their loads in flight, so half the lanes
+template <typename T> constexpr int sweep_threads() { return sizeof(T) == 8 ? 512 : 1024; }
+constexpr int kMaxSegs = 216;        // residue classes of a level: at most 6^3
+constexpr int kMaxColours = 27;
+constexpr int64_t kSweepLdsBytes = 48 * 1024;  // largest x staged in LDS by the single-workgroup sweep
+constexpr int kSweepWgRows = 512;    // default: single-workgroup sweep up to this many rows per colour
 constexpr int kSlice = 16;        // rows per value slice (see Layout)
 constexpr int kRedBlocks = 1024;  // partial sums of the deterministic reductions
 

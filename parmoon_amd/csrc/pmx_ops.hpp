@@ -23,6 +23,9 @@ struct DevLayout {
   BlockInfo *blocks_colour = nullptr;
   std::vector<int> colour_off;  // host: block offsets per colour
   int *colour_off_dev = nullptr;  // the same on the device
+  int *colour_seg_dev = nullptr;  // first segment of each colour (segments are colour-major), ncolours + 1
+  int64_t max_colour_rows = 0;    // rows of the largest colour
+  int nsegs = 0;                  // residue classes with rows
   int ncolours = 0;
 };
 
@@ -42,6 +45,11 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st);
+// whole SOR (ssor 0) / SSOR (ssor 1) sweep with Dirichlet rows x = f, one
+// workgroup with the level's x in LDS (levels with M^3 * s <= kSweepLdsBytes)
+inline bool sweep_wg_fits(const DevLayout &D, size_t s) { return (int64_t)D.N * (int64_t)s <= kSweepLdsBytes; }
+template <typename T>
+void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st);
 // Dirichlet rows: mode 0 y=x, 1 y=f-x, 2 y=f, 3 y=0
 template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st);
 template <typename T>
