@@ -121,8 +121,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *       (0 = the levels it was trained for)
  *  [20] precision 2: learned steps per post-smoothing (default 1)
  *  [21] SOR/SSOR sweeps of levels whose largest colour has at most this many
- *       rows run in one workgroup (same iterate, fewer launches); 0 = 512,
- *       < 0 = never
+ *       rows run in one workgroup (same iterate, fewer launches); 0 = 512 in
+ *       FP32 and never in FP64, < 0 = never
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */

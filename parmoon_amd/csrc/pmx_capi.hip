@@ -368,8 +368,11 @@ template <typename T>
 static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweeps) {
   hipStream_t st = Y.ctx->st;
   Vecs<T> &v = vecs<T>(lv);
-  // levels whose largest colour has at most this many rows sweep in one workgroup
-  const int64_t wg_limit = Y.opt[21] == 0 ? kSweepWgRows : (int64_t)Y.opt[21];
+  // levels whose largest colour has at most this many rows sweep in one
+  // workgroup; by default FP32 only: an FP64 sweep moves twice the bytes
+  // through the one CU and is slower there than launch per colour
+  const int64_t wg_default = sizeof(T) == 4 ? kSweepWgRows : 0;
+  const int64_t wg_limit = Y.opt[21] == 0 ? wg_default : (int64_t)Y.opt[21];
   for (int s = 0; s < sweeps; s++) {
     if (kind == 1) {
       copy<T>(v.aux.p, v.x.p, lv.L.N, st);

@@ -27,8 +27,7 @@
 namespace pmx {
 
 constexpr int kBlock = 256;       // threads per workgroup of the operator kernels
-constexpr int kRowLanes = 4;      // lanes per matrix row in the row kernels
-constexpr int kRowsPerBlock = kBlock / kRowLanes;
+constexpr int kRowsPerBlock = 64; // rows per entry of the block tables (and per row-kernel workgroup)
 // workgroup of the single-workgroup sweep: FP64 rows need twice the
 // registers for their loads in flight, so half the lanes
 template <typename T> constexpr int sweep_threads() { return sizeof(T) == 8 ? 512 : 1024; }

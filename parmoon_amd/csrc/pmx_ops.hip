@@ -22,8 +22,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
 }
 
 // ============================================================ stencil rows
-// One matrix row is handled by R = kRowLanes lanes: lane `part` accumulates
-// slots part, part+R, ... and a shuffle reduction combines them.  Modes:
+// One matrix row is handled by R lanes (R = row_lanes of the level): lane
+// `part` accumulates groups of slots part, part+R, ... and a shuffle
+// reduction combines them.  Modes:
 // 0: y = A x                          (MatVect, Blas2.C:86-118)
 // 1: y = f - A x                      (ScalarDefect, Blas2.C:37-83)
 // 2: y = (1-w) x + w (f - A_off x)/d  (TMGLevel3D::Jacobi, MGLevel3D.C:469-563), x = old iterate
@@ -117,12 +118,13 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   else y[p] = (T)((A)omega * ((fp - acc) / diag - told) + told);
 }
 
-template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_rows(const BlockInfo *__restrict__ blocks,
+// R lanes per row, kRowsPerBlock rows per workgroup; with one lane per row
+// the compiler is held to 128 registers so that 4 waves share a SIMD.
+template <typename T, int MODE, int R>
+__global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(const BlockInfo *__restrict__ blocks,
                                                  const DevSegment *__restrict__ segs, int M,
                                                  const T *__restrict__ val, const T *x, const T *__restrict__ f,
                                                  T *y, double omega) {
-  constexpr int R = kRowLanes;
   const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
   const DevSegment s = segs[bi.seg];
   const int idx = bi.base + threadIdx.x / R;
@@ -151,13 +153,13 @@ __global__ __launch_bounds__(kBlock) void k_rows(const BlockInfo *__restrict__ b
 // launch-per-colour sweep.  On a level whose colours have a few hundred rows
 // this replaces 2 x 27 launch latencies per SSOR sweep by barriers, and the
 // x round trips through memory by LDS accesses.
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(sweep_threads<T>()) void k_sweep_wg(const DevSegment *__restrict__ segs, int nseg,
                                                             const int *__restrict__ colour_seg, int ncol, int M,
                                                             const T *__restrict__ val, T *x,
                                                             const T *__restrict__ f, double omega, int ssor,
                                                             const int *__restrict__ bnd, int nb) {
-  constexpr int R = kRowLanes, NT = sweep_threads<T>();
+  constexpr int NT = sweep_threads<T>();
   extern __shared__ __align__(16) unsigned char sweep_lds[];
   T *lx = reinterpret_cast<T *>(sweep_lds);
   // segment table, colour ranges and each segment's first row inside its
@@ -723,20 +725,38 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
   const BlockInfo *blk = colour < 0 ? D.blocks_all : D.blocks_colour + D.colour_off[colour];
   const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
-switch (mode) {
-    case 0: hipLaunchKernelGGL((k_rows<T, 0>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    case 1: hipLaunchKernelGGL((k_rows<T, 1>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    case 2: hipLaunchKernelGGL((k_rows<T, 2>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
-    default: hipLaunchKernelGGL((k_rows<T, 3>), dim3(nblk), dim3(kBlock), 0, st, blk, D.segs, D.M, val, x, f, y, omega); break;
+#define PMX_ROWS(MODE, R)                                                                                      \
+  hipLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val, x, f, \
+                     y, omega)
+#define PMX_ROWS_R(MODE)                                                                                       \
+  switch (row_lanes(D)) {                                                                                      \
+    case 1: PMX_ROWS(MODE, 1); break;                                                                          \
+    case 2: PMX_ROWS(MODE, 2); break;                                                                          \
+    default: PMX_ROWS(MODE, 4); break;                                                                         \
   }
+  switch (mode) {
+    case 0: PMX_ROWS_R(0); break;
+    case 1: PMX_ROWS_R(1); break;
+    case 2: PMX_ROWS_R(2); break;
+    default: PMX_ROWS_R(3); break;
+  }
+#undef PMX_ROWS_R
+#undef PMX_ROWS
 }
 template <typename T>
 void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st) {
   if (D.nsegs > kMaxSegs || D.ncolours > kMaxColours) throw std::logic_error("sweep_wg: segment table too large");
   if (!sweep_wg_fits(D, sizeof(T))) throw std::logic_error("sweep_wg: level does not fit in LDS");
   const size_t lds = (size_t)D.N * sizeof(T);
-  hipLaunchKernelGGL(k_sweep_wg<T>, dim3(1), dim3(sweep_threads<T>()), lds, st, D.segs, D.nsegs, D.colour_seg_dev,
-                     D.ncolours, D.M, val, x, f, omega, ssor, D.boundary, D.nboundary);
+#define PMX_SWEEP(R)                                                                                            \
+  hipLaunchKernelGGL((k_sweep_wg<T, R>), dim3(1), dim3(sweep_threads<T>()), lds, st, D.segs, D.nsegs,             \
+                     D.colour_seg_dev, D.ncolours, D.M, val, x, f, omega, ssor, D.boundary, D.nboundary)
+  switch (row_lanes(D)) {
+    case 1: PMX_SWEEP(1); break;
+    case 2: PMX_SWEEP(2); break;
+    default: PMX_SWEEP(4); break;
+  }
+#undef PMX_SWEEP
 }
 template <typename T>
 void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st) {

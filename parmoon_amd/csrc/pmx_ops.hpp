@@ -48,6 +48,16 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
 // whole SOR (ssor 0) / SSOR (ssor 1) sweep with Dirichlet rows x = f, one
 // workgroup with the level's x in LDS (levels with M^3 * s <= kSweepLdsBytes)
 inline bool sweep_wg_fits(const DevLayout &D, size_t s) { return (int64_t)D.N * (int64_t)s <= kSweepLdsBytes; }
+// Lanes per row of the row kernels on a level (the same for every kernel and
+// precision of the level, so the single-workgroup and the launch-per-colour
+// sweeps compute each row alike): 4 where a colour has at most kSweepWgRows
+// rows (few rows: more lanes per row shorten the load chain of each), 1 from
+// 2^20 active rows on (one lane per row touches the fewest cache lines per
+// gathered x run), 2 between.
+inline int row_lanes(const DevLayout &D) {
+  if (D.max_colour_rows <= kSweepWgRows) return 4;
+  return D.nactive >= (int64_t(1) << 20) ? 1 : 2;
+}
 template <typename T>
 void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st);
 // Dirichlet rows: mode 0 y=x, 1 y=f-x, 2 y=f, 3 y=0
