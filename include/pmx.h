@@ -123,6 +123,8 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [21] SOR/SSOR sweeps of levels whose largest colour has at most this many
  *       rows run in one workgroup (same iterate, fewer launches); 0 = 512 in
  *       FP32 and never in FP64, < 0 = never
+ *  [22] precision 2: the finest [22] of the [19] levels keep the plain smoother
+ *  [23] precision 2: 1 = the learned steps also replace the pre-smoothing
  * info (out, 8 doubles): [0] iterations (Krylov steps summed) [1] final FP64
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */
@@ -196,8 +198,9 @@ int pmx_learned_adam(pmx_system *sys, double lr);
  * but the coarsest), [5] loss: 0 squared error against the cycle's error,
  * 1 the scaled defect left after the correction (as pmx_learned_gradient_level),
  * [6] samples: 0 a random start of the homogeneous problem, 1 a zero start
- * with a random right-hand side, 2 half each.
- * tinfo (16 doubles): [0] first / [1] last mean loss of the finest network,
+ * with a random right-hand side, 2 half each, [7] the finest levels among [4]
+ * left untrained (for solves with opt[22] of that value).
+ * tinfo (16 doubles): [0] first / [1] last mean loss of the finest trained network,
  * [2] s, [3] total steps, [4 + 2j] / [5 + 2j] first / last mean loss of the
  * network of level L - j. */
 int pmx_learned_train(pmx_system *sys, const double *opt, const double *topt, double *tinfo);

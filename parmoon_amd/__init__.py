@@ -320,10 +320,10 @@ class System:
         _ok(lib().pmx_learned_gradient_level(self.h, level, _p(u), _p(g), _p(loss)), "pmx_learned_gradient_level")
         return g, float(loss[0])
 
-    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=0, loss=1, data=0):
+    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=0, loss=1, data=0, skip=0):
         o = np.zeros(24)
         o[:len(opt)] = opt
-        t = np.array([samples, steps, lr, seed, levels, loss, data, 0], float)
+        t = np.array([samples, steps, lr, seed, levels, loss, data, skip], float)
         info = np.zeros(16)
         _ok(lib().pmx_learned_train(self.h, _p(o), _p(t), _p(info)), "pmx_learned_train")
         return info

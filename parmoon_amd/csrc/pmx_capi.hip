@@ -115,7 +115,7 @@ struct System {
   struct Graph {
     bool valid = false;
     int prec = 0;
-    double key[12] = {0};
+    double key[14] = {0};
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[2];
@@ -125,6 +125,8 @@ struct System {
   static constexpr int kMaxLevels = 10;
   LearnedNet nets[kMaxLevels];
   int learned_levels = 1;
+  int learned_skip = 0;     // finest levels inside learned_levels that keep the plain smoother
+  bool learned_pre = false; // learned steps also replace the pre-smoothing
   int trained_levels = 1;  // levels of the last training
   bool learn_active = false;
   int fallbacks = 0;
@@ -416,6 +418,11 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   learned_step(v.x.p, lv.lc.p, sc + 1, N, st);
 }
 
+// level i smooths with the network in this cycle
+static bool learned_level(const System &Y, int i) {
+  const int n = (int)Y.lev.size();
+  return Y.learn_active && i >= n - Y.learned_levels && i < n - Y.learned_skip;
+}
 template <typename T>
 static void mg_cycle(System &Y, int i, bool post = true) {
   hipStream_t st = Y.ctx->st;
@@ -435,7 +442,14 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   Level &cl = *Y.lev[i - 1];
   Vecs<T> &cv = vecs<T>(cl);
   const int kind = (int)o[5];
-  level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
+  bool pre_done = false;
+  if constexpr (std::is_same<T, float>::value) {
+    if (Y.learned_pre && learned_level(Y, i)) {
+      for (int k = 0; k < std::max(1, (int)o[20]); k++) learned_post(Y, lv, Y.nets[i]);
+      pre_done = true;
+    }
+  }
+  if (!pre_done) level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
   css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   boundary<T>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
   restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st);
@@ -448,7 +462,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   axpy<T>(v.x.p, v.aux.p, nullptr, 1.0, o[13], lv.L.N, st);
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
-    if (Y.learn_active && i >= (int)Y.lev.size() - Y.learned_levels) {
+    if (learned_level(Y, i)) {
       for (int k = 0; k < std::max(1, (int)o[20]); k++) learned_post(Y, lv, Y.nets[i]);
       return;
     }
@@ -485,12 +499,14 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(v.f.p, vin, lv.L.N, st);
   const int pi = std::is_same<T, double>::value ? 0 : 1;
   System::Graph &g = Y.graphs[pi];
-  double key[12];
+  double key[14];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
   const bool learned = pi == 1 && Y.learn_active;
   key[9] = learned ? Y.learned_levels : 0;
   key[10] = learned ? std::max(1.0, Y.opt[20]) : 0;
   key[11] = Y.opt[21];
+  key[12] = learned ? Y.learned_skip : 0;
+  key[13] = learned ? Y.learned_pre : 0;
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -749,8 +765,11 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
 // [2] learning rate, [3] seed, [4] finest levels using the network (default:
 // all but the coarsest), [5] loss: 0 squared error against -x, 1 the scaled
 // defect left after the correction (learned_res_step), [6] samples: 0 random
-// start and f = 0, 1 zero start and a random f, 2 half each.  tinfo: [0] mean loss
-// of the first 10 steps per level, [1] of the last ones, [2] seconds, [3] steps.
+// start and f = 0, 1 zero start and a random f, 2 half each, [7] the finest
+// levels among [4] that are not trained (they keep the plain smoother).
+// tinfo: [0] mean loss of the first 10 steps of the finest trained level, [1]
+// of its last ones, [2] seconds, [3] steps, [4 + 2j] / [5 + 2j] the same for
+// level L - j.
 static void learned_train(System &Y, const double *to, double *ti) {
   hipStream_t st = Y.ctx->st;
   const int ns = to[0] > 0 ? (int)to[0] : 16;
@@ -762,21 +781,23 @@ static void learned_train(System &Y, const double *to, double *ti) {
   const bool residual_loss = to[5] != 0.0;
   const int data_mode = (int)to[6];  // 0 random error, f = 0; 1 x = 0, random f; 2 both
   const int nl = Y.learned_levels;
+  const int j0 = std::max(0, std::min((int)to[7], nl - 1));  // finest levels left untrained
   Y.trained_levels = nl;
   Y.learn_active = false;
   ensure_learned(Y);
-  for (int j = 0; j < nl; j++) {
+  for (int j = j0; j < nl; j++) {
     LearnedNet &net = Y.nets[L - j];
     if (!net.ready) net_init(net, (seed ^ 0x5eedu) + 31u * (uint32_t)j, st);
     if (net_reserve(net, Y.lev[L - j]->L.N, true)) Y.invalidate_graphs();
   }
   // samples of level L - j at off[j] + k N_j
   std::vector<size_t> off(nl + 1, 0);
-  for (int j = 0; j < nl; j++) off[j + 1] = off[j] + (size_t)ns * Y.lev[L - j]->L.N;
+  for (int j = 0; j < nl; j++) off[j + 1] = off[j] + (j >= j0 ? (size_t)ns * Y.lev[L - j]->L.N : 0);
   DevBuf<float> du, dt;
   du.alloc(off[nl]);
   if (!residual_loss) dt.alloc(off[nl]);
-  const int total = steps * nl;
+  const int nt = nl - j0;  // trained levels
+  const int total = steps * nt;
   DevBuf<float> losses;
   losses.alloc(total);
   hipEvent_t e0, e1;
@@ -784,7 +805,7 @@ static void learned_train(System &Y, const double *to, double *ti) {
   check(hipEventCreate(&e1), "event");
   check(hipEventRecord(e0, st), "event");
   set_recursion(Y);
-  for (int j = 0; j < nl; j++) {
+  for (int j = j0; j < nl; j++) {
     Level &lv = *Y.lev[L - j];
     Vecs<float> &v = lv.v32;
     const int64_t N = lv.L.N;
@@ -812,7 +833,7 @@ static void learned_train(System &Y, const double *to, double *ti) {
     }
   }
   for (int k = 0; k < total; k++) {
-    const int j = k % nl, kk = k / nl, smp = kk % ns;
+    const int j = j0 + k % nt, kk = k / nt, smp = kk % ns;
     Level &lv = *Y.lev[L - j];
     LearnedNet &net = Y.nets[L - j];
     const size_t o = off[j] + (size_t)smp * lv.L.N;
@@ -838,10 +859,13 @@ static void learned_train(System &Y, const double *to, double *ti) {
   // mean loss of the first / last 10 steps of each level's network
   if (ti) {
     const int w = std::min(10, steps);
-    for (int j = 0; j < nl && 4 + 2 * j + 1 < 16; j++) {
+    for (int j = j0; j < nl && 4 + 2 * j + 1 < 16; j++) {
       double a = 0, b = 0;
-      for (int q = 0; q < w; q++) { a += h[(size_t)q * nl + j]; b += h[(size_t)(steps - 1 - q) * nl + j]; }
-      if (j == 0) { ti[0] = a / w; ti[1] = b / w; }
+      for (int q = 0; q < w; q++) {
+        a += h[(size_t)q * nt + (j - j0)];
+        b += h[(size_t)(steps - 1 - q) * nt + (j - j0)];
+      }
+      if (j == j0) { ti[0] = a / w; ti[1] = b / w; }
       ti[4 + 2 * j] = a / w;
       ti[5 + 2 * j] = b / w;
     }
@@ -887,7 +911,9 @@ static int solve_device(System &Y, double *info) {
     Y.fallbacks = 0;
     if (prec == 2) {
       Y.learned_levels = std::min(Y.opt[19] > 0 ? (int)Y.opt[19] : Y.trained_levels, (int)Y.lev.size() - 1);
-      for (int i = (int)Y.lev.size() - Y.learned_levels; i < (int)Y.lev.size(); i++)
+      Y.learned_skip = std::max(0, std::min((int)Y.opt[22], Y.learned_levels));
+      Y.learned_pre = Y.opt[23] != 0;
+      for (int i = (int)Y.lev.size() - Y.learned_levels; i < (int)Y.lev.size() - Y.learned_skip; i++)
         if (!Y.nets[i].ready) throw std::logic_error("learned smoother: train or load parameters first");
       ensure_learned(Y);
       Y.learn_active = true;
