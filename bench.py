@@ -33,10 +33,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
-# HBM bytes per FP32 SpMV launch at 64^3 Q2: rocprofv3 FETCH_SIZE x 2.000 +
-# WRITE_SIZE x 1.000 (the factors that make the counters of a 1 GiB device
-# copy read 1 GiB each) = 535.1 MB + 9.4 MB per dispatch
-TRAFFIC_BYTES = 544_480_000
+# HBM bytes per FP32 SpMV launch at 64^3 Q2 (k_rows<float,0,1>): rocprofv3
+# FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (the factors that make the counters
+# of a 1 GiB device copy read 1 GiB each) = 537.7 MB + 18.3 MB per dispatch
+TRAFFIC_BYTES = 556_016_000
 
 
 def cpu_baseline(nref_cpu=4, seed=2024):
@@ -238,12 +238,15 @@ def main():
     ap.add_argument("--nref", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="learned,fp32,fp64")
-    ap.add_argument("--learned-levels", type=int, default=0,
+    # learned-form defaults: the steady Q2 workload smooths the levels below
+    # the finest with two network steps in place of pre- and post-smoothing;
+    # the transient Q1 workload (8-colour SSOR, cheap on coarse levels) uses
+    # one network step as the finest level's post-smoother
+    ap.add_argument("--learned-levels", type=int, default=None,
                     help="finest levels with a network (0: all but the coarsest)")
-    ap.add_argument("--learned-skip", type=int, default=1,
-                    help="the finest of those levels keep SSOR")
-    ap.add_argument("--learned-steps", type=int, default=2, help="network steps per smoothing")
-    ap.add_argument("--learned-pre", type=int, default=1, help="1: the network also replaces pre-smoothing")
+    ap.add_argument("--learned-skip", type=int, default=None, help="the finest of those levels keep SSOR")
+    ap.add_argument("--learned-steps", type=int, default=None, help="network steps per smoothing")
+    ap.add_argument("--learned-pre", type=int, default=None, help="1: the network also replaces pre-smoothing")
     ap.add_argument("--train-steps", type=int, default=1200)
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
@@ -252,6 +255,11 @@ def main():
                     help="train the learned smoother across FIELDS seeded 64^3 Q1 fields instead")
     ap.add_argument("--eval-seeds", type=int, default=4)
     args = ap.parse_args()
+    dflt = dict(learned_levels=1, learned_skip=0, learned_steps=1, learned_pre=0) if args.transient > 0 else \
+        dict(learned_levels=0, learned_skip=1, learned_steps=2, learned_pre=1)
+    for k, v in dflt.items():
+        if getattr(args, k) is None:
+            setattr(args, k, v)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)

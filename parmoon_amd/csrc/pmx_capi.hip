@@ -55,7 +55,7 @@ struct Ctx {
   int device = 0;
   int systems = 0;  // live systems on this context
   hipStream_t st = nullptr;
-  DevBuf<double> part;  // reduction partials
+  DevBuf<double> part;  // reduction partials: 2 x kRedBlocks, then scalars
   double *pinned = nullptr;
 };
 
@@ -406,15 +406,13 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   const double sq = std::sqrt((double)lv.L.nactive);
   css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
-  learned_input(lv.dinv.p, v.d.p, lv.lc.p, N, st);
-  dot<float>(lv.lc.p, lv.lc.p, N, part, sc, 1, st);
+  learned_input_norm(lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, st);
   learned_build_input(net, lv.lc.p, sc, sq, lv.feat.p, N, st);
   net_forward_act(net, lv.L.M, false, st);
   learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
   boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
-  dot<float>(v.d.p, lv.lac.p, N, part, sc + 1, 0, st);
-  dot<float>(lv.lac.p, lv.lac.p, N, part, sc + 2, 0, st);
+  learned_dots(v.d.p, lv.lac.p, N, part, sc + 1, st);
   learned_step(v.x.p, lv.lc.p, sc + 1, N, st);
 }
 
@@ -892,7 +890,7 @@ static int solve_device(System &Y, double *info) {
   double res = 0, res0 = 0;
   const int maxit = (int)Y.opt[2];
   const double rel_target = Y.opt[16];
-  double *nr = Y.ctx->part.p + kRedBlocks;
+  double *nr = Y.ctx->part.p + 2 * kRedBlocks;
   if (prec == 0) {
     double rmin = Y.opt[3];
     if (rel_target > 0) {
@@ -1041,7 +1039,7 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
     auto *c = new pmx_ctx();
     c->c.device = device;
     check(hipStreamCreateWithFlags(&c->c.st, hipStreamNonBlocking), "stream");
-    c->c.part.alloc(kRedBlocks + 64);
+    c->c.part.alloc(2 * kRedBlocks + 64);
     check(hipHostMalloc((void **)&c->c.pinned, 64 * sizeof(double), hipHostMallocDefault), "pinned");
     *out = c;
   })
@@ -1306,7 +1304,7 @@ int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, dou
     to_lattice<double>(Y, lv, b, lv.v64.f.p);
     css_apply<double>(lv.D, 1, lv.v64.val.p, lv.v64.aux.p, lv.v64.f.p, lv.v64.d.p, 0.0, -1, st);
     boundary<double>(lv.D, 1, lv.v64.d.p, lv.v64.aux.p, lv.v64.f.p, st);
-    double *nr = Y.ctx->part.p + kRedBlocks;
+    double *nr = Y.ctx->part.p + 2 * kRedBlocks;
     dot<double>(lv.v64.d.p, lv.v64.d.p, lv.L.N, Y.ctx->part.p, nr, 1, st);
     if (res) *res = read_scalar(Y, nr);
     from_lattice<double>(Y, lv, lv.v64.d.p, r);

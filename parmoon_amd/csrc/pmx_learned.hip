@@ -14,6 +14,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "pmx_kernels.hpp"
 #include "pmx_learned.hpp"
 #include "pmx_ops.hpp"
 
@@ -606,6 +607,61 @@ __global__ void k_learned_input(const float *__restrict__ dinv, const float *__r
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     u[i] = dinv[i] * d[i];
 }
+// u = dinv d together with the partial sums of u^2 (the grid and summation
+// order of k_dot_partial, so k_reduce_norm gives the same ||u|| as dot())
+__global__ __launch_bounds__(kBlock) void k_learned_input_sq(const float *__restrict__ dinv,
+                                                             const float *__restrict__ d, float *__restrict__ u,
+                                                             double *__restrict__ part, int64_t n) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float v = dinv[i] * d[i];
+    u[i] = v;
+    s += (double)v * (double)v;
+  }
+  s = block_sum<float>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// partial sums of (a, b) and (b, b) in one pass (k_dot_partial's order)
+__global__ __launch_bounds__(kBlock) void k_dot2_partial(const float *__restrict__ a, const float *__restrict__ b,
+                                                         double *__restrict__ pab, double *__restrict__ pbb,
+                                                         int64_t n) {
+  double s = 0, t = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const double bv = (double)b[i];
+    s += (double)a[i] * bv;
+    t += bv * bv;
+  }
+  s = block_sum<float>(s);
+  t = block_sum<float>(t);
+  if (threadIdx.x == 0) {
+    pab[blockIdx.x] = s;
+    pbb[blockIdx.x] = t;
+  }
+}
+// out[0] = sqrt(sum(part)) (k_reduce_final's order)
+__global__ __launch_bounds__(kBlock) void k_reduce_norm(const double *__restrict__ part, int np,
+                                                        double *__restrict__ out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+// out[0] = sum(pab), out[1] = sum(pbb) (k_reduce_final's order)
+__global__ __launch_bounds__(kBlock) void k_reduce_final2(const double *__restrict__ pab,
+                                                          const double *__restrict__ pbb, int np,
+                                                          double *__restrict__ out) {
+  double s = 0, t = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+    s += pab[i];
+    t += pbb[i];
+  }
+  s = block_sum<double>(s);
+  t = block_sum<double>(t);
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = t;
+  }
+}
 // act0[p] = feat[p] with channel 0 = bf16(u[p] * sq / *nrm) (no scaling if nrm is null)
 __global__ void k_build_input(const float *__restrict__ u, const double *__restrict__ nrm, double sq,
                               const uint16_t *__restrict__ feat, int64_t n, uint16_t *__restrict__ act0) {
@@ -695,6 +751,20 @@ void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStre
 }
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_learned_input, dim3(grid_for(n)), dim3(256), 0, st, dinv, d, u, n);
+}
+static unsigned red_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kRedBlocks));
+}
+void learned_input_norm(const float *dinv, const float *d, float *u, int64_t n, double *part, double *nrm,
+                        hipStream_t st) {
+  const unsigned g = red_grid(n);
+  hipLaunchKernelGGL(k_learned_input_sq, dim3(g), dim3(kBlock), 0, st, dinv, d, u, part, n);
+  hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(kBlock), 0, st, part, (int)g, nrm);
+}
+void learned_dots(const float *d, const float *ac, int64_t n, double *part, double *out, hipStream_t st) {
+  const unsigned g = red_grid(n);
+  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
+  hipLaunchKernelGGL(k_reduce_final2, dim3(1), dim3(kBlock), 0, st, part, part + kRedBlocks, (int)g, out);
 }
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sq, const uint16_t *feat,
                          int64_t n, hipStream_t st) {

@@ -81,6 +81,11 @@ struct DevLayout;
 void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st);
 // u = dinv * d (lattice), the unscaled network input
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st);
+// u = dinv d and *nrm = ||u|| (part: kRedBlocks partials)
+void learned_input_norm(const float *dinv, const float *d, float *u, int64_t n, double *part, double *nrm,
+                        hipStream_t st);
+// out[0] = (d, ac), out[1] = (ac, ac) (part: 2 kRedBlocks partials)
+void learned_dots(const float *d, const float *ac, int64_t n, double *part, double *out, hipStream_t st);
 // net.act[0] = feat with channel 0 = bf16(u * sqrt_count / *nrm) (nrm: device
 // FP64 2-norm of u; nrm == nullptr: no scaling)
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
