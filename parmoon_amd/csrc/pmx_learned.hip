@@ -262,12 +262,12 @@ __global__ void k_gelu_back(const uint16_t *__restrict__ gpost, const float *__r
   if (i >= n) return;
   gpre[i] = f2bf(bf2f(gpost[i]) * gelu_d(pre[i]));
 }
-__global__ void k_sum_parts(const double *__restrict__ p, int n, float *__restrict__ o) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0;
-    for (int i = 0; i < n; i++) s += p[i];
-    o[0] = (float)s;
-  }
+// o = sum(p) in a fixed order (one workgroup of kBlock lanes)
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const double *__restrict__ p, int n, float *__restrict__ o) {
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) s += p[i];
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) o[0] = (float)s;
 }
 // Weight and bias gradients on MFMA: for every tap t,
 //   dW[o][t][c] = sum over nodes n of gp[n][o] a[n + off_t][c]
@@ -377,11 +377,24 @@ __global__ __launch_bounds__(256) void k_wgrad_mfma(int M, const uint16_t *__res
 }
 
 // grads[e] += sum_b part[b][e] (fixed order)
-__global__ void k_reduce_parts(const float *__restrict__ part, int nb, int ne, float *__restrict__ grads) {
+constexpr int kRedChunk = 32;  // weight-gradient partials summed per first-pass thread
+constexpr int kRedChunks = (512 + kRedChunk - 1) / kRedChunk;  // for the 512 wgrad workgroups
+// grads[e] += sum over blocks b of part[b][e], in two fixed-order passes:
+// chunk c of kRedChunk blocks into tmp[c][e], then the chunks
+__global__ void k_reduce_parts(const float *__restrict__ part, int nb, int ne, float *__restrict__ tmp) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int b0 = blockIdx.y * kRedChunk, b1 = min(nb, b0 + kRedChunk);
+  float s = 0.f;
+#pragma unroll 8
+  for (int b = b0; b < b1; b++) s += part[(int64_t)b * ne + e];
+  tmp[(int64_t)blockIdx.y * ne + e] = s;
+}
+__global__ void k_reduce_chunks(const float *__restrict__ tmp, int nc, int ne, float *__restrict__ grads) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   float s = 0.f;
-  for (int b = 0; b < nb; b++) s += part[(int64_t)b * ne + e];
+  for (int c = 0; c < nc; c++) s += tmp[(int64_t)c * ne + e];
   grads[e] += s;
 }
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
@@ -432,6 +445,7 @@ void LearnedData::release() {
 
 static const int kWStride = 16 * kTaps * 16;  // BF16 weight slots per layer (padded)
 static const int kWgradBlocks = 512;
+static_assert(kRedChunks * kRedChunk >= kWgradBlocks, "first reduction pass must cover the wgrad partials");
 static const int kLossBlocks = 1024;
 
 void net_init(LearnedNet &net, uint32_t seed, hipStream_t st) {
@@ -490,7 +504,8 @@ bool net_reserve(LearnedNet &net, int64_t N, bool training) {
     for (auto &g : net.gbuf) check(hipMalloc(&g, sizeof(uint16_t) * net.cap * kNetCh), "net grad buf");
     if (!net.lpart) check(hipMalloc(&net.lpart, sizeof(double) * kLossBlocks), "net loss partials");
   }
-  const int64_t pc = (int64_t)kWgradBlocks * (16 * kTaps * 16 + 16);
+  // partials of every workgroup, then the first-pass sums
+  const int64_t pc = (int64_t)(kWgradBlocks + kRedChunks) * (16 * kTaps * 16 + 16);
   if (training && net.part_cap < pc) {
     if (net.part) (void)hipFree(net.part);
     check(hipMalloc(&net.part, sizeof(float) * pc), "net part");
@@ -530,7 +545,10 @@ template <int CIN, int COUT>
 static void wgrad(LearnedNet &net, int M, const uint16_t *gp, const uint16_t *a, int layer, hipStream_t st) {
   constexpr int NE = COUT * kTaps * CIN + COUT;
   hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3(kWgradBlocks), dim3(256), 0, st, M, gp, a, net.part);
-  hipLaunchKernelGGL(k_reduce_parts, dim3((NE + 255) / 256), dim3(256), 0, st, net.part, kWgradBlocks, NE,
+  float *tmp = net.part + (int64_t)kWgradBlocks * NE;
+  hipLaunchKernelGGL(k_reduce_parts, dim3((NE + 255) / 256, kRedChunks), dim3(256), 0, st, net.part, kWgradBlocks,
+                     NE, tmp);
+  hipLaunchKernelGGL(k_reduce_chunks, dim3((NE + 255) / 256), dim3(256), 0, st, tmp, kRedChunks, NE,
                      net.grads + net_param_offset(layer));
 }
 
@@ -544,7 +562,7 @@ void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipSt
   // output layer: gy (BF16 in g0[:N])
   hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, st, M, net.out, t, (float)(1.0 / inner), g0, lpart);
   backward_layers(net, M, st);
-  if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, lpart, kLossBlocks, loss_dev);
+  if (loss_dev) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, st, lpart, kLossBlocks, loss_dev);
 }
 
 __global__ void k_split_grad(const float *__restrict__ gy, uint16_t *__restrict__ gb, int64_t n) {
@@ -819,39 +837,48 @@ __global__ __launch_bounds__(256) void k_res_loss(int M, const float *__restrict
   if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) * inv_count;
 }
 // y_j = sum over interior rows i of a_ij g_i, for interior j (0 on the boundary):
-// the transpose of the stored stencil operator restricted to interior nodes.
-// Row i holds a_ij at the slot of offset j - i when |j - i| is within its extents.
+// the transpose of the stored stencil operator restricted to interior nodes:
+// y_j = sum_i a_ij g_i over the interior rows i = j + d whose stencil holds
+// offset -d.  A workgroup takes 64 columns j of one residue class, so the
+// class of every neighbour row j + d, its stencil shape and the slot of -d are
+// the same across the workgroup (scalar loads); only the row index inside
+// that class and the bounds vary per lane.  Boundary entries of y are set by
+// the caller.
 template <int E>
-__global__ __launch_bounds__(256) void k_css_transpose(int M, int S, const DevSegment *__restrict__ segs,
-                                                       const int *__restrict__ segidx, const float *__restrict__ val,
-                                                       const float *__restrict__ g, float *__restrict__ y) {
-  const int64_t N = (int64_t)M * M * M;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(j % M), yy = (int)((j / M) % M), z = (int)(j / ((int64_t)M * M));
-    if (x == 0 || yy == 0 || z == 0 || x == M - 1 || yy == M - 1 || z == M - 1) { y[j] = 0.f; continue; }
-    float acc = 0.f;
-    for (int dz = -E; dz <= E; dz++) {
-      const int iz = z + dz;
-      if (iz < 1 || iz > M - 2) continue;
-      for (int dy = -E; dy <= E; dy++) {
-        const int iy = yy + dy;
-        if (iy < 1 || iy > M - 2) continue;
-        for (int dx = -E; dx <= E; dx++) {
-          const int ix = x + dx;
-          if (ix < 1 || ix > M - 2) continue;
-          const int si = segidx[(ix % S) + S * ((iy % S) + S * (iz % S))];
-          const DevSegment s = segs[si];
-          // offset of column j in row i is -d
-          if (dx < -s.ex || dx > s.ex || dy < -s.ey || dy > s.ey || dz < -s.ez || dz > s.ez) continue;
-          const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
-          const int idx = (ix - s.x0) / S + s.cx * ((iy - s.y0) / S + s.cy * ((iz - s.z0) / S));
-          const int64_t vb = s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
-          acc += val[vb + (int64_t)slot * kSlice] * g[ix + (int64_t)M * (iy + (int64_t)M * iz)];
-        }
-      }
-    }
-    y[j] = acc;
+__global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo *__restrict__ blocks,
+                                                                 const DevSegment *__restrict__ segs,
+                                                                 const int *__restrict__ segidx, int M, int S,
+                                                                 const float *__restrict__ val,
+                                                                 const float *__restrict__ g, float *__restrict__ y) {
+  const BlockInfo bi = blocks[blockIdx.x];
+  const DevSegment sj = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  const bool valid = idx < sj.count;
+  const int rx = sj.x0 % S, ry = sj.y0 % S, rz = sj.z0 % S;
+  int x = 0, yy = 0, z = 0;
+  if (valid) {
+    const int i = idx % sj.cx, t = idx / sj.cx;
+    x = sj.x0 + S * i;
+    yy = sj.y0 + S * (t % sj.cy);
+    z = sj.z0 + S * (t / sj.cy);
   }
+  float acc = 0.f;
+  for (int dz = -E; dz <= E; dz++)
+    for (int dy = -E; dy <= E; dy++)
+      for (int dx = -E; dx <= E; dx++) {
+        const int si = segidx[((rx + dx + S) % S) + S * (((ry + dy + S) % S) + S * ((rz + dz + S) % S))];
+        if (si < 0) continue;
+        const DevSegment s = segs[si];
+        // column j sits at offset -d of row i = j + d
+        if (dx < -s.ex || dx > s.ex || dy < -s.ey || dy > s.ey || dz < -s.ez || dz > s.ez) continue;
+        const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
+        const int ix = x + dx, iy = yy + dy, iz = z + dz;
+        if (!valid || ix < 1 || ix > M - 2 || iy < 1 || iy > M - 2 || iz < 1 || iz > M - 2) continue;
+        const int ri = (ix - s.x0) / S + s.cx * ((iy - s.y0) / S + s.cy * ((iz - s.z0) / S));
+        const int64_t vb = s.voff + (int64_t)(ri / kSlice) * ((int64_t)s.ns * kSlice) + (ri % kSlice);
+        acc += val[vb + (int64_t)slot * kSlice] * g[ix + (int64_t)M * (iy + (int64_t)M * iz)];
+      }
+  if (valid) y[x + (int64_t)M * (yy + (int64_t)M * z)] = acc;
 }
 
 void learned_mask(const float *in, int M, float *out, hipStream_t st) {
@@ -864,14 +891,16 @@ void learned_res_loss(LearnedNet &net, int M, const float *u, const float *dinv,
                      net.lpart);
 }
 void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, net.lpart, kLossBlocks, loss_dev);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, st, net.lpart, kLossBlocks, loss_dev);
 }
 void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st) {
-  const unsigned grid = (unsigned)std::min<int64_t>((D.N + 255) / 256, 8192);
   if (D.order == 2)
-    hipLaunchKernelGGL(k_css_transpose<2>, dim3(grid), dim3(256), 0, st, D.M, D.S, D.segs, D.segidx, val, g, y);
+    hipLaunchKernelGGL(k_css_transpose<2>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs,
+                       D.segidx, D.M, D.S, val, g, y);
   else
-    hipLaunchKernelGGL(k_css_transpose<1>, dim3(grid), dim3(256), 0, st, D.M, D.S, D.segs, D.segidx, val, g, y);
+    hipLaunchKernelGGL(k_css_transpose<1>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs,
+                       D.segidx, D.M, D.S, val, g, y);
+  boundary<float>(D, 3, y, nullptr, nullptr, st);  // y = 0 on the boundary nodes
 }
 
 }  // namespace pmx

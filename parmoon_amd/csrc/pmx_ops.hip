@@ -212,6 +212,12 @@ __global__ __launch_bounds__(sweep_threads<T>()) void k_sweep_wg(const DevSegmen
   for (int i = threadIdx.x; i < N; i += NT) x[i] = lx[i];
 }
 
+// flag = 1 if a and b differ in any bit (all writers store the same value)
+__global__ void k_bits_differ(const double *__restrict__ a, const double *__restrict__ b, int64_t n,
+                              int *__restrict__ flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (__double_as_longlong(a[i]) != __double_as_longlong(b[i])) *flag = 1;
+}
 // ============================================================ transfers
 // 1D taps of the nodal interpolation from the coarse to the fine lattice
 // (the prolongation matrices of FEDatabase3D.C:1656 for Q1/Q2 Lagrange,
@@ -690,6 +696,15 @@ template <typename T>
 void fill(T *x, T v, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, x, v, n);
 }
+bool bits_equal(const double *a, const double *b, int64_t n, int *flag, hipStream_t st) {
+  fill<int>(flag, 0, 1, st);
+  hipLaunchKernelGGL(k_bits_differ, dim3(grid_for(n)), dim3(kBlock), 0, st, a, b, n, flag);
+  int h = 1;
+  if (hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    throw std::runtime_error("bits_equal: copy failed");
+  return h == 0;
+}
 template <typename T>
 void copy(T *y, const T *x, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, n);
@@ -915,6 +930,7 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
 PMX_INST(double)
 PMX_INST(float)
 template void fill<uint16_t>(uint16_t *, uint16_t, int64_t, hipStream_t);
+template void fill<int>(int *, int, int64_t, hipStream_t);
 template void convert<float, double>(float *, const double *, int64_t, hipStream_t);
 template void convert<double, float>(double *, const float *, int64_t, hipStream_t);
 template void add_convert<double, float>(double *, const float *, int64_t, hipStream_t);

@@ -45,6 +45,8 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st);
+// a == b bit for bit (n doubles; flag: one device int of scratch; synchronises st)
+bool bits_equal(const double *a, const double *b, int64_t n, int *flag, hipStream_t st);
 // whole SOR (ssor 0) / SSOR (ssor 1) sweep with Dirichlet rows x = f, one
 // workgroup with the level's x in LDS (levels with M^3 * s <= kSweepLdsBytes)
 inline bool sweep_wg_fits(const DevLayout &D, size_t s) { return (int64_t)D.N * (int64_t)s <= kSweepLdsBytes; }

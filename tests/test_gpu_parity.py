@@ -254,3 +254,34 @@ def test_wg_sweep_matches_launch_per_colour(ctx, order, nref):
         assert ia[0] == ib[0], (precision, smoother, ia[0], ib[0])
         assert np.array_equal(a, b), (precision, smoother, float(np.max(np.abs(a - b))))
     S.close()
+
+
+def test_coarse_inverse_follows_the_operator(ctx):
+    """The coarsest level's inverse is rebuilt only when its operator changed
+    bit for bit: re-assembling with other coefficients must give the solve of
+    a fresh system with those coefficients, and going back must give the
+    first solve again, bit for bit."""
+    print("[case] coarse inverse cache", flush=True)
+    pa_, pb_ = P.config2(nref=4), P.config2(nref=4)
+    pb_[P.EPS] = 10 * pa_[P.EPS]
+    opt = P.solver_opt(solver=16, restart=20, maxit=100, res_norm_min=0.0, red_factor=0.0, rel_target=1e-10,
+                       smoother=3, omega=1.0)
+
+    def solve(S, prm):
+        S.assemble(prm)
+        info, rc = S.solve_resident(opt)
+        assert rc == 0, info
+        return S.solution()
+
+    S = pa.System(ctx, 2, 3, 2)
+    xa = solve(S, pa_)
+    xa2 = solve(S, pa_)
+    xb = solve(S, pb_)
+    xa3 = solve(S, pa_)
+    F = pa.System(ctx, 2, 3, 2)
+    xb_fresh = solve(F, pb_)
+    assert np.array_equal(xa, xa2) and np.array_equal(xa, xa3)
+    assert np.array_equal(xb, xb_fresh)
+    assert not np.array_equal(xa, xb)
+    S.close()
+    F.close()
