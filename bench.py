@@ -360,6 +360,14 @@ def main():
         lv = -1 - args.learned_skip
         kern[f"learned_step_level{lv}_ms"] = S.time_kernel(8, lv, 1, reps=10)
         kern[f"net_forward_level{lv}_ms"] = S.time_kernel(9, lv, 1, reps=10)
+    # assembly with the reference's default 64-point rule (Gauss4); the solves
+    # above use the 27-point rule of INTERNAL_PROBLEM_LINEAR = 1
+    prm4 = prm.copy()
+    prm4[P.LINEAR] = 0.0
+    S.assemble(prm4)
+    asm4_finest_ms = S.time_kernel(4, -1, 0, reps=3)
+    t_asm4 = sum(S.time_kernel(4, lvl, 0, reps=3) for lvl in range(5)) * 1e-3
+    flops_asm4 = 428544.0 * (2 ** nref) ** 3  # Q2, 64-point rule, B^T D B model per cell
 
     main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
     t_main = res[main_form]["t_solve"]
@@ -392,6 +400,9 @@ def main():
         "assembly_s": round(t_asm, 5),
         "assemble_call_s": round(t_asm_call, 5),
         "assembly_kernels_gflops": round(flops_asm / (kern["assembly_finest_ms"] * 1e-3) / 1e9, 1),
+        "assembly_gauss4": {"dofs_per_s": round(N / t_asm4, 1), "s": round(t_asm4, 5),
+                            "finest_ms": round(asm4_finest_ms, 4),
+                            "kernels_gflops": round(flops_asm4 / (asm4_finest_ms * 1e-3) / 1e9, 1)},
         "iterations": {f: res[f]["iters"] for f in res},
         "fallbacks": {f: res[f]["fallbacks"] for f in res},
         "final_rel_residual": {f: res[f]["rel_res"] for f in res},

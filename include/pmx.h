@@ -134,6 +134,11 @@ int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, doub
  * (SystemCD3D.C:356).  The solution stays on the device. */
 int pmx_system_solve_resident(pmx_system *sys, const double *opt, double *info);
 int pmx_system_solution(pmx_system *sys, double *sol);
+/* Errors of the solution held on the device (last solve or time step) against
+ * the example's exact solution at prm[PMX_PRM_TIME], with the quadrature rule
+ * of prm (TFEFunction3D::GetErrors with L2H1Errors, src/FE/FEFunction3D.C:83-300,
+ * 3DPrograms/TCD3D_ParMooN.C:640-655).  err: [0] L2 error [1] H1 semi-norm error. */
+int pmx_system_errors(pmx_system *sys, const double *prm, double *err);
 
 /* ---- time stepping --------------------------------------------------------
  * TSystemTCD3D (src/System/SystemTCD3D.C:208-428) as driven by the time loop

@@ -57,6 +57,7 @@ def lib():
         L.pmx_system_solve.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
         L.pmx_system_solution.argtypes = [vp, vp]
+        L.pmx_system_errors.argtypes = [vp, vp, vp]
         L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
         L.pmx_tcd_init.argtypes = [vp, vp, vp]
         L.pmx_tcd_step.argtypes = [vp, vp, vp, vp, vp]
@@ -240,6 +241,13 @@ class System:
         if check:
             _ok(rc, "pmx_system_solve_resident")
         return info, rc
+
+    def errors(self, prm_block):
+        """(L2 error, H1 semi-norm error) of the device solution against the
+        example's exact solution at prm_block[TIME]."""
+        e = np.zeros(2)
+        _ok(lib().pmx_system_errors(self.h, _p(np.ascontiguousarray(prm_block, float)), _p(e)), "pmx_system_errors")
+        return float(e[0]), float(e[1])
 
     def solution(self):
         s = np.zeros(int(self.dims(-1)[0]))

@@ -1407,6 +1407,21 @@ int pmx_system_solution(pmx_system *sys, double *sol) {
   })
 }
 
+int pmx_system_errors(pmx_system *sys, const double *prm, double *err) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!prm || !err) throw std::invalid_argument("null argument");
+    if (!Y.sol.p) throw std::logic_error("no solution on the device: solve first");
+    hipStream_t st = Y.ctx->st;
+    ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
+    Y.prm.upload(prm, PMX_PRM_LEN, st);
+    double *out = Y.ctx->part.p + 2 * kRedBlocks + 8;
+    fe_errors(Y.lev.back()->D, Y.prm.p, Y.qref.p, Y.tab.p, Y.nq, Y.sol.p, Y.ctx->part.p, out, st);
+    check(hipMemcpyAsync(err, out, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "errors");
+    check(hipStreamSynchronize(st), "errors");
+  })
+}
+
 int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms) {
   PMX_TRY({
     System &Y = sys->s;

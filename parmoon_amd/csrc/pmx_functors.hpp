@@ -110,6 +110,30 @@ __host__ __device__ inline double example_exact(const double *prm, double x, dou
   return sin(pi * x) * sin(pi * y) * sin(pi * z);
 }
 
+// the exact solution and its gradient (the ExactU functions of the examples:
+// Laplace.h:12-19, amc.h:16-26; the rotating blob's reference function has a
+// zero gradient entry, as in the oracle), v = {u, u_x, u_y, u_z}
+__host__ __device__ inline void example_exact_grad(const double *prm, double x, double y, double z, double v[4]) {
+  const double pi = 3.14159265358979323846;
+  const int ex = (int)prm[PMX_PRM_EXAMPLE];
+  const double t = prm[PMX_PRM_TIME];
+  if (ex == PMX_EX_HEAT_AMC) {
+    const double e = exp(-0.1 * t);
+    v[0] = e * sin(pi * x) * cos(pi * y) * cos(pi * z);
+    v[1] = pi * e * cos(pi * x) * cos(pi * y) * cos(pi * z);
+    v[2] = -pi * e * sin(pi * x) * sin(pi * y) * cos(pi * z);
+    v[3] = -pi * e * sin(pi * x) * cos(pi * y) * sin(pi * z);
+  } else if (ex == PMX_EX_ROTATING) {
+    v[0] = example_exact(prm, x, y, z);
+    v[1] = v[2] = v[3] = 0.0;
+  } else {
+    v[0] = sin(pi * x) * sin(pi * y) * sin(pi * z);
+    v[1] = pi * cos(pi * x) * sin(pi * y) * sin(pi * z);
+    v[2] = pi * sin(pi * x) * cos(pi * y) * sin(pi * z);
+    v[3] = pi * sin(pi * x) * sin(pi * y) * cos(pi * z);
+  }
+}
+
 // Compute_SDFEM_delta for SDFEM_TYPE 0/1/2 with CELL_MEASURE 0
 __host__ __device__ inline double sdfem_delta(const double *prm, double hK, double eps, double b1, double b2,
                                               double b3) {

@@ -856,6 +856,83 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, const doub
                      D.nboundary, D.M, prm_dev, rhs);
 }
 
+// ============================================================ error norms
+// ||u - u_h||_L2 and |u - u_h|_H1 of a lattice FE function against the
+// example's exact solution (TFEFunction3D::GetErrors with L2H1Errors,
+// FEFunction3D.C:83-300, MainUtilities.C:1649): one thread per cell gathers
+// its (order + 1)^3 nodal values, evaluates u_h and grad u_h at the rule's
+// points from the reference tables (brick of side h: d/dx = (2/h) d/dxi,
+// |detJ| = h^3/8) and adds w |detJ| (u - u_h)^2 and the gradient term;
+// fixed-order block partials, then k_reduce_errors.
+template <int ORDER>
+__global__ __launch_bounds__(kBlock) void k_errors(const double *__restrict__ prm, int n, int M,
+                                                   const double *__restrict__ qref, const double *__restrict__ tab,
+                                                   int nq, const double *__restrict__ u,
+                                                   double *__restrict__ part) {
+  constexpr int K = ORDER + 1, NB = K * K * K;
+  const int64_t ncells = (int64_t)n * n * n;
+  const double h = 1.0 / n, adet = h * h * h / 8.0, g = 2.0 / h;
+  double e0 = 0, e1 = 0;
+  for (int64_t c = blockIdx.x * (int64_t)kBlock + threadIdx.x; c < ncells; c += (int64_t)gridDim.x * kBlock) {
+    const int ci = (int)(c % n), cj = (int)((c / n) % n), ck = (int)(c / ((int64_t)n * n));
+    double uc[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int a = b % K, bb = (b / K) % K, cc = b / (K * K);
+      uc[b] = u[(ORDER * ci + a) + (int64_t)M * ((ORDER * cj + bb) + (int64_t)M * (ORDER * ck + cc))];
+    }
+    for (int q = 0; q < nq; q++) {
+      const double *r = qref + 4 * q;
+      double uh[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const double *t = tab + ((int64_t)q * NB + b) * 4;
+        uh[0] += uc[b] * t[0];
+        uh[1] += uc[b] * t[1];
+        uh[2] += uc[b] * t[2];
+        uh[3] += uc[b] * t[3];
+      }
+      double ex[4];
+      example_exact_grad(prm, (ci + 0.5 * (r[0] + 1.0)) * h, (cj + 0.5 * (r[1] + 1.0)) * h,
+                         (ck + 0.5 * (r[2] + 1.0)) * h, ex);
+      const double w = r[3] * adet;
+      const double d0 = uh[0] - ex[0], d1 = g * uh[1] - ex[1], d2 = g * uh[2] - ex[2], d3 = g * uh[3] - ex[3];
+      e0 += w * d0 * d0;
+      e1 += w * (d1 * d1 + d2 * d2 + d3 * d3);
+    }
+  }
+  e0 = block_sum<double>(e0);
+  e1 = block_sum<double>(e1);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = e0;
+    part[kRedBlocks + blockIdx.x] = e1;
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_reduce_errors(const double *__restrict__ part, int np,
+                                                          double *__restrict__ out) {
+  double s = 0, t = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+    s += part[i];
+    t += part[kRedBlocks + i];
+  }
+  s = block_sum<double>(s);
+  t = block_sum<double>(t);
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(s);
+    out[1] = sqrt(t);
+  }
+}
+void fe_errors(const DevLayout &D, const double *prm_dev, const double *qref, const double *tab, int nq,
+               const double *u, double *part, double *out, hipStream_t st) {
+  const int64_t ncells = (int64_t)D.n * D.n * D.n;
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((ncells + kBlock - 1) / kBlock, kRedBlocks));
+  if (D.order == 2)
+    hipLaunchKernelGGL(k_errors<2>, dim3(g), dim3(kBlock), 0, st, prm_dev, D.n, D.M, qref, tab, nq, u, part);
+  else
+    hipLaunchKernelGGL(k_errors<1>, dim3(g), dim3(kBlock), 0, st, prm_dev, D.n, D.M, qref, tab, nq, u, part);
+  hipLaunchKernelGGL(k_reduce_errors, dim3(1), dim3(kBlock), 0, st, part, g, out);
+}
+
 // ============================================================ time stepping
 // nodal interpolation of the example's exact (initial) function on the lattice
 // (TFEFunction3D::Interpolate for Lagrange nodes)

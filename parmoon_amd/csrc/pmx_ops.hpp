@@ -45,6 +45,10 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st);
+// out = {||u - u_h||_L2, |u - u_h|_H1} of the lattice function u (FP64) against
+// the example's exact solution with the rule of qref/tab (part: 2 kRedBlocks)
+void fe_errors(const DevLayout &D, const double *prm_dev, const double *qref, const double *tab, int nq,
+               const double *u, double *part, double *out, hipStream_t st);
 // a == b bit for bit (n doubles; flag: one device int of scratch; synchronises st)
 bool bits_equal(const double *a, const double *b, int64_t n, int *flag, hipStream_t st);
 // whole SOR (ssor 0) / SSOR (ssor 1) sweep with Dirichlet rows x = f, one

@@ -285,3 +285,32 @@ def test_coarse_inverse_follows_the_operator(ctx):
     assert not np.array_equal(xa, xb)
     S.close()
     F.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_device_error_norms(ctx, order):
+    """L2 and H1-semi errors computed on the device (pmx_system_errors) agree
+    with the oracle's quadrature of the same solution to 1e-11, for both rules,
+    and fall at the element's rates on the Laplace example (u = sin sin sin):
+    L2 O(h^{k+1}), H1 O(h^k)."""
+    print(f"[case] device error norms order={order}", flush=True)
+    errs = []
+    for nref in (3, 4):
+        for linear in (True, False):
+            prm = P.base(P.EX_LAPLACE, eps=1.0, linear=linear)
+            S = pa.System(ctx, nref - 2, 3, order)
+            S.assemble(prm)
+            opt = P.solver_opt(solver=16, restart=20, maxit=100, res_norm_min=0.0, red_factor=0.0,
+                               rel_target=1e-12, smoother=3, omega=1.0)
+            info, rc = S.solve_resident(opt)
+            assert rc == 0, info
+            O = orc.System(nref - 2, 3, order)
+            e_or = O.levels[-1].errors(prm, S.solution())
+            e_dev = S.errors(prm)
+            assert abs(e_dev[0] - e_or[0]) <= 1e-11 * e_or[0], (e_dev, e_or)
+            assert abs(e_dev[1] - e_or[1]) <= 1e-11 * e_or[1], (e_dev, e_or)
+            if linear:
+                errs.append(e_dev)
+            S.close()
+    r0, r1 = errs[0][0] / errs[1][0], errs[0][1] / errs[1][1]
+    assert abs(np.log2(r0) - (order + 1)) < 0.25 and abs(np.log2(r1) - order) < 0.25, (r0, r1)

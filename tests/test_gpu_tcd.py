@@ -6,7 +6,8 @@ tau = 0.1082532 to T = 2.05681, 4 GMG levels: every L2 error of
 ReadIn/tcd3d_GMG.out to the 7 printed digits, the accumulated L2(0,T;L2)
 within 1.5e-9, and the final solution within 1e-9 (max norm, relative) of the
 oracle's loop.  Errors of the GPU solutions are evaluated by the oracle's
-quadrature on the host.
+quadrature on the host, and the device's own error norms (L2 and H1 semi)
+agree with those to 1e-11.
 """
 import json
 import os
@@ -51,7 +52,11 @@ def test_golden_tcd3d_log_gpu(ctx, precision):
         p[P.TIME] = t
         info, rc = S.tcd_step(p, td, opt)
         assert rc == 0, info
-        e = fine.errors(p, S.solution())[0]
+        e_or = fine.errors(p, S.solution())
+        e = e_or[0]
+        # the device error norms (pmx_system_errors) agree with the oracle's quadrature
+        e_dev = S.errors(p)
+        assert abs(e_dev[0] - e_or[0]) <= 1e-11 * e_or[0] and abs(e_dev[1] - e_or[1]) <= 1e-11 * e_or[1]
         if m > 0:
             acc += (e * e + e_old * e_old) * tau / 2.0
         e_old = e
