@@ -25,6 +25,12 @@ FLAGS = [
 ]
 
 
+# per-source extras: the FP64 MFMA of the cell-matrix assembly keeps its
+# loop-carried accumulators in VGPRs (without this the compiler copies them
+# to and from AGPRs around every MFMA)
+EXTRA = {"pmx_ops.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def build(verbose=True):
     objs = []
     bdir = os.path.join(HERE, "_build")
@@ -36,7 +42,7 @@ def build(verbose=True):
             continue
         obj = os.path.join(bdir, s + ".o")
         lang = ["-x", "hip"] if s.endswith(".hip") else []
-        cmd = ["hipcc", *FLAGS, *lang, "-c", src, "-o", obj]
+        cmd = ["hipcc", *FLAGS, *EXTRA.get(s, []), *lang, "-c", src, "-o", obj]
         procs.append((s, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
     failed = False

@@ -103,6 +103,9 @@ struct System {
   int nq = 0;
   int tab_key = -1;                   // rule of the uploaded tables
   int example = 0;                    // prm[PMX_PRM_EXAMPLE] of the last assembly
+  int form = 0;                       // prm[PMX_PRM_FORM] of the last assembly
+  bool mfma_assembly = true;          // Q2 one-matrix forms through the cell buffer
+  DevBuf<double> kc;                  // Q2 cell matrices (finest level's cells x 27 x 28)
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
   DevBuf<double> oldrhs, oldsol, tmp1, tmp2;  // time stepping (finest level)
@@ -295,6 +298,14 @@ static void ensure_tables(System &Y, bool linear) {
   Y.nq = nq;
   Y.qpbuf.alloc((size_t)Y.lev.back()->Nb.ncells * nq * 8);
   Y.tab_key = key;
+}
+
+// the cell buffer of the MFMA assembly (Q2 only; sized for the finest level)
+static double *cell_buffer(System &Y) {
+  if (Y.order != 2 || !Y.mfma_assembly) return nullptr;
+  const int64_t n = Y.lev.back()->L.n;
+  Y.kc.alloc((size_t)(n * n * n * kCellMatrixDoubles));
+  return Y.kc.p;
 }
 
 static void coarse_inverse(System &Y, Level &lv) {
@@ -1173,14 +1184,15 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
     if (form < PMX_FORM_GALERKIN || form > PMX_FORM_TCD_SUPG_AK) throw std::invalid_argument("unknown form");
     Y.prm.upload(prm, PMX_PRM_LEN, st);
     Y.example = (int)prm[PMX_PRM_EXAMPLE];
+    Y.form = form;
     ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
     const int nq = Y.nq;
     Y.two_matrices = form == PMX_FORM_TCD_SUPG_AK;
     for (auto &lp : Y.lev) {
       Level &lv = *lp;
       if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
-      assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr,
-               lv.rhs.p, st);
+      assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
+               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
       lv.have_val32 = false;
       lv.have_dinv = false;
     }
@@ -1198,12 +1210,14 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
   hipStream_t st = Y.ctx->st;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
   Y.example = (int)prm[PMX_PRM_EXAMPLE];
+  Y.form = (int)prm[PMX_PRM_FORM];
   ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
   const int nq = Y.nq;
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (mass) lv.mass.alloc(lv.L.nvals);
-    assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
+    assemble(lv.D, Y.prm.p, Y.example, (int)prm[PMX_PRM_FORM], Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
+             mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
   }
 }
 
@@ -1444,7 +1458,7 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 1: css_apply<double>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
           case 2: level_smooth<double>(Y, lv, 1, 0.8, 1); break;
           case 3: level_smooth<double>(Y, lv, 3, 1.0, 1); break;
-          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
+          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
           case 5: if (li > 0) prolongate<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.f.p, v.d.p, st); break;
           case 7: mg_prec<double>(Y, v.aux.p, v.d.p); break;

@@ -826,15 +826,185 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
                      kcol, dof2lat, D.N, active, D.M, D.S, D.segidx, D.segs, val, out);
 }
 
-void assemble(const DevLayout &D, const double *prm_dev, int example, const double *qref, int nq, const double *tab,
-              double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st) {
-  const int64_t ncq = (int64_t)D.n * D.n * D.n * nq;
+// ---- Q2 one-matrix forms on MFMA: local matrices per cell, then a row gather.
+// With Phi_q the 4 x 27 table of (phi, dphi/dxi, dphi/deta, dphi/dzeta) at
+// point q, every form above is K = sum_q Phi_q^T D_q Phi_q (rows: test
+// functions a, columns: trial functions b) with, for the convection-diffusion
+// forms (g = 2/h folds the reference derivatives into D, s = delta):
+//   D[0][0] = w c          D[0][d] = w g b_d
+//   D[e][0] = w g c s b_e  D[e][d] = w g^2 (eps [e == d] + s b_e b_d)
+// and the right-hand side f_a = sum_q r_q . Phi_q[:, a] with r = w f (1, g s b);
+// the mass form has D[0][0] = w and r = (w f, 0, 0, 0).  One wavefront
+// computes a cell as a 32 x 32 product (27 x 27 padded, column 27 carrying the
+// right-hand side) over K = 4 nq, one v_mfma_f64_16x16x4f64 per 16 x 16 tile
+// and quadrature point: lane l holds A = Phi_q[l/16][16R + l%16] and
+// B = (D_q Phi_q)[l/16][16C + l%16]; the result lane l, register r is entry
+// (16R + l/16 + 4r, 16C + l%16).  The tables sit in LDS ([q][comp][32]).
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+constexpr int kKcRow = 28;  // doubles per local row in the cell buffer: 27 entries + rhs
+template <bool INL, bool MASS>
+__global__ __launch_bounds__(256) void k_cell_mfma(int n, int nq, int form, const double *__restrict__ prm,
+                                                  const double *__restrict__ qref, const double *__restrict__ qp,
+                                                  const double *__restrict__ tab, double *__restrict__ kc) {
+  extern __shared__ __align__(16) unsigned char cell_lds[];
+  double *lt = reinterpret_cast<double *>(cell_lds);  // [nq][4][32]
+  const int tid = threadIdx.x;
+  for (int i = tid; i < nq * 128; i += 256) {
+    const int q = i >> 7, comp = (i >> 5) & 3, b = i & 31;
+    lt[i] = b < 27 ? tab[((int64_t)q * 27 + b) * 4 + comp] : 0.0;
+  }
+  __syncthreads();
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int l = tid & 63, comp = l >> 4, col = l & 15;
+  const double g = 2.0 * n;
+  // per-lane selectors of row `comp` of D (branch-free loop body keeps the
+  // accumulators in place across quadrature points)
+  const bool r0 = comp == 0;
+  const double s1 = comp == 1 ? 1.0 : 0.0, s2 = comp == 2 ? 1.0 : 0.0, s3 = comp == 3 ? 1.0 : 0.0;
+  const bool rhs_col = col == 11;  // column 27 = 16 + 11 of the second column tile
+  const int64_t ncells = (int64_t)n * n * n;
+  for (int64_t cell = (int64_t)blockIdx.x * 4 + wave; cell < ncells; cell += (int64_t)gridDim.x * 4) {
+    const int cx = (int)(cell % n), cy = (int)((cell / n) % n), cz = (int)(cell / ((int64_t)n * n));
+    doublex4 a00 = {0.0, 0.0, 0.0, 0.0}, a01 = a00, a10 = a00, a11 = a00;
+    for (int q = 0; q < nq; q++) {
+      double Qq[8];
+      if (INL) {
+        qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) Qq[e] = qp[(cell * nq + q) * 8 + e];
+      }
+      const double w = Qq[0], eps = Qq[1], cc = Qq[5], ff = Qq[6], dl = Qq[7];
+      const double b0 = Qq[2], b1 = Qq[3], b2 = Qq[4];
+      double d0, d1, d2, d3, rr;
+      if (MASS) {
+        d0 = r0 ? w : 0.0;
+        d1 = d2 = d3 = 0.0;
+        rr = r0 ? w * ff : 0.0;
+      } else {
+        const double be = s1 * b0 + s2 * b1 + s3 * b2;  // b_comp (0 for comp 0)
+        const double wg = w * g, wgg = wg * g;
+        d0 = r0 ? w * cc : wg * cc * dl * be;
+        d1 = r0 ? wg * b0 : wgg * (dl * be * b0 + s1 * eps);
+        d2 = r0 ? wg * b1 : wgg * (dl * be * b1 + s2 * eps);
+        d3 = r0 ? wg * b2 : wgg * (dl * be * b2 + s3 * eps);
+        rr = r0 ? w * ff : w * ff * dl * be * g;
+      }
+      const double *T = lt + q * 128;
+      const double bv0 = d0 * T[col] + d1 * T[32 + col] + d2 * T[64 + col] + d3 * T[96 + col];
+      double bv1 = d0 * T[16 + col] + d1 * T[48 + col] + d2 * T[80 + col] + d3 * T[112 + col];
+      bv1 = rhs_col ? rr : bv1;
+      const double av0 = T[comp * 32 + col], av1 = T[comp * 32 + 16 + col];
+      a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, a00, 0, 0, 0);
+      a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, a01, 0, 0, 0);
+      a10 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, a10, 0, 0, 0);
+      a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, a11, 0, 0, 0);
+    }
+    double *K = kc + cell * (27 * kKcRow);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int a0 = comp + 4 * r, a1 = 16 + comp + 4 * r, b0 = col, b1 = 16 + col;
+      K[a0 * kKcRow + b0] = a00[r];
+      if (b1 < kKcRow) K[a0 * kKcRow + b1] = a01[r];
+      if (a1 < 27) {
+        K[a1 * kKcRow + b0] = a10[r];
+        if (b1 < kKcRow) K[a1 * kKcRow + b1] = a11[r];
+      }
+    }
+  }
+}
+// Row gather from the cell buffer: the thread of active node p adds row a of
+// each incident cell's local matrix into its CSS row and sums the rhs column.
+template <int EX, int EY, int EZ>
+__device__ __forceinline__ void assemble_row_kc(const DevSegment &s, int idx, int M, int n,
+                                                const double *__restrict__ kc, double *__restrict__ val,
+                                                double *__restrict__ rhs) {
+  constexpr int NCX = EX == 1 ? 1 : 2, NCY = EY == 1 ? 1 : 2, NCZ = EZ == 1 ? 1 : 2;
+  const int i = idx % s.cx, t = idx / s.cx;
+  const int j = t % s.cy, k = t / s.cy;
+  const int px = s.x0 + s.stride * i, py = s.y0 + s.stride * j, pz = s.z0 + s.stride * k;
+  double *v = val + vbase(s, idx);
+  double frhs = 0;
+  const int cx0 = (px - 1) / 2, cy0 = (py - 1) / 2, cz0 = (pz - 1) / 2;
+  for (int iz = 0; iz < NCZ; iz++)
+    for (int iy = 0; iy < NCY; iy++)
+      for (int ix = 0; ix < NCX; ix++) {
+        const int cx = cx0 + ix, cy = cy0 + iy, cz = cz0 + iz;
+        const int a = (px - 2 * cx) + 3 * ((py - 2 * cy) + 3 * (pz - 2 * cz));
+        const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        const double2 *Kr = reinterpret_cast<const double2 *>(kc + (cell * 27 + a) * kKcRow);
+        double row[kKcRow];
+#pragma unroll
+        for (int e = 0; e < kKcRow / 2; e++) {
+          const double2 u = Kr[e];
+          row[2 * e] = u.x;
+          row[2 * e + 1] = u.y;
+        }
+#pragma unroll
+        for (int b = 0; b < 27; b++) {
+          const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
+          const int dx = 2 * cx + bx - px, dy = 2 * cy + by - py, dz = 2 * cz + bz - pz;
+          const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
+          v[slot * kSlice] += row[b];
+        }
+        frhs += row[27];
+      }
+  rhs[px + M * (py + M * pz)] = frhs;
+}
+__global__ __launch_bounds__(kRowsPerBlock) void k_assemble_kc(const BlockInfo *__restrict__ blocks,
+                                                               const DevSegment *__restrict__ segs, int M, int n,
+                                                               const double *__restrict__ kc,
+                                                               double *__restrict__ val, double *__restrict__ rhs) {
+  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+#define PMX_KROW(EX, EY, EZ) assemble_row_kc<EX, EY, EZ>(s, idx, M, n, kc, val, rhs)
+  switch (s.ex * 100 + s.ey * 10 + s.ez) {
+    case 111: PMX_KROW(1, 1, 1); break;
+    case 211: PMX_KROW(2, 1, 1); break;
+    case 121: PMX_KROW(1, 2, 1); break;
+    case 221: PMX_KROW(2, 2, 1); break;
+    case 112: PMX_KROW(1, 1, 2); break;
+    case 212: PMX_KROW(2, 1, 2); break;
+    case 122: PMX_KROW(1, 2, 2); break;
+    default: PMX_KROW(2, 2, 2); break;
+  }
+#undef PMX_KROW
+}
+
+void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
+              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st) {
+  const int64_t ncells = (int64_t)D.n * D.n * D.n;
+  const int64_t ncq = ncells * nq;
   const bool inl = coeffs_inline(example);
   if (!inl)
     hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
                        nq, D.n, qpbuf);
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
+  if (kc && D.order == 2 && !val2) {
+    // local matrices on MFMA, then the row gather from the cell buffer
+    const unsigned grid = (unsigned)std::min<int64_t>((ncells + 3) / 4, 8192);
+    const size_t lds = (size_t)nq * 128 * sizeof(double);
+#define PMX_CELL(IN, MS)                                                                                   \
+  hipLaunchKernelGGL((k_cell_mfma<IN, MS>), dim3(grid), dim3(256), lds, st, D.n, nq, form, prm_dev, qref, qpbuf, tab, \
+                     kc)
+    const bool ms = form == PMX_FORM_MASS;
+    if (inl) {
+      if (ms) PMX_CELL(true, true);
+      else PMX_CELL(true, false);
+    } else {
+      if (ms) PMX_CELL(false, true);
+      else PMX_CELL(false, false);
+    }
+#undef PMX_CELL
+    hipLaunchKernelGGL(k_assemble_kc, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, D.n,
+                       kc, val, rhs);
+    hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
+                       D.nboundary, D.M, prm_dev, rhs);
+    return;
+  }
 #define PMX_ASM(O, TW, IN)                                                                                   \
   hipLaunchKernelGGL((k_assemble<O, TW, IN>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,      \
                      D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, val, val2, rhs)

@@ -86,9 +86,12 @@ template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
                 const T *val, double *out, hipStream_t st);
 // example: prm[PMX_PRM_EXAMPLE] (selects evaluating the coefficients inline or
-// through the quadrature-point buffer)
-void assemble(const DevLayout &D, const double *prm_dev, int example, const double *qref, int nq, const double *tab,
-              double *qpbuf, double *val, double *val2, double *rhs, hipStream_t st);
+// through the quadrature-point buffer); form: prm[PMX_PRM_FORM]; kc: cell
+// buffer of n^3 x 27 x 28 doubles for the Q2 one-matrix forms on MFMA
+// (nullptr: the per-row evaluation)
+void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
+              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st);
+constexpr int64_t kCellMatrixDoubles = 27 * 28;  // cell buffer entries per Q2 cell
 // time stepping (TSystemTCD3D): nodal interpolation of the example's initial
 // function, val = mass + c val, and the Crank-Nicolson / theta right-hand side
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st);

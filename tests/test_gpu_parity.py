@@ -35,6 +35,10 @@ CASES = [
     ("curl-Q2-sdfem-27pt", 2, 2, lambda: P.config2(nref=2)),
     ("curl-Q2-sdfem-64pt", 2, 2, lambda: P.config2(nref=2, linear=False)),
     ("config1-Q2-galerkin", 2, 2, lambda: P.config1(form=P.FORM_GALERKIN)),
+    ("config1-Q2-sdfem-inline", 2, 2, lambda: P.config1()),
+    ("heat-Q2-mass", 2, 2, lambda: P.heat_amc(0.3, form=P.FORM_MASS)),
+    ("heat-Q2-tcdA", 2, 2, lambda: P.heat_amc(0.3)),
+    ("rotating-Q2-supgAK", 2, 2, lambda: dict_supg(P.rotating())),
     ("lognormal-Q1-sdfem", 1, 3, lambda: P.lognormal(nref=3, peclet=10.0)),
     ("rotating-Q1-supgAK", 1, 3, lambda: dict_supg(P.rotating())),
 ]
