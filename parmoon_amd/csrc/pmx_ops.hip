@@ -841,15 +841,17 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
 // B = (D_q Phi_q)[l/16][16C + l%16]; the result lane l, register r is entry
 // (16R + l/16 + 4r, 16C + l%16).  The tables sit in LDS ([q][comp][32]).
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+// 8 wavefronts share one copy of the tables (64 KB for the 64-point rule)
+constexpr int kCellWaves = 8, kCellThreads = 64 * kCellWaves;
 constexpr int kKcRow = 28;  // doubles per local row in the cell buffer: 27 entries + rhs
 template <bool INL, bool MASS>
-__global__ __launch_bounds__(256) void k_cell_mfma(int n, int nq, int form, const double *__restrict__ prm,
+__global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int form, const double *__restrict__ prm,
                                                   const double *__restrict__ qref, const double *__restrict__ qp,
                                                   const double *__restrict__ tab, double *__restrict__ kc) {
   extern __shared__ __align__(16) unsigned char cell_lds[];
   double *lt = reinterpret_cast<double *>(cell_lds);  // [nq][4][32]
   const int tid = threadIdx.x;
-  for (int i = tid; i < nq * 128; i += 256) {
+  for (int i = tid; i < nq * 128; i += kCellThreads) {
     const int q = i >> 7, comp = (i >> 5) & 3, b = i & 31;
     lt[i] = b < 27 ? tab[((int64_t)q * 27 + b) * 4 + comp] : 0.0;
   }
@@ -863,32 +865,47 @@ __global__ __launch_bounds__(256) void k_cell_mfma(int n, int nq, int form, cons
   const double s1 = comp == 1 ? 1.0 : 0.0, s2 = comp == 2 ? 1.0 : 0.0, s3 = comp == 3 ? 1.0 : 0.0;
   const bool rhs_col = col == 11;  // column 27 = 16 + 11 of the second column tile
   const int64_t ncells = (int64_t)n * n * n;
-  for (int64_t cell = (int64_t)blockIdx.x * 4 + wave; cell < ncells; cell += (int64_t)gridDim.x * 4) {
+  for (int64_t cell = (int64_t)blockIdx.x * kCellWaves + wave; cell < ncells;
+       cell += (int64_t)gridDim.x * kCellWaves) {
     const int cx = (int)(cell % n), cy = (int)((cell / n) % n), cz = (int)(cell / ((int64_t)n * n));
     doublex4 a00 = {0.0, 0.0, 0.0, 0.0}, a01 = a00, a10 = a00, a11 = a00;
+    // coefficients of the next quadrature point are loaded one step ahead
+    double Qn[8];
+    if (!INL) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) Qn[e] = qp[cell * nq * 8 + e];
+    }
     for (int q = 0; q < nq; q++) {
       double Qq[8];
       if (INL) {
         qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq);
       } else {
 #pragma unroll
-        for (int e = 0; e < 8; e++) Qq[e] = qp[(cell * nq + q) * 8 + e];
+        for (int e = 0; e < 8; e++) Qq[e] = Qn[e];
+        const int qn = q + 1 < nq ? q + 1 : q;
+#pragma unroll
+        for (int e = 0; e < 8; e++) Qn[e] = qp[(cell * nq + qn) * 8 + e];
       }
       const double w = Qq[0], eps = Qq[1], cc = Qq[5], ff = Qq[6], dl = Qq[7];
       const double b0 = Qq[2], b1 = Qq[3], b2 = Qq[4];
+      // row comp of D = s v + t e_comp with v = (c, g b): s = w for row 0 and
+      // w g delta b_comp below it, t = w g^2 eps on the diagonal of rows 1-3;
+      // the right-hand-side weight is f s
       double d0, d1, d2, d3, rr;
       if (MASS) {
         d0 = r0 ? w : 0.0;
         d1 = d2 = d3 = 0.0;
-        rr = r0 ? w * ff : 0.0;
+        rr = ff * d0;
       } else {
         const double be = s1 * b0 + s2 * b1 + s3 * b2;  // b_comp (0 for comp 0)
-        const double wg = w * g, wgg = wg * g;
-        d0 = r0 ? w * cc : wg * cc * dl * be;
-        d1 = r0 ? wg * b0 : wgg * (dl * be * b0 + s1 * eps);
-        d2 = r0 ? wg * b1 : wgg * (dl * be * b1 + s2 * eps);
-        d3 = r0 ? wg * b2 : wgg * (dl * be * b2 + s3 * eps);
-        rr = r0 ? w * ff : w * ff * dl * be * g;
+        const double sc = r0 ? w : w * g * dl * be;
+        const double t = r0 ? 0.0 : w * g * g * eps;
+        const double gb0 = g * b0, gb1 = g * b1, gb2 = g * b2;
+        d0 = sc * cc;
+        d1 = sc * gb0 + s1 * t;
+        d2 = sc * gb1 + s2 * t;
+        d3 = sc * gb2 + s3 * t;
+        rr = ff * sc;
       }
       const double *T = lt + q * 128;
       const double bv0 = d0 * T[col] + d1 * T[32 + col] + d2 * T[64 + col] + d3 * T[96 + col];
@@ -985,11 +1002,11 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   if (kc && D.order == 2 && !val2) {
     // local matrices on MFMA, then the row gather from the cell buffer
-    const unsigned grid = (unsigned)std::min<int64_t>((ncells + 3) / 4, 8192);
+    const unsigned grid = (unsigned)std::min<int64_t>((ncells + kCellWaves - 1) / kCellWaves, 4096);
     const size_t lds = (size_t)nq * 128 * sizeof(double);
 #define PMX_CELL(IN, MS)                                                                                   \
-  hipLaunchKernelGGL((k_cell_mfma<IN, MS>), dim3(grid), dim3(256), lds, st, D.n, nq, form, prm_dev, qref, qpbuf, tab, \
-                     kc)
+  hipLaunchKernelGGL((k_cell_mfma<IN, MS>), dim3(grid), dim3(kCellThreads), lds, st, D.n, nq, form, prm_dev, qref,   \
+                     qpbuf, tab, kc)
     const bool ms = form == PMX_FORM_MASS;
     if (inl) {
       if (ms) PMX_CELL(true, true);
