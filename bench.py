@@ -377,6 +377,7 @@ def main():
     achieved, ms_spmv, traffic_alg, bytes_fmt, bw_fmt = roof[1]
     t64 = res.get("fp64", {}).get("t_solve")
     flops_asm = 180792.0 * (2 ** nref) ** 3  # Q2, 27-point rule, B^T D B model per cell
+    ssor_alg = 2.0 * (traffic_alg + 2 * N * 4)  # FP32: 2 x (SpMV + f + x_old)
     out = {
         "metric": "time-to-1e-10 FP64 residual (s), learned+FP32 vs plain FP64 GMG; assembly DOF/s",
         "value": round(t_main, 5),
@@ -415,6 +416,14 @@ def main():
                      "algorithmic_bytes": int(traffic_alg),
                      "format_bytes": int(bytes_fmt), "format_GBs": round(bw_fmt, 1),
                      "format_frac": round(bw_fmt / HBM_PEAK_GBS, 4)},
+        # the FP32 SSOR sweep of the finest level (the largest share of the
+        # solve): algorithmic bytes of a multicolour SSOR sweep = 2 x those of a
+        # damped-Jacobi sweep (SpMV + f + x_old)
+        "roofline_ssor_sweep": {"bound": "hbm", "kernel": "k_rows<float,3> x 54 colour launches (finest level)",
+                                "achieved": round(ssor_alg / (kern["ssor_sweep_fp32_ms"] * 1e-3) / 1e9, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(ssor_alg / (kern["ssor_sweep_fp32_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "algorithmic_bytes": int(ssor_alg)},
     }
     if not args.no_cpu:
         try:

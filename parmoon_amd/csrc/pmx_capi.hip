@@ -375,6 +375,10 @@ static void ensure_learned(System &Y) {
       lv.dinv.alloc(lv.L.N);
       lv.lc.alloc(lv.L.N);
       lv.lac.alloc(lv.L.N);
+      // A c on the Dirichlet rows is c there, which the network's output
+      // mask makes zero; the row kernels write interior rows only, so these
+      // entries are set once here
+      fill<float>(lv.lac.p, 0.0f, lv.L.N, Y.ctx->st);
       lv.feat.alloc((size_t)lv.L.N * kNetIn);
     }
     if (!lv.have_dinv) {
@@ -431,8 +435,7 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   learned_build_input(net, lv.lc.p, sc, sq, lv.feat.p, N, st);
   net_forward_act(net, lv.L.M, false, st);
   learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
-  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
-  boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
+  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   learned_dots(v.d.p, lv.lac.p, N, part, sc + 1, st);
   learned_step(v.x.p, lv.lc.p, sc + 1, N, st);
 }
@@ -767,8 +770,7 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
   learned_build_input(net, u, nullptr, 1.0, lv.feat.p, lv.L.N, st);
   net_forward_act(net, M, true, st);
   learned_mask(net.out, M, lv.lc.p, st);
-  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);
-  boundary<float>(lv.D, 0, lv.lac.p, lv.lc.p, nullptr, st);
+  css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   learned_res_loss(net, M, u, lv.dinv.p, lv.lac.p, v.aux.p, st);
   css_transpose(lv.D, v.val.p, v.aux.p, v.d.p, st);
   net_backward_from(net, M, v.d.p, st);
