@@ -9,7 +9,7 @@ residual 1e-10 with V(2,2) multicolour-SSOR GMG preconditioning FGMRES(30):
   * fp32    : FP32 operator/MG/Krylov vectors inside an FP64 defect
               correction (FP64 residual, Hessenberg/Givens, coarse solve);
   * learned : fp32 with learned smoothing on the levels below the finest
-              (two network steps replace the pre- and the post-smoothing
+              (three network steps replace the pre- and the post-smoothing
               there; the finest level keeps SSOR).  The networks are trained
               once before the timed steps; training time is reported
               separately, not part of the step.
@@ -239,7 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="learned,fp32,fp64")
     # learned-form defaults: the steady Q2 workload smooths the levels below
-    # the finest with two network steps in place of pre- and post-smoothing;
+    # the finest with three network steps in place of pre- and post-smoothing;
     # the transient Q1 workload (8-colour SSOR, cheap on coarse levels) uses
     # one network step as the finest level's post-smoother
     ap.add_argument("--learned-levels", type=int, default=None,
@@ -247,7 +247,8 @@ def main():
     ap.add_argument("--learned-skip", type=int, default=None, help="the finest of those levels keep SSOR")
     ap.add_argument("--learned-steps", type=int, default=None, help="network steps per smoothing")
     ap.add_argument("--learned-pre", type=int, default=None, help="1: the network also replaces pre-smoothing")
-    ap.add_argument("--train-steps", type=int, default=1200)
+    ap.add_argument("--train-steps", type=int, default=None,
+                    help="Adam steps per level (default: 4800 steady, 1200 transient, 16 per field for --train-set)")
     ap.add_argument("--profile-kernels", action="store_true")
     ap.add_argument("--transient", type=int, default=0, metavar="STEPS",
                     help="run the TCD3D 128^3 Q1 transient workload for STEPS time steps instead")
@@ -255,8 +256,12 @@ def main():
                     help="train the learned smoother across FIELDS seeded 64^3 Q1 fields instead")
     ap.add_argument("--eval-seeds", type=int, default=4)
     args = ap.parse_args()
-    dflt = dict(learned_levels=1, learned_skip=0, learned_steps=1, learned_pre=0) if args.transient > 0 else \
-        dict(learned_levels=0, learned_skip=1, learned_steps=2, learned_pre=1)
+    if args.transient > 0:
+        dflt = dict(learned_levels=1, learned_skip=0, learned_steps=1, learned_pre=0, train_steps=1200)
+    elif args.train_set > 0:
+        dflt = dict(learned_levels=0, learned_skip=1, learned_steps=2, learned_pre=1, train_steps=16)
+    else:
+        dflt = dict(learned_levels=0, learned_skip=1, learned_steps=3, learned_pre=1, train_steps=4800)
     for k, v in dflt.items():
         if getattr(args, k) is None:
             setattr(args, k, v)

@@ -863,8 +863,13 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo
     z = sj.z0 + S * (t / sj.cy);
   }
   float acc = 0.f;
+  // unrolled, so that the uniform segment lookups of all offsets can be
+  // issued ahead instead of one dependent scalar load pair per offset
+#pragma unroll
   for (int dz = -E; dz <= E; dz++)
+#pragma unroll
     for (int dy = -E; dy <= E; dy++)
+#pragma unroll
       for (int dx = -E; dx <= E; dx++) {
         const int si = segidx[((rx + dx + S) % S) + S * (((ry + dy + S) % S) + S * ((rz + dz + S) % S))];
         if (si < 0) continue;
