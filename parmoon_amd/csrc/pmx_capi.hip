@@ -358,6 +358,9 @@ static void ensure_val32(System &Y) {
       Y.invalidate_graphs();
       lv.v32.val.alloc(lv.L.nvals);
       for (auto *b : {&lv.v32.x, &lv.v32.f, &lv.v32.d, &lv.v32.aux}) b->alloc(lv.L.N);
+      // the defect's Dirichlet entries are read by the learned step with zero
+      // weights only (dinv and A c vanish there), so they just have to be finite
+      fill<float>(lv.v32.d.p, 0.0f, lv.L.N, Y.ctx->st);
     }
     convert<float, double>(lv.v32.val.p, lv.v64.val.p, lv.L.nvals, Y.ctx->st);
     lv.have_val32 = true;
@@ -429,15 +432,14 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   const int64_t N = lv.L.N;
   double *sc = Y.lscal.p, *part = Y.ctx->part.p;
   const double sq = std::sqrt((double)lv.L.nactive);
+  // defect of the interior rows; its Dirichlet entries meet zero weights
+  // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
   css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
-  boundary<float>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
-  learned_input_norm(lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, st);
-  learned_build_input(net, lv.lc.p, sc, sq, lv.feat.p, N, st);
+  learned_scaled_input(net, lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
   net_forward_act(net, lv.L.M, false, st);
   learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
-  learned_dots(v.d.p, lv.lac.p, N, part, sc + 1, st);
-  learned_step(v.x.p, lv.lc.p, sc + 1, N, st);
+  learned_update(v.d.p, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
 }
 
 // level i smooths with the network in this cycle

@@ -742,6 +742,52 @@ __global__ void k_interior_scale(const float *__restrict__ in, int M, const doub
     c[i] = interior ? in[i] * sc : 0.f;
   }
 }
+// act0 = feat with channel 0 = bf16(u sq / ||u||), ||u|| reduced by every
+// workgroup from the partials of k_learned_input_sq in k_reduce_norm's order
+// (the same value everywhere); workgroup 0 stores it for the output scaling
+__global__ __launch_bounds__(kBlock) void k_build_input_parts(const float *__restrict__ u,
+                                                              const double *__restrict__ part, int np, double sq,
+                                                              const uint16_t *__restrict__ feat, int64_t n,
+                                                              uint16_t *__restrict__ act0, double *__restrict__ nrm) {
+  __shared__ double sh_norm;
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) {
+    sh_norm = sqrt(s);
+    if (blockIdx.x == 0) nrm[0] = sh_norm;
+  }
+  __syncthreads();
+  const double nv = sh_norm;
+  const float sc = nv > 0 ? (float)(sq / nv) : 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int piece = 0; piece < kNetIn / 8; piece++) {
+      u16x8 v = *reinterpret_cast<const u16x8 *>(feat + i * kNetIn + 8 * piece);
+      if (piece == 0) v[0] = f2bf(u[i] * sc);
+      *reinterpret_cast<u16x8 *>(act0 + i * kNetIn + 8 * piece) = v;
+    }
+  }
+}
+// x += alpha c, alpha = max(0, (d, Ac) / (Ac, Ac)) reduced by every workgroup
+// from the partials of k_dot2_partial in k_reduce_final2's order
+__global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict__ x, const float *__restrict__ c,
+                                                               const double *__restrict__ pab,
+                                                               const double *__restrict__ pbb, int np, int64_t n) {
+  __shared__ float sh_alpha;
+  double s = 0, t = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+    s += pab[i];
+    t += pbb[i];
+  }
+  s = block_sum<double>(s);
+  t = block_sum<double>(t);
+  if (threadIdx.x == 0) sh_alpha = (float)(t > 0 ? fmax(0.0, s / t) : 0.0);
+  __syncthreads();
+  const float af = sh_alpha;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    x[i] += af * c[i];
+}
 __global__ void k_learned_step(float *__restrict__ x, const float *__restrict__ c, const double *__restrict__ p,
                                int64_t n) {
   const double a = p[1] > 0 ? fmax(0.0, p[0] / p[1]) : 0.0;
@@ -783,6 +829,20 @@ void learned_dots(const float *d, const float *ac, int64_t n, double *part, doub
   const unsigned g = red_grid(n);
   hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
   hipLaunchKernelGGL(k_reduce_final2, dim3(1), dim3(kBlock), 0, st, part, part + kRedBlocks, (int)g, out);
+}
+void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
+                          double *nrm, double sq, const uint16_t *feat, hipStream_t st) {
+  const unsigned g = red_grid(n);
+  hipLaunchKernelGGL(k_learned_input_sq, dim3(g), dim3(kBlock), 0, st, dinv, d, u, part, n);
+  hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, part, (int)g, sq, feat, n,
+                     net.act[0], nrm);
+}
+void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,
+                    hipStream_t st) {
+  const unsigned g = red_grid(n);
+  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
+  hipLaunchKernelGGL(k_learned_step_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, x, c, part, part + kRedBlocks,
+                     (int)g, n);
 }
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sq, const uint16_t *feat,
                          int64_t n, hipStream_t st) {

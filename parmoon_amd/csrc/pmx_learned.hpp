@@ -86,6 +86,14 @@ void learned_input_norm(const float *dinv, const float *d, float *u, int64_t n, 
                         hipStream_t st);
 // out[0] = (d, ac), out[1] = (ac, ac) (part: 2 kRedBlocks partials)
 void learned_dots(const float *d, const float *ac, int64_t n, double *part, double *out, hipStream_t st);
+// u = dinv d, *nrm = ||u|| and net.act[0] = feat with channel 0 =
+// bf16(u sqrt_count / ||u||), in two launches (part: kRedBlocks partials)
+void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
+                          double *nrm, double sqrt_count, const uint16_t *feat, hipStream_t st);
+// x += alpha c with alpha = max(0, (d, ac) / (ac, ac)), in two launches
+// (part: 2 kRedBlocks partials)
+void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,
+                    hipStream_t st);
 // net.act[0] = feat with channel 0 = bf16(u * sqrt_count / *nrm) (nrm: device
 // FP64 2-norm of u; nrm == nullptr: no scaling)
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
