@@ -626,7 +626,7 @@ __global__ void k_learned_input(const float *__restrict__ dinv, const float *__r
     u[i] = dinv[i] * d[i];
 }
 // u = dinv d together with the partial sums of u^2 (the grid and summation
-// order of k_dot_partial, so k_reduce_norm gives the same ||u|| as dot())
+// order of k_dot_partial)
 __global__ __launch_bounds__(kBlock) void k_learned_input_sq(const float *__restrict__ dinv,
                                                              const float *__restrict__ d, float *__restrict__ u,
                                                              double *__restrict__ part, int64_t n) {
@@ -654,30 +654,6 @@ __global__ __launch_bounds__(kBlock) void k_dot2_partial(const float *__restrict
   if (threadIdx.x == 0) {
     pab[blockIdx.x] = s;
     pbb[blockIdx.x] = t;
-  }
-}
-// out[0] = sqrt(sum(part)) (k_reduce_final's order)
-__global__ __launch_bounds__(kBlock) void k_reduce_norm(const double *__restrict__ part, int np,
-                                                        double *__restrict__ out) {
-  double s = 0;
-  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
-  s = block_sum<double>(s);
-  if (threadIdx.x == 0) out[0] = sqrt(s);
-}
-// out[0] = sum(pab), out[1] = sum(pbb) (k_reduce_final's order)
-__global__ __launch_bounds__(kBlock) void k_reduce_final2(const double *__restrict__ pab,
-                                                          const double *__restrict__ pbb, int np,
-                                                          double *__restrict__ out) {
-  double s = 0, t = 0;
-  for (int i = threadIdx.x; i < np; i += kBlock) {
-    s += pab[i];
-    t += pbb[i];
-  }
-  s = block_sum<double>(s);
-  t = block_sum<double>(t);
-  if (threadIdx.x == 0) {
-    out[0] = s;
-    out[1] = t;
   }
 }
 // act0[p] = feat[p] with channel 0 = bf16(u[p] * sq / *nrm) (no scaling if nrm is null)
@@ -743,8 +719,9 @@ __global__ void k_interior_scale(const float *__restrict__ in, int M, const doub
   }
 }
 // act0 = feat with channel 0 = bf16(u sq / ||u||), ||u|| reduced by every
-// workgroup from the partials of k_learned_input_sq in k_reduce_norm's order
-// (the same value everywhere); workgroup 0 stores it for the output scaling
+// workgroup from the partials of k_learned_input_sq in the fixed order of
+// k_reduce_final (the same value everywhere); workgroup 0 stores it for the
+// output scaling
 __global__ __launch_bounds__(kBlock) void k_build_input_parts(const float *__restrict__ u,
                                                               const double *__restrict__ part, int np, double sq,
                                                               const uint16_t *__restrict__ feat, int64_t n,
@@ -770,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_build_input_parts(const float *__res
   }
 }
 // x += alpha c, alpha = max(0, (d, Ac) / (Ac, Ac)) reduced by every workgroup
-// from the partials of k_dot2_partial in k_reduce_final2's order
+// from the partials of k_dot2_partial in the fixed order of k_reduce_final
 __global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict__ x, const float *__restrict__ c,
                                                                const double *__restrict__ pab,
                                                                const double *__restrict__ pbb, int np, int64_t n) {
@@ -786,13 +763,6 @@ __global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict
   __syncthreads();
   const float af = sh_alpha;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    x[i] += af * c[i];
-}
-__global__ void k_learned_step(float *__restrict__ x, const float *__restrict__ c, const double *__restrict__ p,
-                               int64_t n) {
-  const double a = p[1] > 0 ? fmax(0.0, p[0] / p[1]) : 0.0;
-  const float af = (float)a;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     x[i] += af * c[i];
 }
 __global__ void k_learned_random(float *__restrict__ xv, int M, uint32_t seed) {
@@ -818,17 +788,6 @@ void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipSt
 }
 static unsigned red_grid(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kRedBlocks));
-}
-void learned_input_norm(const float *dinv, const float *d, float *u, int64_t n, double *part, double *nrm,
-                        hipStream_t st) {
-  const unsigned g = red_grid(n);
-  hipLaunchKernelGGL(k_learned_input_sq, dim3(g), dim3(kBlock), 0, st, dinv, d, u, part, n);
-  hipLaunchKernelGGL(k_reduce_norm, dim3(1), dim3(kBlock), 0, st, part, (int)g, nrm);
-}
-void learned_dots(const float *d, const float *ac, int64_t n, double *part, double *out, hipStream_t st) {
-  const unsigned g = red_grid(n);
-  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
-  hipLaunchKernelGGL(k_reduce_final2, dim3(1), dim3(kBlock), 0, st, part, part + kRedBlocks, (int)g, out);
 }
 void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
                           double *nrm, double sq, const uint16_t *feat, hipStream_t st) {
@@ -856,9 +815,6 @@ void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipS
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sq, float *c, hipStream_t st) {
   hipLaunchKernelGGL(k_interior_scale<0>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, net.out, M, nrm, sq,
                      1.0f, c);
-}
-void learned_step(float *x, const float *c, const double *p, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_learned_step, dim3(grid_for(n)), dim3(256), 0, st, x, c, p, n);
 }
 void learned_random(float *x, int M, uint32_t seed, hipStream_t st) {
   hipLaunchKernelGGL(k_learned_random, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, seed);

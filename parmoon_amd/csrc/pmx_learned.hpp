@@ -81,11 +81,6 @@ struct DevLayout;
 void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st);
 // u = dinv * d (lattice), the unscaled network input
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st);
-// u = dinv d and *nrm = ||u|| (part: kRedBlocks partials)
-void learned_input_norm(const float *dinv, const float *d, float *u, int64_t n, double *part, double *nrm,
-                        hipStream_t st);
-// out[0] = (d, ac), out[1] = (ac, ac) (part: 2 kRedBlocks partials)
-void learned_dots(const float *d, const float *ac, int64_t n, double *part, double *out, hipStream_t st);
 // u = dinv d, *nrm = ||u|| and net.act[0] = feat with channel 0 =
 // bf16(u sqrt_count / ||u||), in two launches (part: kRedBlocks partials)
 void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
@@ -105,8 +100,6 @@ void learned_build_input(LearnedNet &net, const float *u, const double *nrm, dou
 void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipStream_t st);
 // c = net.out * (*nrm) / sqrt_count on interior nodes of an M^3 lattice, 0 on its boundary
 void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
-// x += alpha c with alpha = max(0, p[0] / p[1]) (minimal-residual step; 0 if p[1] == 0)
-void learned_step(float *x, const float *c, const double *p, int64_t n, hipStream_t st);
 // out = in on interior nodes of an M^3 lattice, 0 on its boundary
 void learned_mask(const float *in, int M, float *out, hipStream_t st);
 // residual loss mean over interior nodes of (u - dinv * Ac)^2: its gradient
