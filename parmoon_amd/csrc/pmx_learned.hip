@@ -856,14 +856,15 @@ __global__ __launch_bounds__(256) void k_res_loss(int M, const float *__restrict
 // the transpose of the stored stencil operator restricted to interior nodes:
 // y_j = sum_i a_ij g_i over the interior rows i = j + d whose stencil holds
 // offset -d.  A workgroup takes 64 columns j of one residue class, so the
-// class of every neighbour row j + d, its stencil shape and the slot of -d are
-// the same across the workgroup (scalar loads); only the row index inside
-// that class and the bounds vary per lane.  Boundary entries of y are set by
-// the caller.
-template <int E>
+// class of every neighbour row j + d, its stencil shape, the slot of -d and
+// the shift of the class-lattice indices are the same across the workgroup
+// (scalar work, S a compile-time constant); per lane only the neighbour's row
+// index and the bounds remain, without divisions.  Boundary entries of y are
+// set by the caller.
+template <int E, int S>
 __global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo *__restrict__ blocks,
                                                                  const DevSegment *__restrict__ segs,
-                                                                 const int *__restrict__ segidx, int M, int S,
+                                                                 const int *__restrict__ segidx, int M,
                                                                  const float *__restrict__ val,
                                                                  const float *__restrict__ g, float *__restrict__ y) {
   const BlockInfo bi = blocks[blockIdx.x];
@@ -871,16 +872,15 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo
   const int idx = bi.base + threadIdx.x;
   const bool valid = idx < sj.count;
   const int rx = sj.x0 % S, ry = sj.y0 % S, rz = sj.z0 % S;
-  int x = 0, yy = 0, z = 0;
+  int i = 0, j = 0, k = 0;
   if (valid) {
-    const int i = idx % sj.cx, t = idx / sj.cx;
-    x = sj.x0 + S * i;
-    yy = sj.y0 + S * (t % sj.cy);
-    z = sj.z0 + S * (t / sj.cy);
+    i = idx % sj.cx;
+    const int t = idx / sj.cx;
+    j = t % sj.cy;
+    k = t / sj.cy;
   }
+  const int64_t p = (sj.x0 + S * i) + (int64_t)M * ((sj.y0 + S * j) + (int64_t)M * (sj.z0 + S * k));
   float acc = 0.f;
-  // unrolled, so that the uniform segment lookups of all offsets can be
-  // issued ahead instead of one dependent scalar load pair per offset
 #pragma unroll
   for (int dz = -E; dz <= E; dz++)
 #pragma unroll
@@ -893,13 +893,20 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo
         // column j sits at offset -d of row i = j + d
         if (dx < -s.ex || dx > s.ex || dy < -s.ey || dy > s.ey || dz < -s.ez || dz > s.ez) continue;
         const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
-        const int ix = x + dx, iy = yy + dy, iz = z + dz;
-        if (!valid || ix < 1 || ix > M - 2 || iy < 1 || iy > M - 2 || iz < 1 || iz > M - 2) continue;
-        const int ri = (ix - s.x0) / S + s.cx * ((iy - s.y0) / S + s.cy * ((iz - s.z0) / S));
+        // class-lattice index of j + d in the neighbour's class (exact division)
+        const int ti = i + (sj.x0 + dx - s.x0) / S, tj = j + (sj.y0 + dy - s.y0) / S,
+                  tk = k + (sj.z0 + dz - s.z0) / S;
+        // loads are unconditional (row 0 of the class and node p when j + d is
+        // not an interior row, with weight 0), so the wavefront issues them
+        // without waiting per offset
+        const bool ok = valid && ti >= 0 && ti < s.cx && tj >= 0 && tj < s.cy && tk >= 0 && tk < s.cz;
+        const int ri = ok ? ti + s.cx * (tj + s.cy * tk) : 0;
         const int64_t vb = s.voff + (int64_t)(ri / kSlice) * ((int64_t)s.ns * kSlice) + (ri % kSlice);
-        acc += val[vb + (int64_t)slot * kSlice] * g[ix + (int64_t)M * (iy + (int64_t)M * iz)];
+        const int64_t gi = ok ? p + dx + (int64_t)M * (dy + (int64_t)M * dz) : 0;
+        const float prod = val[vb + (int64_t)slot * kSlice] * g[gi];
+        acc += ok ? prod : 0.f;
       }
-  if (valid) y[x + (int64_t)M * (yy + (int64_t)M * z)] = acc;
+  if (valid) y[p] = acc;
 }
 
 void learned_mask(const float *in, int M, float *out, hipStream_t st) {
@@ -915,12 +922,13 @@ void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, st, net.lpart, kLossBlocks, loss_dev);
 }
 void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st) {
+  if (D.S != (D.order == 2 ? 6 : 2)) throw std::logic_error("css_transpose: unexpected residue modulus");
   if (D.order == 2)
-    hipLaunchKernelGGL(k_css_transpose<2>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs,
-                       D.segidx, D.M, D.S, val, g, y);
+    hipLaunchKernelGGL((k_css_transpose<2, 6>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,
+                       D.segs, D.segidx, D.M, val, g, y);
   else
-    hipLaunchKernelGGL(k_css_transpose<1>, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs,
-                       D.segidx, D.M, D.S, val, g, y);
+    hipLaunchKernelGGL((k_css_transpose<1, 2>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,
+                       D.segs, D.segidx, D.M, val, g, y);
   boundary<float>(D, 3, y, nullptr, nullptr, st);  // y = 0 on the boundary nodes
 }
 
