@@ -223,80 +223,124 @@ __global__ void k_bits_differ(const double *__restrict__ a, const double *__rest
 // (the prolongation matrices of FEDatabase3D.C:1656 for Q1/Q2 Lagrange,
 // averaged over the cells sharing a fine DOF as MGComponents3D.C:208-488
 // does, which for a continuous interpolant is the interpolant itself).
-__device__ __forceinline__ int prolong_taps(int x, int order, int nc, int *ci, double *w) {
-  if (order == 1) {
-    if ((x & 1) == 0) { ci[0] = x >> 1; w[0] = 1; return 1; }
-    ci[0] = x >> 1; ci[1] = (x >> 1) + 1; w[0] = w[1] = 0.5; return 2;
+// Taps are held in fixed slots (Q1: 2 prolongation / 3 restriction slots per
+// axis, Q2: 3 / 5) in increasing lattice order; a slot that a node does not
+// use gets weight 0 and a clamped index, and the kernels add its term as an
+// exact 0.  Every load of a node is then issued before the first product
+// (no data-dependent trip counts), and the sum runs over the same terms in
+// the same order as over the used taps alone, so the result is the same bit
+// for bit.
+template <int ORDER> struct TransferTaps {
+  static constexpr int P = ORDER == 1 ? 2 : 3;  // prolongation taps per axis
+  static constexpr int R = ORDER == 1 ? 3 : 5;  // restriction taps per axis
+};
+template <int ORDER>
+__device__ __forceinline__ void prolong_taps(int x, int nc, int *ci, double *w) {
+  if (ORDER == 1) {
+    const int c = x >> 1;
+    const bool odd = x & 1;
+    ci[0] = c; w[0] = odd ? 0.5 : 1.0;
+    ci[1] = odd ? c + 1 : c; w[1] = odd ? 0.5 : 0.0;
+    return;
   }
   int e = x >> 2;
   if (e > nc - 1) e = nc - 1;
   const int r = x - 4 * e;
-  if (r == 0) { ci[0] = 2 * e; w[0] = 1; return 1; }
-  if (r == 2) { ci[0] = 2 * e + 1; w[0] = 1; return 1; }
-  if (r == 4) { ci[0] = 2 * e + 2; w[0] = 1; return 1; }
   ci[0] = 2 * e; ci[1] = 2 * e + 1; ci[2] = 2 * e + 2;
-  if (r == 1) { w[0] = 0.375; w[1] = 0.75; w[2] = -0.125; }
-  else { w[0] = -0.125; w[1] = 0.75; w[2] = 0.375; }
-  return 3;
+  w[0] = r == 0 ? 1.0 : r == 1 ? 0.375 : r == 3 ? -0.125 : 0.0;
+  w[1] = r == 2 ? 1.0 : (r == 1 || r == 3) ? 0.75 : 0.0;
+  w[2] = r == 4 ? 1.0 : r == 1 ? -0.125 : r == 3 ? 0.375 : 0.0;
 }
 // transposed taps: fine nodes feeding coarse node y (DefectRestriction = P^T)
-__device__ __forceinline__ int restrict_taps(int y, int order, int Mf, int *fi, double *w) {
-  int n = 0;
-  auto add = [&](int x, double ww) {
-    if (x >= 0 && x < Mf) { fi[n] = x; w[n] = ww; n++; }
-  };
-  if (order == 1) {
-    add(2 * y - 1, 0.5); add(2 * y, 1.0); add(2 * y + 1, 0.5);
-  } else if ((y & 1) == 0) {
-    const int c = 2 * y;
-    add(c - 3, -0.125); add(c - 1, 0.375); add(c, 1.0); add(c + 1, 0.375); add(c + 3, -0.125);
+template <int ORDER>
+__device__ __forceinline__ void restrict_taps(int y, int Mf, int *fi, double *w) {
+  constexpr int R = TransferTaps<ORDER>::R;
+  int off[R];
+  double ww[R];
+  if (ORDER == 1) {
+    off[0] = -1; off[1] = 0; off[2] = 1;
+    ww[0] = 0.5; ww[1] = 1.0; ww[2] = 0.5;
   } else {
-    const int c = 2 * y;  // = 4e+2
-    add(c - 1, 0.75); add(c, 1.0); add(c + 1, 0.75);
+    const bool even = (y & 1) == 0;
+    off[0] = -3; off[1] = -1; off[2] = 0; off[3] = 1; off[4] = 3;
+    ww[0] = even ? -0.125 : 0.0; ww[1] = even ? 0.375 : 0.75; ww[2] = 1.0; ww[3] = even ? 0.375 : 0.75;
+    ww[4] = even ? -0.125 : 0.0;
   }
-  return n;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int x = 2 * y + off[k];
+    const bool in = x >= 0 && x < Mf;
+    fi[k] = in ? x : 0;
+    w[k] = in ? ww[k] : 0.0;
+  }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf, int Mc, int Mf,
-                                                    int order) {
-  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  const int64_t Nf = (int64_t)Mf * Mf * Mf;
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf, int Mc, int Mf) {
+  constexpr int P = TransferTaps<ORDER>::P;
+  // 32-bit index arithmetic (the launcher checks that the lattice fits)
+  const unsigned id = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned Nf = (unsigned)Mf * Mf * Mf;
   if (id >= Nf) return;
-  const int x = id % Mf, y = (id / Mf) % Mf, z = id / ((int64_t)Mf * Mf);
-  const int nc = (Mc - 1) / order;
-  int ix[3], iy[3], iz[3];
-  double wx[3], wy[3], wz[3];
-  const int nx = prolong_taps(x, order, nc, ix, wx), ny = prolong_taps(y, order, nc, iy, wy),
-            nz = prolong_taps(z, order, nc, iz, wz);
-  double s = 0;
-  for (int c = 0; c < nz; c++)
-    for (int b = 0; b < ny; b++) {
-      const double wzy = wz[c] * wy[b];
-      const T *row = xc + (int64_t)Mc * (iy[b] + (int64_t)Mc * iz[c]);
-      for (int a = 0; a < nx; a++) s += wzy * wx[a] * (double)row[ix[a]];
+  const unsigned yz = id / (unsigned)Mf;
+  const int x = id - yz * Mf, y = yz % (unsigned)Mf, z = yz / (unsigned)Mf;
+  const int nc = (Mc - 1) / ORDER;
+  int ix[P], iy[P], iz[P];
+  double wx[P], wy[P], wz[P];
+  prolong_taps<ORDER>(x, nc, ix, wx);
+  prolong_taps<ORDER>(y, nc, iy, wy);
+  prolong_taps<ORDER>(z, nc, iz, wz);
+  T v[P][P][P];
+#pragma unroll
+  for (int c = 0; c < P; c++)
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      const T *row = xc + (unsigned)Mc * (iy[b] + (unsigned)Mc * iz[c]);
+#pragma unroll
+      for (int a = 0; a < P; a++) v[c][b][a] = row[ix[a]];
     }
-  xf[id] = (T)s;
+  // arithmetic in T: the tap weights and their products are exact in FP32
+  T s = 0;
+#pragma unroll
+  for (int c = 0; c < P; c++)
+#pragma unroll
+    for (int b = 0; b < P; b++) {
+      const T wzy = (T)wz[c] * (T)wy[b];
+#pragma unroll
+      for (int a = 0; a < P; a++) {
+        const T t = wzy * (T)wx[a] * v[c][b][a];
+        s += (wz[c] != 0.0 && wy[b] != 0.0 && wx[a] != 0.0) ? t : T(0);
+      }
+    }
+  xf[id] = s;
 }
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T *__restrict__ rf, int Mc, int Mf,
-                                                     int order) {
-  const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  const int64_t Nc = (int64_t)Mc * Mc * Mc;
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T *__restrict__ rf, int Mc, int Mf) {
+  constexpr int R = TransferTaps<ORDER>::R;
+  const unsigned id = blockIdx.x * kBlock + threadIdx.x;
+  const unsigned Nc = (unsigned)Mc * Mc * Mc;
   if (id >= Nc) return;
-  const int x = id % Mc, y = (id / Mc) % Mc, z = id / ((int64_t)Mc * Mc);
-  int ix[5], iy[5], iz[5];
-  double wx[5], wy[5], wz[5];
-  const int nx = restrict_taps(x, order, Mf, ix, wx), ny = restrict_taps(y, order, Mf, iy, wy),
-            nz = restrict_taps(z, order, Mf, iz, wz);
-  double s = 0;
-  for (int c = 0; c < nz; c++)
-    for (int b = 0; b < ny; b++) {
-      const double wzy = wz[c] * wy[b];
-      const T *row = rf + (int64_t)Mf * (iy[b] + (int64_t)Mf * iz[c]);
-      for (int a = 0; a < nx; a++) s += wzy * wx[a] * (double)row[ix[a]];
+  const unsigned yz = id / (unsigned)Mc;
+  const int x = id - yz * Mc, y = yz % (unsigned)Mc, z = yz / (unsigned)Mc;
+  int ix[R], iy[R], iz[R];
+  double wx[R], wy[R], wz[R];
+  restrict_taps<ORDER>(x, Mf, ix, wx);
+  restrict_taps<ORDER>(y, Mf, iy, wy);
+  restrict_taps<ORDER>(z, Mf, iz, wz);
+  T s = 0;
+#pragma unroll
+  for (int c = 0; c < R; c++)
+#pragma unroll
+    for (int b = 0; b < R; b++) {
+      const T wzy = (T)wz[c] * (T)wy[b];
+      const T *row = rf + (unsigned)Mf * (iy[b] + (unsigned)Mf * iz[c]);
+#pragma unroll
+      for (int a = 0; a < R; a++) {
+        const T t = wzy * (T)wx[a] * row[ix[a]];
+        s += (wz[c] != 0.0 && wy[b] != 0.0 && wx[a] != 0.0) ? t : T(0);
+      }
     }
-  rc[id] = (T)s;
+  rc[id] = s;
 }
 
 // ============================================================ assembly
@@ -785,11 +829,19 @@ void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStr
 }
 template <typename T>
 void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st) {
-  hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, F.order);
+  if (F.N >= (int64_t(1) << 31)) throw std::runtime_error("prolongate: lattice too large for 32-bit indices");
+  if (F.order == 2)
+    hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M);
+  else
+    hipLaunchKernelGGL((k_prolong<T, 1>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M);
 }
 template <typename T>
 void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M, F.order);
+  if (F.N >= (int64_t(1) << 31)) throw std::runtime_error("restrict: lattice too large for 32-bit indices");
+  if (F.order == 2)
+    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M);
+  else
+    hipLaunchKernelGGL((k_restrict<T, 1>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M);
 }
 template <typename T>
 void gemv_coarse(int nI, int N, const double *W, const int *iidx, const T *x, T *y, hipStream_t st) {
