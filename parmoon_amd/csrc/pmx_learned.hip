@@ -35,18 +35,16 @@ int net_param_offset(int l) {
   return s;
 }
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
-  u += 0x7fffu + ((u >> 16) & 1u);                                           // round to nearest even
-  return (uint16_t)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even (NaN stays NaN): the conversion
+// instruction of gfx950 (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 // erf by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7, far below the BF16
-// rounding of the activations): one exponential and one reciprocal
+// rounding of the activations): one exponential and one reciprocal (the
+// hardware's 1-ulp v_rcp_f32)
 __device__ __forceinline__ float erf_as(float x) {
   const float ax = fabsf(x);
-  const float t = __frcp_rn(1.0f + 0.3275911f * ax);
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
   const float poly =
       t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
   const float r = 1.0f - poly * __expf(-ax * ax);
