@@ -411,10 +411,14 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
     } else if (lv.D.max_colour_rows <= wg_limit && sweep_wg_fits(lv.D, sizeof(T))) {
       sweep_wg<T>(lv.D, v.val.p, v.x.p, v.f.p, omega, kind == 3 ? 1 : 0, st);
     } else {
+      // with omega = 1 a colour right after itself (the turn of an SSOR sweep,
+      // the start of the next one) reproduces its rows bit for bit and is skipped
+      const bool skip = kind == 3 && omega == 1.0;
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
-      for (int c = 0; c < lv.L.ncolours; c++) css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
+      for (int c = (skip && s > 0) ? 1 : 0; c < lv.L.ncolours; c++)
+        css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
       if (kind == 3)
-        for (int c = lv.L.ncolours - 1; c >= 0; c--)
+        for (int c = lv.L.ncolours - (skip ? 2 : 1); c >= 0; c--)
           css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
     }
   }

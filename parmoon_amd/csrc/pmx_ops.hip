@@ -115,6 +115,10 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   if (MODE == 0) y[p] = (T)acc;
   else if (MODE == 1) y[p] = (T)(fp - acc);
   else if (MODE == 2) y[p] = (T)((1 - (A)omega) * told + (A)omega * (fp - acc) / diag);
+  // SOR: the Gauss-Seidel value itself when omega = 1, so that a colour
+  // applied twice in a row gives the same bits (the neighbours it reads are
+  // unchanged) and sweeps may skip such a repeat
+  else if (omega == 1.0) y[p] = (T)((fp - acc) / diag);
   else y[p] = (T)((A)omega * ((fp - acc) / diag - told) + told);
 }
 
