@@ -440,8 +440,7 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net) {
   // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
   css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   learned_scaled_input(net, lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
-  net_forward_act(net, lv.L.M, false, st);
-  learned_output(net, lv.L.M, sc, sq, lv.lc.p, st);
+  net_forward_correction(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   learned_update(v.d.p, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
 }

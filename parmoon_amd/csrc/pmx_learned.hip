@@ -69,6 +69,8 @@ __device__ __forceinline__ void tap_off(int t, int &dx, int &dy, int &dz) {
 // The B fragments (weights) stay in registers for the whole brick.
 // MODE 0: hidden layer, out_bf = bf16(GELU(conv + b)), pre = conv + b (if set)
 // MODE 1: output layer (COUT = 1), out_f = conv + b
+// MODE 3: output layer (COUT = 1) rescaled into the correction:
+//         out_f = (conv + b) * (*nrm / sq) on interior nodes, 0 on the boundary
 // MODE 2: input-gradient convolution, out_bf = bf16(conv), no bias
 constexpr int kBX = 16, kBY = 4, kBZ = 4;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2, kHalo = kHX * kHY * kHZ;
@@ -76,7 +78,8 @@ constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2, kHalo = kHX * kHY * k
 template <int CIN, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                               const float *__restrict__ bias, float *__restrict__ pre,
-                                              uint16_t *__restrict__ out_bf, float *__restrict__ out_f) {
+                                              uint16_t *__restrict__ out_bf, float *__restrict__ out_f,
+                                              const double *__restrict__ nrm = nullptr, double sq = 1.0) {
   // K chunks of 32: CIN 1 -> taps 0..26 in one chunk; CIN 8 -> 4 taps per
   // chunk (7 chunks); CIN 16 -> 2 taps per chunk (14 chunks)
   constexpr int TPC = 32 / CIN;
@@ -163,8 +166,9 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
   const int co = l & 15;
   const float bv = (MODE != 2 && co < COUT) ? bias[co] : 0.f;
   const int y = Y0 + wave;
-  if (MODE == 1) {
+  if (MODE == 1 || MODE == 3) {
     if (co != 0) return;
+    const float sc = MODE == 3 ? (float)(nrm[0] / sq) : 1.f;
 #pragma unroll
     for (int q = 0; q < kBZ; q++) {
       const int z = Z0 + q;
@@ -172,7 +176,14 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int xn = X0 + 4 * g + i;
-        if (xn < M) out_f[xn + (int64_t)M * (y + (int64_t)M * z)] = acc[q][i] + bv;
+        if (xn >= M) continue;
+        const float v = acc[q][i] + bv;
+        float o = v;
+        if (MODE == 3) {
+          const bool interior = xn > 0 && y > 0 && z > 0 && xn < M - 1 && y < M - 1 && z < M - 1;
+          o = interior ? v * sc : 0.f;
+        }
+        out_f[xn + (int64_t)M * (y + (int64_t)M * z)] = o;
       }
     }
     return;
@@ -537,6 +548,19 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
                      nullptr, nullptr, net.out);
 }
 
+void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st) {
+  const unsigned g = conv_grid(M);
+  auto P = [&](int l) { return net.params + net_param_offset(l) + net_wsize(l); };  // bias
+  hipLaunchKernelGGL((k_conv<kNetIn, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride, P(0),
+                     nullptr, net.act[1], nullptr, nullptr, 1.0);
+  hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[1], net.wbf + 1 * kWStride, P(1),
+                     nullptr, net.act[2], nullptr, nullptr, 1.0);
+  hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[2], net.wbf + 2 * kWStride, P(2),
+                     nullptr, net.act[3], nullptr, nullptr, 1.0);
+  hipLaunchKernelGGL((k_conv<16, 1, 3>), dim3(g), dim3(256), 0, st, M, net.act[3], net.wbf + 3 * kWStride, P(3),
+                     nullptr, nullptr, c, nrm, sqrt_count);
+}
+
 void net_zero_grads(LearnedNet &net, hipStream_t st) { fill<float>(net.grads, 0.f, net_total_params(), st); }
 
 template <int CIN, int COUT>
@@ -701,15 +725,12 @@ __global__ void k_features(const BlockInfo *__restrict__ blocks, const DevSegmen
   *reinterpret_cast<u16x8 *>(feat + p * kNetIn + 8) = v1;
 }
 
-// MODE 0: c = in * nrm / sq;  MODE 1: c = sign * in * sq / nrm  (interior only)
-template <int MODE>
+// c = sign * in * sq / nrm (interior only)
 __global__ void k_interior_scale(const float *__restrict__ in, int M, const double *__restrict__ nrm, double sq,
                                  float sign, float *__restrict__ c) {
   const int64_t N = (int64_t)M * M * M;
   const double nv = nrm[0];
-  float sc;
-  if (MODE == 0) sc = (float)(nv / sq);
-  else sc = nv > 0 ? (float)(sign * sq / nv) : 0.f;
+  const float sc = nv > 0 ? (float)(sign * sq / nv) : 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     const int x = (int)(i % M), y = (int)((i / M) % M), z = (int)(i / ((int64_t)M * M));
     const bool interior = x > 0 && y > 0 && z > 0 && x < M - 1 && y < M - 1 && z < M - 1;
@@ -810,15 +831,11 @@ void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipS
   hipLaunchKernelGGL(k_features, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M, val,
                      feat);
 }
-void learned_output(const LearnedNet &net, int M, const double *nrm, double sq, float *c, hipStream_t st) {
-  hipLaunchKernelGGL(k_interior_scale<0>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, net.out, M, nrm, sq,
-                     1.0f, c);
-}
 void learned_random(float *x, int M, uint32_t seed, hipStream_t st) {
   hipLaunchKernelGGL(k_learned_random, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, seed);
 }
 void learned_target(const float *x, int M, const double *nrm, double sq, float sign, float *t, hipStream_t st) {
-  hipLaunchKernelGGL(k_interior_scale<1>, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, nrm, sq, sign,
+  hipLaunchKernelGGL(k_interior_scale, dim3(grid_for((int64_t)M * M * M)), dim3(256), 0, st, x, M, nrm, sq, sign,
                      t);
 }
 

@@ -68,6 +68,10 @@ void net_sync_bf16(LearnedNet &net, hipStream_t st);
 void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStream_t st);
 // the same from the BF16 input already in net.act[0]
 void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
+// inference from net.act[0] straight into the correction: c = out * (*nrm) /
+// sqrt_count on interior nodes, 0 on the boundary (net_forward_act followed
+// by learned_output, bit for bit, with the scaling in the output layer)
+void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
 // backward for the loss mean((out - t)^2) over interior nodes; accumulates into net.grads
 // and returns (asynchronously) the loss into *loss_dev
 void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st);
@@ -98,8 +102,6 @@ void learned_build_input(LearnedNet &net, const float *u, const double *nrm, dou
 // x-, y+, y-, z+, z-; 0 where the stencil has no such entry); channel 0, 13..15
 // and boundary nodes 0
 void learned_features(const DevLayout &D, const float *val, uint16_t *feat, hipStream_t st);
-// c = net.out * (*nrm) / sqrt_count on interior nodes of an M^3 lattice, 0 on its boundary
-void learned_output(const LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
 // out = in on interior nodes of an M^3 lattice, 0 on its boundary
 void learned_mask(const float *in, int M, float *out, hipStream_t st);
 // residual loss mean over interior nodes of (u - dinv * Ac)^2: its gradient
