@@ -424,7 +424,7 @@ def main():
         # the FP32 SSOR sweep of the finest level (the largest share of the
         # solve): algorithmic bytes of a multicolour SSOR sweep = 2 x those of a
         # damped-Jacobi sweep (SpMV + f + x_old)
-        "roofline_ssor_sweep": {"bound": "hbm", "kernel": "k_rows<float,3> x 54 colour launches (finest level)",
+        "roofline_ssor_sweep": {"bound": "hbm", "kernel": "k_rows<float,3> x 53 colour launches (finest level: 27 forward, 26 backward)",
                                 "achieved": round(ssor_alg / (kern["ssor_sweep_fp32_ms"] * 1e-3) / 1e9, 1),
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(ssor_alg / (kern["ssor_sweep_fp32_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

@@ -69,8 +69,8 @@ void net_forward(LearnedNet &net, int M, const float *in, bool training, hipStre
 // the same from the BF16 input already in net.act[0]
 void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
 // inference from net.act[0] straight into the correction: c = out * (*nrm) /
-// sqrt_count on interior nodes, 0 on the boundary (net_forward_act followed
-// by learned_output, bit for bit, with the scaling in the output layer)
+// sqrt_count on interior nodes, 0 on the boundary (the output layer applies
+// the scaling and the boundary mask to its FP32 result)
 void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
 // backward for the loss mean((out - t)^2) over interior nodes; accumulates into net.grads
 // and returns (asynchronously) the loss into *loss_dev
