@@ -609,13 +609,7 @@ static int fgmres(System &Y, T *x, const T *b, int m, int maxit, double rmin, do
       T *vi = V + (size_t)i * N, *zi = Z + (size_t)i * N;
       mg_prec<T>(Y, zi, vi);
       fine_matvec<T>(Y, zi, w);
-      for (int k = 0; k <= i; k++) {
-        T *vk = V + (size_t)k * N;
-        dot<T>(w, vk, N, part, H + (size_t)i * ld + k, 0, st);
-        axpy<T>(w, vk, H + (size_t)i * ld + k, -1.0, 0.0, N, st);
-      }
-      dot<T>(w, w, N, part, H + (size_t)i * ld + i + 1, 1, st);
-      scale_inv<T>(V + (size_t)(i + 1) * N, w, H + (size_t)i * ld + i + 1, N, st);
+      mgs_arnoldi<T>(w, V, N, i + 1, H + (size_t)i * ld, V + (size_t)(i + 1) * N, part, st);
       givens(H, cs, sn, s, i, ld, res, st);
       const double r_prev = r;
       r = read_scalar(Y, res);

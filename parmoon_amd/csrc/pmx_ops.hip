@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <stdexcept>
+#include <utility>
 
 #include "pmx_functors.hpp"
 #include "pmx_kernels.hpp"
@@ -782,6 +783,64 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, part, g, out, do_sqrt);
 }
 
+// ------------------------------------------------------- Gram-Schmidt
+// sum of the np partials in k_reduce_final's order, in every workgroup (the
+// same bits everywhere), broadcast to the workgroup
+__device__ __forceinline__ double sum_partials(const double *__restrict__ part, int np) {
+  __shared__ double sh_total;
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) sh_total = s;
+  __syncthreads();
+  return sh_total;
+}
+// one modified Gram-Schmidt step: h = (w, v) from the partials of the
+// previous launch (stored to *h by workgroup 0), w -= h v, and the partials
+// of (w, next) for the following step (next = w: the squared norm)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mgs_step(T *w, const T *__restrict__ v, const T *next,
+                                                     const double *__restrict__ part_in, int np,
+                                                     double *__restrict__ part_out, double *__restrict__ h,
+                                                     int64_t n) {
+  const double hv = sum_partials(part_in, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) h[0] = hv;
+  const T at = (T)(-hv);
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    T wi = w[i];
+    wi += at * v[i];
+    w[i] = wi;
+    s += (double)wi * (double)(next == w ? wi : next[i]);
+  }
+  s = block_sum<T>(s);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+}
+// *h = ||w|| from the partials of (w, w), y = w / ||w||
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_norm_scale(const T *__restrict__ w, const double *__restrict__ part,
+                                                       int np, double *__restrict__ h, T *__restrict__ y, int64_t n) {
+  const double nv = sqrt(sum_partials(part, np));
+  if (blockIdx.x == 0 && threadIdx.x == 0) h[0] = nv;
+  const T sc = (T)(1.0 / nv);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    y[i] = w[i] * sc;
+}
+template <typename T>
+void mgs_arnoldi(T *w, const T *V, int64_t n, int k1, double *h, T *vnext, double *part, hipStream_t st) {
+  int g = (int)((n + kBlock - 1) / kBlock);
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+  double *pa = part, *pb = part + kRedBlocks;
+  hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(kBlock), 0, st, w, V, pa, n);
+  for (int k = 0; k < k1; k++) {
+    const T *next = k + 1 < k1 ? V + (size_t)(k + 1) * n : w;
+    hipLaunchKernelGGL(k_mgs_step<T>, dim3(g), dim3(kBlock), 0, st, w, V + (size_t)k * n, next, pa, g, pb, h + k, n);
+    std::swap(pa, pb);
+  }
+  hipLaunchKernelGGL(k_norm_scale<T>, dim3(g), dim3(kBlock), 0, st, w, pa, g, h + k1, vnext, n);
+}
+
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st) {
@@ -1237,6 +1296,7 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void axpy<T>(T *, const T *, const double *, double, double, int64_t, hipStream_t);              \
   template void scale_inv<T>(T *, const T *, const double *, int64_t, hipStream_t);                         \
   template void dot<T>(const T *, const T *, int64_t, double *, double *, int, hipStream_t);                \
+  template void mgs_arnoldi<T>(T *, const T *, int64_t, int, double *, T *, double *, hipStream_t);         \
   template void css_apply<T>(const DevLayout &, int, const T *, const T *, const T *, T *, double, int,     \
                              hipStream_t);                                                                  \
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \

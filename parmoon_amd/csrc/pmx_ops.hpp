@@ -40,6 +40,13 @@ template <typename T, typename U> void add_convert(T *y, const U *x, int64_t n, 
 template <typename T> void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st);
 template <typename T>
 void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st);
+// modified Gram-Schmidt of w against the k1 vectors V[0..k1) (stride n) and
+// normalisation: h[k] = (w, v_k) with w -= h[k] v_k in turn, h[k1] = ||w||,
+// vnext = w / ||w||, in k1 + 2 launches (each step reduces the previous
+// step's partial sums itself; the values are those of dot + axpy in the same
+// order).  part: 2 kRedBlocks doubles.
+template <typename T>
+void mgs_arnoldi(T *w, const T *V, int64_t n, int k1, double *h, T *vnext, double *part, hipStream_t st);
 
 // mode 0 spmv, 1 defect, 2 jacobi (x = old iterate, y = new), 3 sor (in place on y); colour -1 = all classes
 template <typename T>
