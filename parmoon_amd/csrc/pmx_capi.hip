@@ -55,7 +55,7 @@ struct Ctx {
   int device = 0;
   int systems = 0;  // live systems on this context
   hipStream_t st = nullptr;
-  DevBuf<double> part;  // reduction partials: 2 x kRedBlocks, then scalars
+  DevBuf<double> part;  // reduction partials: 3 x kRedBlocks, then scalars
   double *pinned = nullptr;
 };
 
@@ -430,19 +430,33 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
 // alpha = max(0, (d, Ac) / (Ac, Ac)), so the level defect never grows in the
 // 2-norm (the step-length idea of TMGLevel3D::StepLengthControl,
 // MGLevel3D.C:614-863, applied to the network's correction).
-static void learned_post(System &Y, Level &lv, LearnedNet &net) {
+static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = true, bool carry = false) {
   hipStream_t st = Y.ctx->st;
   Vecs<float> &v = lv.v32;
   const int64_t N = lv.L.N;
   double *sc = Y.lscal.p, *part = Y.ctx->part.p;
   const double sq = std::sqrt((double)lv.L.nactive);
-  // defect of the interior rows; its Dirichlet entries meet zero weights
-  // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
-  css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
-  learned_scaled_input(net, lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
+  if (fresh) {
+    // defect of the interior rows; its Dirichlet entries meet zero weights
+    // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
+    css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+    learned_scaled_input(net, lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
+  } else {
+    // d, u = D^-1 d and the partials of ||u||^2 were left by the previous step
+    learned_carried_input(net, lv.lc.p, N, part + 2 * kRedBlocks, sc, sq, lv.feat.p, st);
+  }
   net_forward_correction(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
-  learned_update(v.d.p, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
+  if (carry)
+    learned_update_carry(v.d.p, lv.lac.p, lv.lc.p, v.x.p, lv.dinv.p, lv.lc.p, N, part, part + 2 * kRedBlocks, st);
+  else
+    learned_update(v.d.p, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
+}
+// `steps` learned steps on a level: the first from the level's defect, the
+// next ones from the defect the previous step updates (d - alpha A c: the
+// same defect to rounding, without a product with A and a pass over d)
+static void learned_smooth(System &Y, Level &lv, LearnedNet &net, int steps) {
+  for (int k = 0; k < steps; k++) learned_post(Y, lv, net, k == 0, k + 1 < steps);
 }
 
 // level i smooths with the network in this cycle
@@ -472,7 +486,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   bool pre_done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learned_pre && learned_level(Y, i)) {
-      for (int k = 0; k < std::max(1, (int)o[20]); k++) learned_post(Y, lv, Y.nets[i]);
+      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]));
       pre_done = true;
     }
   }
@@ -490,7 +504,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
     if (learned_level(Y, i)) {
-      for (int k = 0; k < std::max(1, (int)o[20]); k++) learned_post(Y, lv, Y.nets[i]);
+      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]));
       return;
     }
   }
@@ -912,7 +926,7 @@ static int solve_device(System &Y, double *info) {
   double res = 0, res0 = 0;
   const int maxit = (int)Y.opt[2];
   const double rel_target = Y.opt[16];
-  double *nr = Y.ctx->part.p + 2 * kRedBlocks;
+  double *nr = Y.ctx->part.p + 3 * kRedBlocks;
   if (prec == 0) {
     double rmin = Y.opt[3];
     if (rel_target > 0) {
@@ -1061,7 +1075,7 @@ int pmx_ctx_create(int device, pmx_ctx **out) {
     auto *c = new pmx_ctx();
     c->c.device = device;
     check(hipStreamCreateWithFlags(&c->c.st, hipStreamNonBlocking), "stream");
-    c->c.part.alloc(2 * kRedBlocks + 64);
+    c->c.part.alloc(3 * kRedBlocks + 64);
     check(hipHostMalloc((void **)&c->c.pinned, 64 * sizeof(double), hipHostMallocDefault), "pinned");
     *out = c;
   })
@@ -1329,7 +1343,7 @@ int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, dou
     to_lattice<double>(Y, lv, b, lv.v64.f.p);
     css_apply<double>(lv.D, 1, lv.v64.val.p, lv.v64.aux.p, lv.v64.f.p, lv.v64.d.p, 0.0, -1, st);
     boundary<double>(lv.D, 1, lv.v64.d.p, lv.v64.aux.p, lv.v64.f.p, st);
-    double *nr = Y.ctx->part.p + 2 * kRedBlocks;
+    double *nr = Y.ctx->part.p + 3 * kRedBlocks;
     dot<double>(lv.v64.d.p, lv.v64.d.p, lv.L.N, Y.ctx->part.p, nr, 1, st);
     if (res) *res = read_scalar(Y, nr);
     from_lattice<double>(Y, lv, lv.v64.d.p, r);
@@ -1430,7 +1444,7 @@ int pmx_system_errors(pmx_system *sys, const double *prm, double *err) {
     hipStream_t st = Y.ctx->st;
     ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
     Y.prm.upload(prm, PMX_PRM_LEN, st);
-    double *out = Y.ctx->part.p + 2 * kRedBlocks + 8;
+    double *out = Y.ctx->part.p + 3 * kRedBlocks + 8;
     fe_errors(Y.lev.back()->D, Y.prm.p, Y.qref.p, Y.tab.p, Y.nq, Y.sol.p, Y.ctx->part.p, out, st);
     check(hipMemcpyAsync(err, out, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "errors");
     check(hipStreamSynchronize(st), "errors");

@@ -784,6 +784,40 @@ __global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     x[i] += af * c[i];
 }
+// the step of k_learned_step_parts, and from the updated defect the next
+// step's input: d -= alpha Ac, u = dinv d (u may be c itself: each thread
+// reads c[i] before it writes u[i]) and the partials of ||u||^2 in
+// k_learned_input_sq's order (same grid)
+__global__ __launch_bounds__(kBlock) void k_learned_step_carry(float *__restrict__ x, const float *c,
+                                                               float *__restrict__ d, const float *__restrict__ ac,
+                                                               const float *__restrict__ dinv, float *u,
+                                                               const double *__restrict__ pab,
+                                                               const double *__restrict__ pbb, int np,
+                                                               double *__restrict__ pu, int64_t n) {
+  __shared__ float sh_alpha;
+  double s = 0, t = 0;
+  for (int i = threadIdx.x; i < np; i += kBlock) {
+    s += pab[i];
+    t += pbb[i];
+  }
+  s = block_sum<double>(s);
+  t = block_sum<double>(t);
+  if (threadIdx.x == 0) sh_alpha = (float)(t > 0 ? fmax(0.0, s / t) : 0.0);
+  __syncthreads();
+  const float af = sh_alpha;
+  double q = 0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float ci = c[i];
+    x[i] += af * ci;
+    const float dn = d[i] - af * ac[i];
+    d[i] = dn;
+    const float v = dinv[i] * dn;
+    u[i] = v;
+    q += (double)v * (double)v;
+  }
+  q = block_sum<float>(q);
+  if (threadIdx.x == 0) pu[blockIdx.x] = q;
+}
 __global__ void k_learned_random(float *__restrict__ xv, int M, uint32_t seed) {
   const int64_t N = (int64_t)M * M * M;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
@@ -814,6 +848,18 @@ void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, fl
   hipLaunchKernelGGL(k_learned_input_sq, dim3(g), dim3(kBlock), 0, st, dinv, d, u, part, n);
   hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, part, (int)g, sq, feat, n,
                      net.act[0], nrm);
+}
+void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const double *pu, double *nrm, double sq,
+                           const uint16_t *feat, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, pu, (int)red_grid(n), sq, feat,
+                     n, net.act[0], nrm);
+}
+void learned_update_carry(float *d, const float *ac, const float *c, float *x, const float *dinv, float *u, int64_t n,
+                          double *part, double *pu, hipStream_t st) {
+  const unsigned g = red_grid(n);
+  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
+  hipLaunchKernelGGL(k_learned_step_carry, dim3(g), dim3(kBlock), 0, st, x, c, d, ac, dinv, u, part,
+                     part + kRedBlocks, (int)g, pu, n);
 }
 void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,
                     hipStream_t st) {

@@ -93,6 +93,15 @@ void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, fl
 // (part: 2 kRedBlocks partials)
 void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,
                     hipStream_t st);
+// net.act[0] from u and the partials pu of ||u||^2 that learned_update_carry
+// left (*nrm = ||u||)
+void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const double *pu, double *nrm,
+                           double sqrt_count, const uint16_t *feat, hipStream_t st);
+// learned_update, then from the updated defect d -= alpha ac the next step's
+// u = dinv d (u may alias c) and the partials of ||u||^2 into pu (kRedBlocks
+// doubles apart from part's 2 kRedBlocks), in two launches
+void learned_update_carry(float *d, const float *ac, const float *c, float *x, const float *dinv, float *u, int64_t n,
+                          double *part, double *pu, hipStream_t st);
 // net.act[0] = feat with channel 0 = bf16(u * sqrt_count / *nrm) (nrm: device
 // FP64 2-norm of u; nrm == nullptr: no scaling)
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
