@@ -454,9 +454,10 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = tru
 }
 // `steps` learned steps on a level: the first from the level's defect, the
 // next ones from the defect the previous step updates (d - alpha A c: the
-// same defect to rounding, without a product with A and a pass over d)
-static void learned_smooth(System &Y, Level &lv, LearnedNet &net, int steps) {
-  for (int k = 0; k < steps; k++) learned_post(Y, lv, net, k == 0, k + 1 < steps);
+// same defect to rounding, without a product with A and a pass over d);
+// carry_last: the last step leaves the updated defect too (interior rows)
+static void learned_smooth(System &Y, Level &lv, LearnedNet &net, int steps, bool carry_last = false) {
+  for (int k = 0; k < steps; k++) learned_post(Y, lv, net, k == 0, carry_last || k + 1 < steps);
 }
 
 // level i smooths with the network in this cycle
@@ -486,12 +487,15 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   bool pre_done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learned_pre && learned_level(Y, i)) {
-      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]));
+      // the steps leave the defect of their last update in v.d
+      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]), true);
       pre_done = true;
     }
   }
-  if (!pre_done) level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
-  css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+  if (!pre_done) {
+    level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
+    css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
+  }
   boundary<T>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
   restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st);
   boundary<T>(cl.D, 3, cv.f.p, nullptr, nullptr, st);  // CorrectDefect
