@@ -74,6 +74,10 @@ __device__ __forceinline__ void tap_off(int t, int &dx, int &dy, int &dz) {
 // MODE 2: input-gradient convolution, out_bf = bf16(conv), no bias
 constexpr int kBX = 16, kBY = 4, kBZ = 4;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2, kHalo = kHX * kHY * kHZ;
+// LDS offset (elements) of tap t relative to the centre tap in the halo brick
+template <int CIN> __host__ __device__ constexpr int tap_lds(int t) {
+  return ((t % 3 - 1) + kHX * ((t / 3) % 3 - 1 + kHY * (t / 9 - 1))) * CIN;
+}
 
 template <int CIN, int COUT, int MODE>
 __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
@@ -129,6 +133,12 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
     bw[c] = b;
   }
   __syncthreads();
+  // LDS element of this lane's A fragment for tap 13 (the centre) at q = 0;
+  // tap t and line q add tap_lds<CIN>(t) + q kHX kHY CIN, compile-time
+  // constants once the chunk loop is unrolled (the lane group only selects
+  // among the TPC taps of a chunk)
+  const int gsel = CIN == 1 ? 0 : g / (4 / TPC);
+  const int abase = ((row + 1) + kHX * ((wave + 1) + kHY)) * CIN + (CIN == 1 ? 0 : 8 * (g % (4 / TPC)));
   // x-lines (y, z) = (wave, 0..3) of the brick
   floatx4 acc[kBZ];
 #pragma unroll
@@ -149,14 +159,12 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
           }
         }
       } else {
-        const int t = TPC * c + g / (4 / TPC);
-        const int cb = 8 * (g % (4 / TPC));
-        if (t < kTaps) {
-          int dx, dy, dz;
-          tap_off(t, dx, dy, dz);
-          a = *reinterpret_cast<const u16x8 *>(
-              sh + ((row + 1 + dx) + kHX * ((wave + 1 + dy) + kHY * (q + 1 + dz))) * CIN + cb);
-        }
+        int off = tap_lds<CIN>(TPC * c);
+#pragma unroll
+        for (int k = 1; k < TPC; k++)
+          if (gsel == k) off = tap_lds<CIN>(TPC * c + k);
+        if (TPC * c + gsel < kTaps)
+          a = *reinterpret_cast<const u16x8 *>(sh + abase + q * (kHX * kHY * CIN) + off);
       }
       acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bw[c]),
                                                        acc[q], 0, 0, 0);
