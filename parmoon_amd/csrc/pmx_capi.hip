@@ -414,7 +414,8 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
       // with omega = 1 a colour right after itself (the turn of an SSOR sweep,
       // the start of the next one) reproduces its rows bit for bit and is skipped
       const bool skip = kind == 3 && omega == 1.0;
-      boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
+      // Dirichlet rows x = f (no sweep changes them afterwards)
+      if (s == 0) boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
       for (int c = (skip && s > 0) ? 1 : 0; c < lv.L.ncolours; c++)
         css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
       if (kind == 3)
@@ -497,14 +498,12 @@ static void mg_cycle(System &Y, int i, bool post = true) {
     css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   }
   boundary<T>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
-  restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st);
-  boundary<T>(cl.D, 3, cv.f.p, nullptr, nullptr, st);  // CorrectDefect
-  fill<T>(cv.x.p, (T)0, cl.L.N, st);                     // Reset
+  restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st, true);  // with CorrectDefect
+  fill<T>(cv.x.p, (T)0, cl.L.N, st);                   // Reset
   for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1);
   if ((int)o[12] < 1) Y.rec[i] = 1;
-  prolongate<T>(cl.D, lv.D, cv.x.p, v.aux.p, st);
-  boundary<T>(lv.D, 3, v.aux.p, nullptr, nullptr, st);  // CorrectNodes
-  axpy<T>(v.x.p, v.aux.p, nullptr, 1.0, o[13], lv.L.N, st);
+  // prolongation, CorrectNodes and x += SC_GMG_DAMP e
+  prolongate_add<T>(cl.D, lv.D, cv.x.p, v.x.p, o[13], st);
   if (!post) return;
   if constexpr (std::is_same<T, float>::value) {
     if (learned_level(Y, i)) {

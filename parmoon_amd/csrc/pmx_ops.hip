@@ -280,8 +280,12 @@ __device__ __forceinline__ void restrict_taps(int y, int Mf, int *fi, double *w)
   }
 }
 
-template <typename T, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf, int Mc, int Mf) {
+// ADD = 0: xf = P xc.  ADD = 1: xf += damp (P xc with 0 on the lattice
+// surface), i.e. prolongation, CorrectNodes and the damped update in one
+// pass (the Dirichlet nodes of every layout are its lattice surface)
+template <typename T, int ORDER, int ADD>
+__global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf, int Mc, int Mf,
+                                                    double damp) {
   constexpr int P = TransferTaps<ORDER>::P;
   // 32-bit index arithmetic (the launcher checks that the lattice fits)
   const unsigned id = blockIdx.x * kBlock + threadIdx.x;
@@ -317,10 +321,17 @@ __global__ __launch_bounds__(kBlock) void k_prolong(const T *__restrict__ xc, T 
         s += (wz[c] != 0.0 && wy[b] != 0.0 && wx[a] != 0.0) ? t : T(0);
       }
     }
-  xf[id] = s;
+  if (ADD) {
+    const bool surface = x == 0 || y == 0 || z == 0 || x == Mf - 1 || y == Mf - 1 || z == Mf - 1;
+    xf[id] += (T)damp * (surface ? T(0) : s);
+  } else {
+    xf[id] = s;
+  }
 }
+// zero_surface: 0 on the coarse lattice surface (CorrectDefect fused)
 template <typename T, int ORDER>
-__global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T *__restrict__ rf, int Mc, int Mf) {
+__global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T *__restrict__ rf, int Mc, int Mf,
+                                                     int zero_surface) {
   constexpr int R = TransferTaps<ORDER>::R;
   const unsigned id = blockIdx.x * kBlock + threadIdx.x;
   const unsigned Nc = (unsigned)Mc * Mc * Mc;
@@ -345,6 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T
         s += (wz[c] != 0.0 && wy[b] != 0.0 && wx[a] != 0.0) ? t : T(0);
       }
     }
+  if (zero_surface && (x == 0 || y == 0 || z == 0 || x == Mc - 1 || y == Mc - 1 || z == Mc - 1)) s = 0;
   rc[id] = s;
 }
 
@@ -894,17 +906,27 @@ template <typename T>
 void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st) {
   if (F.N >= (int64_t(1) << 31)) throw std::runtime_error("prolongate: lattice too large for 32-bit indices");
   if (F.order == 2)
-    hipLaunchKernelGGL((k_prolong<T, 2>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M);
+    hipLaunchKernelGGL((k_prolong<T, 2, 0>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, 1.0);
   else
-    hipLaunchKernelGGL((k_prolong<T, 1>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M);
+    hipLaunchKernelGGL((k_prolong<T, 1, 0>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, 1.0);
 }
 template <typename T>
-void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st) {
+void prolongate_add(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, double damp, hipStream_t st) {
+  if (F.N >= (int64_t(1) << 31)) throw std::runtime_error("prolongate: lattice too large for 32-bit indices");
+  if (F.order == 2)
+    hipLaunchKernelGGL((k_prolong<T, 2, 1>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, damp);
+  else
+    hipLaunchKernelGGL((k_prolong<T, 1, 1>), dim3(grid_for(F.N)), dim3(kBlock), 0, st, xc, xf, C.M, F.M, damp);
+}
+template <typename T>
+void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st, bool zero_surface) {
   if (F.N >= (int64_t(1) << 31)) throw std::runtime_error("restrict: lattice too large for 32-bit indices");
   if (F.order == 2)
-    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M);
+    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M,
+                       (int)zero_surface);
   else
-    hipLaunchKernelGGL((k_restrict<T, 1>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M);
+    hipLaunchKernelGGL((k_restrict<T, 1>), dim3(grid_for(C.N)), dim3(kBlock), 0, st, rc, rf, C.M, F.M,
+                       (int)zero_surface);
 }
 template <typename T>
 void gemv_coarse(int nI, int N, const double *W, const int *iidx, const T *x, T *y, hipStream_t st) {
@@ -1302,7 +1324,8 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
   template void sweep_wg<T>(const DevLayout &, const T *, T *, const T *, double, int, hipStream_t);          \
   template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
-  template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t);            \
+  template void prolongate_add<T>(const DevLayout &, const DevLayout &, const T *, T *, double, hipStream_t); \
+  template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t, bool);      \
   template void gemv_coarse<T>(int, int, const double *, const int *, const T *, T *, hipStream_t);         \
   template void csr_spmv<T>(int, const int *, const int *, const T *, const T *, T *, hipStream_t);         \
   template void css_to_csr<T>(const DevLayout &, const int *, const int *, const int *, int64_t, const T *, \

@@ -77,7 +77,14 @@ void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, 
 template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st);
 template <typename T>
 void prolongate(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, hipStream_t st);
-template <typename T> void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st);
+// xf += damp * (P xc with the fine Dirichlet nodes set to 0): prolongation,
+// CorrectNodes and the damped update of TMultiGrid3D::Cycle in one launch
+// (the Dirichlet nodes of a layout are its lattice surface)
+template <typename T>
+void prolongate_add(const DevLayout &C, const DevLayout &F, const T *xc, T *xf, double damp, hipStream_t st);
+// zero_surface: the coarse Dirichlet entries are set to 0 (CorrectDefect)
+template <typename T>
+void restrict_(const DevLayout &C, const DevLayout &F, T *rc, const T *rf, hipStream_t st, bool zero_surface = false);
 // coarsest-level solve: y_I = W x with W = A_II^-1 [I, -A_IB] (n_I x N, lattice
 // columns; iidx: interior row -> lattice node); y_B = x_B is set separately
 template <typename T>
