@@ -497,8 +497,11 @@ static void mg_cycle(System &Y, int i, bool post = true) {
     level_smooth<T>(Y, lv, kind, o[8], (int)o[6]);
     css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   }
-  boundary<T>(lv.D, 1, v.d.p, v.x.p, v.f.p, st);
-  restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st, true);  // with CorrectDefect
+  // the defect's Dirichlet entries are not set: the restriction to an
+  // interior coarse node reads interior fine nodes only (taps 2y-3..2y+3 and
+  // 2y-1..2y+1 with 1 <= y <= Mc-2), and the coarse surface is set to 0
+  // (CorrectDefect)
+  restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st, true);
   fill<T>(cv.x.p, (T)0, cl.L.N, st);                   // Reset
   for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1);
   if ((int)o[12] < 1) Y.rec[i] = 1;
