@@ -809,34 +809,72 @@ __device__ __forceinline__ double sum_partials(const double *__restrict__ part, 
 }
 // one modified Gram-Schmidt step: h = (w, v) from the partials of the
 // previous launch (stored to *h by workgroup 0), w -= h v, and the partials
-// of (w, next) for the following step (next = w: the squared norm)
+// of (w, next) for the following step (next = w: the squared norm).  The
+// thread's first kPrefetch elements are loaded before the partials are
+// summed, so their latency overlaps that reduction; elements and their
+// order per thread are those of the plain grid-stride loop.
+constexpr int kPrefetch = 8;
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mgs_step(T *w, const T *__restrict__ v, const T *next,
                                                      const double *__restrict__ part_in, int np,
                                                      double *__restrict__ part_out, double *__restrict__ h,
                                                      int64_t n) {
+  const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+  const bool self = next == w;
+  T wv[kPrefetch], vv[kPrefetch], nv[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; k++) {
+    const int64_t i = i0 + k * stride;
+    wv[k] = vv[k] = nv[k] = T(0);
+    if (i < n) {
+      wv[k] = w[i];
+      vv[k] = v[i];
+      if (!self) nv[k] = next[i];
+    }
+  }
   const double hv = sum_partials(part_in, np);
   if (blockIdx.x == 0 && threadIdx.x == 0) h[0] = hv;
   const T at = (T)(-hv);
   double s = 0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+  for (int k = 0; k < kPrefetch; k++) {
+    const int64_t i = i0 + k * stride;
+    if (i < n) {
+      T wi = wv[k];
+      wi += at * vv[k];
+      w[i] = wi;
+      s += (double)wi * (double)(self ? wi : nv[k]);
+    }
+  }
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) {
     T wi = w[i];
     wi += at * v[i];
     w[i] = wi;
-    s += (double)wi * (double)(next == w ? wi : next[i]);
+    s += (double)wi * (double)(self ? wi : next[i]);
   }
   s = block_sum<T>(s);
   if (threadIdx.x == 0) part_out[blockIdx.x] = s;
 }
-// *h = ||w|| from the partials of (w, w), y = w / ||w||
+// *h = ||w|| from the partials of (w, w), y = w / ||w|| (prefetch as above)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_norm_scale(const T *__restrict__ w, const double *__restrict__ part,
                                                        int np, double *__restrict__ h, T *__restrict__ y, int64_t n) {
+  const int64_t i0 = blockIdx.x * (int64_t)kBlock + threadIdx.x, stride = (int64_t)gridDim.x * kBlock;
+  T wv[kPrefetch];
+#pragma unroll
+  for (int k = 0; k < kPrefetch; k++) {
+    const int64_t i = i0 + k * stride;
+    wv[k] = i < n ? w[i] : T(0);
+  }
   const double nv = sqrt(sum_partials(part, np));
   if (blockIdx.x == 0 && threadIdx.x == 0) h[0] = nv;
   const T sc = (T)(1.0 / nv);
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    y[i] = w[i] * sc;
+#pragma unroll
+  for (int k = 0; k < kPrefetch; k++) {
+    const int64_t i = i0 + k * stride;
+    if (i < n) y[i] = wv[k] * sc;
+  }
+  for (int64_t i = i0 + kPrefetch * stride; i < n; i += stride) y[i] = w[i] * sc;
 }
 template <typename T>
 void mgs_arnoldi(T *w, const T *V, int64_t n, int k1, double *h, T *vnext, double *part, hipStream_t st) {
