@@ -129,6 +129,17 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */
 int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, double *sol, double *info);
+/* Step length control of the MG cycle (TMGLevel3D::StepLengthControl,
+ * src/FE/MGLevel3D.C:908-960, applied as in TMultiGrid3D::Cycle,
+ * src/FE/MultiGrid3D.C:465-471, 562-566, 671-679): on the levels it covers,
+ * u and d = f - A u are kept before the pre-smoothing, and after the
+ * post-smoothing u = uold + omega (u - uold) with
+ * omega = (d, A(u - uold)) / ||A(u - uold)||^2 (opt[8] when that is undefined
+ * or |omega| < 0.1).  mode: 0 off (the CD3D/TCD3D input files' setting),
+ * 1 finest level (SC_STEP_LENGTH_CONTROL_FINE_SCALAR), 2 every level above
+ * the coarsest (SC_STEP_LENGTH_CONTROL_ALL_SCALAR, Database.C:688's default).
+ * Applies to the following solves of the system. */
+int pmx_system_set_step_length_control(pmx_system *sys, int mode);
 /* Device-resident variant: solves with the assembled right-hand side already
  * in HBM, zero initial guess with the Dirichlet values of the rhs
  * (SystemCD3D.C:356).  The solution stays on the device. */

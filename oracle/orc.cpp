@@ -883,13 +883,32 @@ struct System {
   std::vector<std::vector<double>> sol, f, aux0, aux1;
   std::vector<int> rec;
   double opt[16];
+  // SC_STEP_LENGTH_CONTROL_FINE_SCALAR (1) / _ALL_SCALAR (2) / off (0)
+  int slc = 0;
+  std::vector<std::vector<double>> old_sol, old_def;
 };
 
 static void mg_smooth(System &Y, int l, int kind, int n, double omega) {
   for (int k = 0; k < n; k++)
     smooth(Y.lev[l], Y.E[l].data(), kind, omega, Y.sol[l].data(), Y.f[l].data(), Y.aux1[l].data());
 }
-// TMultiGrid3D::Cycle (src/FE/MultiGrid3D.C:449-695) without step length control
+// TMGLevel3D::StepLengthControl (src/FE/MGLevel3D.C:908-960): with
+// x = u - uold and y = A x, omega = (def, y) / (y, y); omega = Parameters[0]
+// (the smoother damping, as the system passes SC_SMOOTH_DAMP_FACTOR_SCALAR)
+// when (y, y) <= 0 or |omega| < 0.1
+static double step_length(const Space &S, const double *E, const double *u, const double *uold, const double *def,
+                          double param0) {
+  std::vector<double> x(S.N), y(S.N, 0.0);
+  for (int i = 0; i < S.N; i++) x[i] = u[i] - uold[i];
+  matvec(S, E, x.data(), y.data());
+  const double num = ddot(S.N, def, y.data()), nom = ddot(S.N, y.data(), y.data());
+  double omega = nom > 0 ? num / nom : param0;
+  if (std::fabs(omega) < 0.1) omega = param0;
+  return omega;
+}
+// TMultiGrid3D::Cycle (src/FE/MultiGrid3D.C:449-695); step length control
+// (slc, MultiGrid3D.C:465-471, 534-538, 562-566, 671-679) on the finest level
+// (1) or every level above the coarsest (2)
 static void mg_cycle(System &Y, int i) {
   const double *o = Y.opt;
   const Space &S = Y.lev[i];
@@ -919,6 +938,12 @@ static void mg_cycle(System &Y, int i) {
   }
   const Space &C = Y.lev[i - 1];
   const int kind = (int)o[5];
+  const bool slc = Y.slc == 2 || (Y.slc == 1 && i == (int)Y.lev.size() - 1);
+  if (slc) {
+    Y.old_sol[i] = Y.sol[i];
+    Y.old_def[i].assign(S.N, 0.0);
+    defect(S, Y.E[i].data(), Y.sol[i].data(), Y.f[i].data(), Y.old_def[i].data());
+  }
   mg_smooth(Y, i, kind, (int)o[6], o[8]);
   defect(S, Y.E[i].data(), Y.sol[i].data(), Y.f[i].data(), Y.aux0[i].data());
   restrict_defect(C, S, Y.tr[i], Y.f[i - 1].data(), Y.aux0[i].data());
@@ -931,6 +956,10 @@ static void mg_cycle(System &Y, int i) {
   std::fill(Y.aux0[i].begin() + S.active, Y.aux0[i].end(), 0.0);  // CorrectNodes
   daxpy(S.N, o[13], Y.aux0[i].data(), Y.sol[i].data());           // Update
   mg_smooth(Y, i, kind, (int)o[7], o[8]);
+  if (slc) {
+    const double a = step_length(S, Y.E[i].data(), Y.sol[i].data(), Y.old_sol[i].data(), Y.old_def[i].data(), o[8]);
+    for (int j = 0; j < S.N; j++) Y.sol[i][j] = Y.old_sol[i][j] + a * (Y.sol[i][j] - Y.old_sol[i][j]);
+  }
 }
 // TMultiGridScaIte::Iterate (src/FE/MultiGridScaIte.C:89-130), zero start
 static void mg_prec(System &Y, double *z, const double *v) {
@@ -1259,6 +1288,12 @@ double *orc_system_entries(void *sys, int level) { return static_cast<System *>(
 double *orc_system_rhs(void *sys, int level) { return static_cast<System *>(sys)->rhs[level].data(); }
 int orc_system_solve(void *sys, const double *opt, const double *rhs, double *sol, double *info) {
   return solve(*static_cast<System *>(sys), opt, rhs, sol, info);
+}
+void orc_system_set_slc(void *sys, int mode) {
+  System &Y = *static_cast<System *>(sys);
+  Y.slc = mode;
+  Y.old_sol.resize(Y.lev.size());
+  Y.old_def.resize(Y.lev.size());
 }
 void orc_errors(void *sp, const double *prm, const double *sol, double *err) { errors(SP(sp), prm, sol, err); }
 // which = 0: exact solution values at the nodes (TFEFunction3D::Interpolate)

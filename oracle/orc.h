@@ -70,6 +70,10 @@ double *orc_system_rhs(void *sys, int level);
  * info out: [0] iterations [1] final residual [2] initial residual */
 int orc_system_solve(void *sys, const double *opt, const double *rhs,
                      double *sol, double *info);
+/* step length control of the MG cycle (TMGLevel3D::StepLengthControl,
+ * MGLevel3D.C:908): 0 off, 1 finest level (SC_STEP_LENGTH_CONTROL_FINE_SCALAR),
+ * 2 every level above the coarsest (SC_STEP_LENGTH_CONTROL_ALL_SCALAR) */
+void orc_system_set_slc(void *sys, int mode);
 
 /* ---- errors: L2 and H1-semi against the example's exact solution at prm time ---- */
 void orc_errors(void *sp, const double *prm, const double *sol, double *err);

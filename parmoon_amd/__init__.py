@@ -56,6 +56,7 @@ def lib():
         L.pmx_csr_matvec.argtypes = [vp, ip, vp, vp, vp, vp, vp]
         L.pmx_system_solve.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
+        L.pmx_system_set_step_length_control.argtypes = [vp, ctypes.c_int]
         L.pmx_system_solution.argtypes = [vp, vp]
         L.pmx_system_errors.argtypes = [vp, vp, vp]
         L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
@@ -232,6 +233,11 @@ class System:
         if check:
             _ok(rc, "pmx_system_solve")
         return s, info, rc
+
+    def set_step_length_control(self, mode):
+        """MG step length control (TMGLevel3D::StepLengthControl): 0 off,
+        1 finest level, 2 every level above the coarsest."""
+        _ok(lib().pmx_system_set_step_length_control(self.h, int(mode)), "pmx_system_set_step_length_control")
 
     def solve_resident(self, opt, check=True):
         o = np.zeros(24)

@@ -64,6 +64,7 @@ template <typename T>
 struct Vecs {
   DevBuf<T> val;                      // CSS values in this precision
   DevBuf<T> x, f, d, aux;             // MG level vectors
+  DevBuf<T> sold, sdef;               // step length control: u and f - A u before the pre-smoothing
 };
 
 struct Level {
@@ -119,7 +120,7 @@ struct System {
   struct Graph {
     bool valid = false;
     int prec = 0;
-    double key[14] = {0};
+    double key[15] = {0};
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[2];
@@ -132,6 +133,10 @@ struct System {
   int learned_skip = 0;     // finest levels inside learned_levels that keep the plain smoother
   bool learned_pre = false; // learned steps also replace the pre-smoothing
   int trained_levels = 1;  // levels of the last training
+  // step length control of the MG cycle: 0 off, 1 finest level
+  // (SC_STEP_LENGTH_CONTROL_FINE_SCALAR), 2 every level above the coarsest
+  // (SC_STEP_LENGTH_CONTROL_ALL_SCALAR)
+  int slc = 0;
   bool learn_active = false;
   int fallbacks = 0;
   DevBuf<double> lscal;               // norm, (d, Ac), (Ac, Ac)
@@ -430,7 +435,7 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
 // rms(u)); x += alpha c with the minimal-residual step
 // alpha = max(0, (d, Ac) / (Ac, Ac)), so the level defect never grows in the
 // 2-norm (the step-length idea of TMGLevel3D::StepLengthControl,
-// MGLevel3D.C:614-863, applied to the network's correction).
+// MGLevel3D.C:908-960, applied to the network's correction).
 static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = true, bool carry = false) {
   hipStream_t st = Y.ctx->st;
   Vecs<float> &v = lv.v32;
@@ -485,6 +490,15 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   Level &cl = *Y.lev[i - 1];
   Vecs<T> &cv = vecs<T>(cl);
   const int kind = (int)o[5];
+  // step length control (MultiGrid3D.C:465-471, 534-538, 562-566): keep u
+  // and its defect before the pre-smoothing
+  const bool slc = Y.slc == 2 || (Y.slc == 1 && i == (int)Y.lev.size() - 1);
+  if (slc) {
+    if (!v.sold.p || !v.sdef.p) throw std::logic_error("step length control buffers missing");
+    copy<T>(v.sold.p, v.x.p, lv.L.N, st);
+    css_apply<T>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.sdef.p, 0.0, -1, st);
+    boundary<T>(lv.D, 1, v.sdef.p, v.x.p, v.f.p, st);
+  }
   bool pre_done = false;
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learned_pre && learned_level(Y, i)) {
@@ -507,14 +521,27 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if ((int)o[12] < 1) Y.rec[i] = 1;
   // prolongation, CorrectNodes and x += SC_GMG_DAMP e
   prolongate_add<T>(cl.D, lv.D, cv.x.p, v.x.p, o[13], st);
-  if (!post) return;
-  if constexpr (std::is_same<T, float>::value) {
-    if (learned_level(Y, i)) {
-      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]));
-      return;
+  if (post) {
+    bool post_done = false;
+    if constexpr (std::is_same<T, float>::value) {
+      if (learned_level(Y, i)) {
+        learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]));
+        post_done = true;
+      }
     }
+    if (!post_done) level_smooth<T>(Y, lv, kind, o[8], (int)o[7]);
   }
-  level_smooth<T>(Y, lv, kind, o[8], (int)o[7]);
+  if (slc) {
+    // TMGLevel3D::StepLengthControl (MGLevel3D.C:908-960), MultiGrid3D.C:671-679:
+    // e = u - uold, y = A e, omega = (def, y) / (y, y), u = uold + omega e
+    double *num = Y.ctx->part.p + 3 * kRedBlocks + 16;
+    axpy_diff<T>(v.aux.p, v.x.p, v.sold.p, lv.L.N, st);
+    css_apply<T>(lv.D, 0, v.val.p, v.aux.p, nullptr, v.d.p, 0.0, -1, st);
+    boundary<T>(lv.D, 0, v.d.p, v.aux.p, nullptr, st);
+    dot<T>(v.sdef.p, v.d.p, lv.L.N, Y.ctx->part.p, num, 0, st);
+    dot<T>(v.d.p, v.d.p, lv.L.N, Y.ctx->part.p, num + 1, 0, st);
+    step_length_update<T>(v.x.p, v.sold.p, num, o[8], lv.L.N, st);
+  }
 }
 // TMultiGrid3D::SetRecursion (MultiGrid3D.C:697-725): gamma cycles per level
 static void set_recursion(System &Y) {
@@ -546,7 +573,7 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(v.f.p, vin, lv.L.N, st);
   const int pi = std::is_same<T, double>::value ? 0 : 1;
   System::Graph &g = Y.graphs[pi];
-  double key[14];
+  double key[15];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
   const bool learned = pi == 1 && Y.learn_active;
   key[9] = learned ? Y.learned_levels : 0;
@@ -554,6 +581,7 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   key[11] = Y.opt[21];
   key[12] = learned ? Y.learned_skip : 0;
   key[13] = learned ? Y.learned_pre : 0;
+  key[14] = Y.slc;
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1433,6 +1461,21 @@ int pmx_system_solve_resident(pmx_system *sys, const double *opt, double *info) 
     boundary<double>(lv.D, 2, Y.sol.p, nullptr, Y.b.p, st);
     const int rc = solve_device(Y, info);
     if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+int pmx_system_set_step_length_control(pmx_system *sys, int mode) {
+  PMX_TRY({
+    if (mode < 0 || mode > 2) throw std::invalid_argument("step length control: mode must be 0, 1 or 2");
+    System &Y = sys->s;
+    // the kept iterate and defect of every level, both precisions (allocated
+    // here: the cycle may be captured into a graph, where nothing allocates)
+    if (mode > 0)
+      for (auto &l : Y.lev)
+        for (DevBuf<double> *b : {&l->v64.sold, &l->v64.sdef}) b->alloc(l->L.N);
+    if (mode > 0)
+      for (auto &l : Y.lev)
+        for (DevBuf<float> *b : {&l->v32.sold, &l->v32.sdef}) b->alloc(l->L.N);
+    Y.slc = mode;
   })
 }
 int pmx_system_solution(pmx_system *sys, double *sol) {

@@ -795,6 +795,32 @@ void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sq
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, part, g, out, do_sqrt);
 }
 
+// ------------------------------------------------------- step length control
+template <typename T>
+__global__ void k_diff(T *__restrict__ e, const T *__restrict__ u, const T *__restrict__ uold, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    e[i] = u[i] - uold[i];
+}
+// omega = num / nom (param0 if nom <= 0 or |omega| < 0.1), u = uold + omega (u - uold)
+template <typename T>
+__global__ void k_step_length(T *__restrict__ u, const T *__restrict__ uold, const double *__restrict__ num_nom,
+                              double param0, int64_t n) {
+  const double num = num_nom[0], nom = num_nom[1];
+  double omega = nom > 0 ? num / nom : param0;
+  if (fabs(omega) < 0.1) omega = param0;
+  const T a = (T)omega;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    u[i] = uold[i] + a * (u[i] - uold[i]);
+}
+template <typename T>
+void axpy_diff(T *e, const T *u, const T *uold, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_diff<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, e, u, uold, n);
+}
+template <typename T>
+void step_length_update(T *u, const T *uold, const double *num_nom, double param0, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_length<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, u, uold, num_nom, param0, n);
+}
+
 // ------------------------------------------------------- Gram-Schmidt
 // sum of the np partials in k_reduce_final's order, in every workgroup (the
 // same bits everywhere), broadcast to the workgroup
@@ -1357,6 +1383,8 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void scale_inv<T>(T *, const T *, const double *, int64_t, hipStream_t);                         \
   template void dot<T>(const T *, const T *, int64_t, double *, double *, int, hipStream_t);                \
   template void mgs_arnoldi<T>(T *, const T *, int64_t, int, double *, T *, double *, hipStream_t);         \
+  template void axpy_diff<T>(T *, const T *, const T *, int64_t, hipStream_t);                              \
+  template void step_length_update<T>(T *, const T *, const double *, double, int64_t, hipStream_t);        \
   template void css_apply<T>(const DevLayout &, int, const T *, const T *, const T *, T *, double, int,     \
                              hipStream_t);                                                                  \
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \

@@ -40,6 +40,12 @@ template <typename T, typename U> void add_convert(T *y, const U *x, int64_t n, 
 template <typename T> void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st);
 template <typename T>
 void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st);
+// e = u - uold
+template <typename T> void axpy_diff(T *e, const T *u, const T *uold, int64_t n, hipStream_t st);
+// TMGLevel3D::StepLengthControl's update: omega = num_nom[0] / num_nom[1]
+// (param0 when num_nom[1] <= 0 or |omega| < 0.1), u = uold + omega (u - uold)
+template <typename T>
+void step_length_update(T *u, const T *uold, const double *num_nom, double param0, int64_t n, hipStream_t st);
 // modified Gram-Schmidt of w against the k1 vectors V[0..k1) (stride n) and
 // normalisation: h[k] = (w, v_k) with w -= h[k] v_k in turn, h[k1] = ||w||,
 // vnext = w / ||w||, in k1 + 2 launches (each step reduces the previous

@@ -50,6 +50,7 @@ def lib():
         L.orc_system_rhs.argtypes = [vp, ip]
         L.orc_system_solve.argtypes = [vp, vp, vp, vp, vp]
         L.orc_system_solve.restype = ip
+        L.orc_system_set_slc.argtypes = [vp, ip]
         L.orc_errors.argtypes = [vp, vp, vp, vp]
         L.orc_interpolate.argtypes = [vp, vp, ip, vp]
         L.orc_tcd3d_run.argtypes = [vp, vp, vp, vp, vp, vp, ip]
@@ -179,6 +180,9 @@ class System:
     def rhs(self, level=-1):
         l = level % len(self.levels)
         return np.ctypeslib.as_array(self.L.orc_system_rhs(self.h, l), (self.levels[l].N,)).copy()
+
+    def set_slc(self, mode):
+        self.L.orc_system_set_slc(self.h, int(mode))
 
     def solve(self, opt, rhs, sol):
         o = np.zeros(24)

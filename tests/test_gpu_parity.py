@@ -318,3 +318,31 @@ def test_device_error_norms(ctx, order):
             S.close()
     r0, r1 = errs[0][0] / errs[1][0], errs[0][1] / errs[1][1]
     assert abs(np.log2(r0) - (order + 1)) < 0.25 and abs(np.log2(r1) - order) < 0.25, (r0, r1)
+
+
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_step_length_control_cycle(request, which, mode):
+    """One damped-Jacobi V(2,2) cycle with MG step length control
+    (TMGLevel3D::StepLengthControl) on the finest level (mode 1) or every
+    level above the coarsest (mode 2), FP64, against the oracle's cycle
+    (Jacobi sweeps do not depend on the row order)."""
+    S, O, prm = request.getfixturevalue(which)
+    print(f"[case] step length control {which} mode {mode}", flush=True)
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    x0 = np.zeros(fine.N)
+    x0[fine.active:] = rhs[fine.active:]
+    opt = P.solver_opt(solver=0, smoother=1, omega=0.8, pre=2, post=2)
+    O.set_slc(mode)
+    ref, _ = O.solve(opt, rhs, x0)
+    O.set_slc(0)
+    plain, _ = O.solve(opt, rhs, x0)
+    S.set_step_length_control(mode)
+    try:
+        s, info, rc = S.solve(opt, rhs, x0)
+    finally:
+        S.set_step_length_control(0)
+    assert rc == 0
+    assert rel(s, ref) < 1e-12, rel(s, ref)
+    assert rel(plain, ref) > 1e-6

@@ -266,3 +266,36 @@ def test_dense_lu():
     A = rng.standard_normal((40, 40)) + 40 * np.eye(40)
     b = rng.standard_normal(40)
     assert np.allclose(orc.dense_lu_solve(A, b), np.linalg.solve(A, b), atol=1e-12)
+
+
+def test_step_length_control_is_the_minimal_residual_step():
+    """TMGLevel3D::StepLengthControl on the finest level (MGLevel3D.C:908-960):
+    one cycle with control returns x0 + omega (u - x0), u the cycle without
+    control from the same start x0, omega minimising ||d0 - omega A (u - x0)||
+    with d0 = b - A x0 (the scipy matrix of the assembled entries)."""
+    S = orc.System(1, 3, 1)
+    S.assemble(P.config1b())
+    fine = S.levels[-1]
+    rhs = S.rhs()
+    x0 = np.zeros(fine.N)
+    x0[fine.active:] = rhs[fine.active:]
+    sp = orc.Space(3, 1)
+    A = csr(sp, S.entries())
+    opt = P.solver_opt(solver=0, smoother=1, omega=0.8, pre=2, post=2)
+    u, _ = S.solve(opt, rhs, x0)
+    S.set_slc(1)
+    u1, _ = S.solve(opt, rhs, x0)
+    S.set_slc(0)
+    e = u - x0
+    y = A @ e
+    omega = float((rhs - A @ x0) @ y / (y @ y))
+    assert abs(omega) >= 0.1 and abs(omega - 1.0) > 1e-3
+    assert np.abs(u1 - (x0 + omega * e)).max() < 1e-12 * np.abs(u1).max()
+    # with control on every level the coarse corrections change too; the
+    # finest step is again the minimal-residual one, so the finest defect
+    # does not grow (omega = 0 is among the candidates)
+    S.set_slc(2)
+    u2, _ = S.solve(opt, rhs, x0)
+    S.set_slc(0)
+    assert np.abs(u2 - u1).max() > 1e-8 * np.abs(u1).max()
+    assert np.linalg.norm(rhs - A @ u2) <= np.linalg.norm(rhs - A @ x0)
