@@ -109,8 +109,12 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *      13 BiCGStab (Bcgs.C:111), 0 = one MG cycle
  *  [1] restart  [2] maxit  [3] res_norm_min (absolute)  [4] red_factor
  *  [5] smoother 1 Jacobi 2 SOR 3 SSOR  [6] pre  [7] post  [8] omega
- *  [9] coarse: 17 exact (FP64 dense inverse), 1/2/3 = that many-kind sweeps
- *  [10] coarse sweeps  [11] unused  [12] cycle 1 V 2 W 0 F  [13] GMG damp
+ *  [9] coarse: 17 exact (FP64 dense inverse), 1/2/3 = Jacobi/SOR/SSOR sweeps
+ *      until the coarse defect is reduced by [11] or [10] sweeps were made
+ *      (MultiGrid3D.C:225-337; one workgroup where the level fits its LDS)
+ *  [10] coarse maxit (SC_COARSE_MAXIT_SCALAR)
+ *  [11] coarse reduction factor (SC_COARSE_RED_FACTOR_SCALAR)
+ *  [12] cycle 1 V 2 W 0 F  [13] GMG damp
  *  [14] BiCGStab flexible branch (1 = v = A y)
  *  [15] precision: 0 FP64 throughout, 1 FP32 operator/MG/Krylov vectors
  *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother

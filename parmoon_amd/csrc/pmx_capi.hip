@@ -482,6 +482,10 @@ static void mg_cycle(System &Y, int i, bool post = true) {
     if (cs == 17 && lv.ainv_n > 0) {
       gemv_coarse<T>(lv.ainv_n, (int)lv.L.N, lv.ainv.p, lv.iidx.p, v.f.p, v.x.p, st);
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
+    } else if (cs >= 1 && cs <= 3 && sweep_wg_fits(lv.D, sizeof(T))) {
+      // sweeps until the defect is reduced by SC_COARSE_RED_FACTOR_SCALAR
+      // (opt[11]) or SC_COARSE_MAXIT_SCALAR (opt[10]) sweeps were made
+      coarse_wg<T>(lv.D, v.val.p, v.x.p, v.f.p, o[8], cs, std::max(1, (int)o[10]), o[11], v.d.p, v.aux.p, st);
     } else {
       level_smooth<T>(Y, lv, cs == 17 ? 3 : cs, o[8], std::max(1, (int)o[10]));
     }

@@ -149,6 +149,58 @@ __global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(cons
 #undef PMX_ROW
 }
 
+// The rows of colour c of a level staged in LDS (segment table lseg, colour
+// ranges lcs, each segment's first row in its colour lrow0, rows per colour
+// lrows), R lanes per row, computed by row_split in mode MODE with x in LDS.
+template <typename T, int R, int MODE, int NT>
+__device__ __forceinline__ void wg_colour_rows(const DevSegment *lseg, const int *lcs, const int *lrow0,
+                                               const int *lrows, int c, int M, const T *__restrict__ val,
+                                               const T *lx, const T *__restrict__ f, T *y, double omega) {
+  const int part = threadIdx.x % R;
+  const int s0 = lcs[c], s1 = lcs[c + 1], n = lrows[c];
+  // the R lanes of a row share r, so they run the same trip count
+  for (int r = threadIdx.x / R; r < n; r += NT / R) {
+    int si = s0;
+    while (si + 1 < s1 && r >= lrow0[si + 1]) si++;
+    const DevSegment s = lseg[si];
+    const int idx = r - lrow0[si];
+#define PMX_ROW(EX, EY, EZ) row_split<T, EX, EY, EZ, R, MODE, true>(s, idx, part, M, val, lx, f, y, omega)
+    switch (s.ex * 100 + s.ey * 10 + s.ez) {
+      case 111: PMX_ROW(1, 1, 1); break;
+      case 211: PMX_ROW(2, 1, 1); break;
+      case 121: PMX_ROW(1, 2, 1); break;
+      case 221: PMX_ROW(2, 2, 1); break;
+      case 112: PMX_ROW(1, 1, 2); break;
+      case 212: PMX_ROW(2, 1, 2); break;
+      case 122: PMX_ROW(1, 2, 2); break;
+      default: PMX_ROW(2, 2, 2); break;
+    }
+#undef PMX_ROW
+  }
+}
+// stage the segment table, colour ranges and row offsets, and x with
+// Dirichlet rows x = f, in LDS
+template <typename T, int NT>
+__device__ __forceinline__ void wg_stage(const DevSegment *__restrict__ segs, int nseg,
+                                         const int *__restrict__ colour_seg, int ncol, int N, const T *x,
+                                         const T *__restrict__ f, const int *__restrict__ bnd, int nb,
+                                         DevSegment *lseg, int *lcs, int *lrow0, int *lrows, T *lx) {
+  for (int i = threadIdx.x; i < nseg; i += NT) lseg[i] = segs[i];
+  for (int i = threadIdx.x; i <= ncol; i += NT) lcs[i] = colour_seg[i];
+  for (int i = threadIdx.x; i < N; i += NT) lx[i] = x[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < nb; i += NT) lx[bnd[i]] = f[bnd[i]];
+  for (int c = threadIdx.x; c < ncol; c += NT) {
+    int n = 0;
+    for (int si = lcs[c]; si < lcs[c + 1]; si++) {
+      lrow0[si] = n;
+      n += (int)lseg[si].count;
+    }
+    lrows[c] = n;
+  }
+  __syncthreads();
+}
+
 // A whole SOR (ssor = 0) or SSOR (ssor = 1) sweep of a small level in one
 // workgroup: the level's x is copied into LDS (Dirichlet rows x = f), the
 // colours run one after another (backwards again for SSOR) with a workgroup
@@ -167,52 +219,85 @@ __global__ __launch_bounds__(sweep_threads<T>()) void k_sweep_wg(const DevSegmen
   constexpr int NT = sweep_threads<T>();
   extern __shared__ __align__(16) unsigned char sweep_lds[];
   T *lx = reinterpret_cast<T *>(sweep_lds);
-  // segment table, colour ranges and each segment's first row inside its
-  // colour, staged once so that locating a row costs LDS reads only
   __shared__ DevSegment lseg[kMaxSegs];
   __shared__ int lcs[kMaxColours + 1];
   __shared__ int lrow0[kMaxSegs + 1];
   __shared__ int lrows[kMaxColours];
   const int N = M * M * M;
-  for (int i = threadIdx.x; i < nseg; i += NT) lseg[i] = segs[i];
-  for (int i = threadIdx.x; i <= ncol; i += NT) lcs[i] = colour_seg[i];
-  for (int i = threadIdx.x; i < N; i += NT) lx[i] = x[i];
-  __syncthreads();
-  for (int i = threadIdx.x; i < nb; i += NT) lx[bnd[i]] = f[bnd[i]];
-  for (int c = threadIdx.x; c < ncol; c += NT) {
-    int n = 0;
-    for (int si = lcs[c]; si < lcs[c + 1]; si++) {
-      lrow0[si] = n;
-      n += (int)lseg[si].count;
-    }
-    lrows[c] = n;
-  }
-  __syncthreads();
-  const int part = threadIdx.x % R;
+  wg_stage<T, NT>(segs, nseg, colour_seg, ncol, N, x, f, bnd, nb, lseg, lcs, lrow0, lrows, lx);
   const int nsteps = ssor ? 2 * ncol : ncol;
   for (int step = 0; step < nsteps; step++) {
     const int c = step < ncol ? step : 2 * ncol - 1 - step;
-    const int s0 = lcs[c], s1 = lcs[c + 1], n = lrows[c];
-    // the R lanes of a row share r, so they run the same trip count
-    for (int r = threadIdx.x / R; r < n; r += NT / R) {
-      int si = s0;
-      while (si + 1 < s1 && r >= lrow0[si + 1]) si++;
-      const DevSegment s = lseg[si];
-      const int idx = r - lrow0[si];
-#define PMX_ROW(EX, EY, EZ) row_split<T, EX, EY, EZ, R, 3, true>(s, idx, part, M, val, lx, f, lx, omega)
-      switch (s.ex * 100 + s.ey * 10 + s.ez) {
-        case 111: PMX_ROW(1, 1, 1); break;
-        case 211: PMX_ROW(2, 1, 1); break;
-        case 121: PMX_ROW(1, 2, 1); break;
-        case 221: PMX_ROW(2, 2, 1); break;
-        case 112: PMX_ROW(1, 1, 2); break;
-        case 212: PMX_ROW(2, 1, 2); break;
-        case 122: PMX_ROW(1, 2, 2); break;
-        default: PMX_ROW(2, 2, 2); break;
-      }
-#undef PMX_ROW
-    }
+    wg_colour_rows<T, R, 3, NT>(lseg, lcs, lrow0, lrows, c, M, val, lx, f, lx, omega);
     __syncthreads();
+  }
+  for (int i = threadIdx.x; i < N; i += NT) x[i] = lx[i];
+}
+
+// ||f - A x|| of the LDS iterate over all rows (Dirichlet rows: 0), the
+// same value in every thread.  d: global scratch of N entries whose
+// Dirichlet entries hold 0.
+template <typename T, int R, int NT>
+__device__ double wg_defect_norm(const DevSegment *lseg, const int *lcs, const int *lrow0, const int *lrows,
+                                 int ncol, int M, const T *__restrict__ val, const T *lx, const T *__restrict__ f,
+                                 T *d, double *lred) {
+  const int N = M * M * M;
+  for (int c = 0; c < ncol; c++) wg_colour_rows<T, R, 1, NT>(lseg, lcs, lrow0, lrows, c, M, val, lx, f, d, 0.0);
+  __syncthreads();
+  double s = 0;
+  for (int i = threadIdx.x; i < N; i += NT) s += (double)d[i] * (double)d[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (threadIdx.x % 64 == 0) lred[threadIdx.x / 64] = s;
+  __syncthreads();
+  double t = 0;
+  for (int w = 0; w < NT / 64; w++) t += lred[w];
+  __syncthreads();
+  return sqrt(t);
+}
+// The coarse-level loop of TMultiGrid3D::Smooth(CoarseSmooth)
+// (MultiGrid3D.C:225-337) in one workgroup: res = ||f - A x||, then while
+// res > red * res_0 and fewer than maxit sweeps: one Jacobi (kind 1, omega
+// damped), SOR (2) or SSOR (3) sweep and res again.  The sweeps compute each
+// row as the launched sweeps do (bit-identical iterates); d and xj are
+// global scratch of N entries (defect, Jacobi output).
+template <typename T, int R>
+__global__ __launch_bounds__(sweep_threads<T>()) void k_coarse_wg(const DevSegment *__restrict__ segs, int nseg,
+                                                             const int *__restrict__ colour_seg, int ncol, int M,
+                                                             const T *__restrict__ val, T *x,
+                                                             const T *__restrict__ f, double omega, int kind,
+                                                             int maxit, double red, const int *__restrict__ bnd,
+                                                             int nb, T *d, T *xj) {
+  constexpr int NT = sweep_threads<T>();
+  extern __shared__ __align__(16) unsigned char sweep_lds[];
+  T *lx = reinterpret_cast<T *>(sweep_lds);
+  __shared__ DevSegment lseg[kMaxSegs];
+  __shared__ int lcs[kMaxColours + 1];
+  __shared__ int lrow0[kMaxSegs + 1];
+  __shared__ int lrows[kMaxColours];
+  __shared__ double lred[NT / 64];
+  const int N = M * M * M;
+  for (int i = threadIdx.x; i < nb; i += NT) {
+    d[bnd[i]] = T(0);
+    xj[bnd[i]] = f[bnd[i]];
+  }
+  wg_stage<T, NT>(segs, nseg, colour_seg, ncol, N, x, f, bnd, nb, lseg, lcs, lrow0, lrows, lx);
+  double res = wg_defect_norm<T, R, NT>(lseg, lcs, lrow0, lrows, ncol, M, val, lx, f, d, lred);
+  const double reduction = red * res;
+  for (int it = 0; res > reduction && it < maxit; it++) {
+    if (kind == 1) {
+      for (int c = 0; c < ncol; c++) wg_colour_rows<T, R, 2, NT>(lseg, lcs, lrow0, lrows, c, M, val, lx, f, xj, omega);
+      __syncthreads();
+      for (int i = threadIdx.x; i < N; i += NT) lx[i] = xj[i];
+      __syncthreads();
+    } else {
+      const int nsteps = kind == 3 ? 2 * ncol : ncol;
+      for (int step = 0; step < nsteps; step++) {
+        const int c = step < ncol ? step : 2 * ncol - 1 - step;
+        wg_colour_rows<T, R, 3, NT>(lseg, lcs, lrow0, lrows, c, M, val, lx, f, lx, omega);
+        __syncthreads();
+      }
+    }
+    res = wg_defect_norm<T, R, NT>(lseg, lcs, lrow0, lrows, ncol, M, val, lx, f, d, lred);
   }
   for (int i = threadIdx.x; i < N; i += NT) x[i] = lx[i];
 }
@@ -957,6 +1042,24 @@ void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, 
 #undef PMX_SWEEP
 }
 template <typename T>
+void coarse_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int kind, int maxit, double red,
+               T *d, T *xj, hipStream_t st) {
+  if (D.nsegs > kMaxSegs || D.ncolours > kMaxColours) throw std::logic_error("coarse_wg: segment table too large");
+  if (!sweep_wg_fits(D, sizeof(T))) throw std::logic_error("coarse_wg: level does not fit in LDS");
+  if (kind < 1 || kind > 3) throw std::invalid_argument("coarse_wg: smoother must be 1, 2 or 3");
+  const size_t lds = (size_t)D.N * sizeof(T);
+#define PMX_COARSE(R)                                                                                           \
+  hipLaunchKernelGGL((k_coarse_wg<T, R>), dim3(1), dim3(sweep_threads<T>()), lds, st, D.segs, D.nsegs,           \
+                     D.colour_seg_dev, D.ncolours, D.M, val, x, f, omega, kind, maxit, red, D.boundary,          \
+                     D.nboundary, d, xj)
+  switch (row_lanes(D)) {
+    case 1: PMX_COARSE(1); break;
+    case 2: PMX_COARSE(2); break;
+    default: PMX_COARSE(4); break;
+  }
+#undef PMX_COARSE
+}
+template <typename T>
 void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st) {
   const int g = (D.nboundary + kBlock - 1) / kBlock;
   switch (mode) {
@@ -1389,6 +1492,8 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
                              hipStream_t);                                                                  \
   template void boundary<T>(const DevLayout &, int, T *, const T *, const T *, hipStream_t);                \
   template void sweep_wg<T>(const DevLayout &, const T *, T *, const T *, double, int, hipStream_t);          \
+  template void coarse_wg<T>(const DevLayout &, const T *, T *, const T *, double, int, int, double, T *, T *, \
+                             hipStream_t);                                                                  \
   template void prolongate<T>(const DevLayout &, const DevLayout &, const T *, T *, hipStream_t);           \
   template void prolongate_add<T>(const DevLayout &, const DevLayout &, const T *, T *, double, hipStream_t); \
   template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t, bool);      \

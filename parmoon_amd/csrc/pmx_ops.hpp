@@ -79,6 +79,13 @@ inline int row_lanes(const DevLayout &D) {
 }
 template <typename T>
 void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st);
+// the coarse-level smoothing loop of TMultiGrid3D (MultiGrid3D.C:225-337) in
+// one workgroup (levels with sweep_wg_fits): sweeps of kind 1 Jacobi, 2 SOR,
+// 3 SSOR while ||f - A x|| > red ||f - A x_0|| and fewer than maxit were
+// made; d, xj: scratch of D.N entries
+template <typename T>
+void coarse_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int kind, int maxit, double red,
+               T *d, T *xj, hipStream_t st);
 // Dirichlet rows: mode 0 y=x, 1 y=f-x, 2 y=f, 3 y=0
 template <typename T> void boundary(const DevLayout &D, int mode, T *y, const T *x, const T *f, hipStream_t st);
 template <typename T>

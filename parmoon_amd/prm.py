@@ -129,7 +129,7 @@ def rotating(seed=0, width=0.05, form=FORM_TCD_A):
 
 
 def solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-10, red_factor=0.0, smoother=1,
-               pre=2, post=2, omega=0.8, coarse=17, coarse_sweeps=10, cycle=1, gmg_damp=1.0,
+               pre=2, post=2, omega=0.8, coarse=17, coarse_sweeps=10, coarse_red=0.1, cycle=1, gmg_damp=1.0,
                bicg_flexible=1, precision=0, rel_target=0.0, inner_red=1e-4, eager=0, learned_levels=0,
                learned_steps=1, wg_sweep_rows=0, learned_skip=0, learned_pre=0):
     """The 24-entry solver option vector; entries 0-15 are shared with the oracle.
@@ -139,6 +139,6 @@ def solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-10, red_factor=
     smoother; learned_pre: the learned steps also replace pre-smoothing."""
     o = np.zeros(24)
     o[:24] = [solver, restart, maxit, res_norm_min, red_factor, smoother, pre, post, omega, coarse,
-              coarse_sweeps, 0.1, cycle, gmg_damp, bicg_flexible, precision, rel_target, inner_red, eager,
+              coarse_sweeps, coarse_red, cycle, gmg_damp, bicg_flexible, precision, rel_target, inner_red, eager,
               learned_levels, learned_steps, wg_sweep_rows, learned_skip, learned_pre]
     return o

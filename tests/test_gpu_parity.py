@@ -346,3 +346,32 @@ def test_step_length_control_cycle(request, which, mode):
     assert rc == 0
     assert rel(s, ref) < 1e-12, rel(s, ref)
     assert rel(plain, ref) > 1e-6
+
+
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+@pytest.mark.parametrize("coarse", [1, 3])
+def test_iterative_coarse_solve(request, which, coarse):
+    """Coarsest level smoothed until its defect drops by SC_COARSE_RED_FACTOR
+    (0.1) or SC_COARSE_MAXIT (10) sweeps (TMultiGrid3D CoarseSmooth).  Damped
+    Jacobi everywhere: one cycle equals the oracle's (the sweeps do not depend
+    on the row order); SSOR on the coarsest level: the FGMRES solutions (to a
+    relative residual of 1e-10) agree to 1e-8."""
+    S, O, prm = request.getfixturevalue(which)
+    print(f"[case] iterative coarse solve {which} coarse {coarse}", flush=True)
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    x0 = np.zeros(fine.N)
+    x0[fine.active:] = rhs[fine.active:]
+    if coarse == 1:
+        opt = P.solver_opt(solver=0, smoother=1, omega=0.8, pre=2, post=2, coarse=1, coarse_sweeps=10)
+        ref, _ = O.solve(opt, rhs, x0)
+        exact, _ = O.solve(P.solver_opt(solver=0, smoother=1, omega=0.8, pre=2, post=2), rhs, x0)
+        s, info, rc = S.solve(opt, rhs, x0)
+        assert rc == 0 and rel(s, ref) < 1e-12, rel(s, ref)
+        assert rel(exact, ref) > 1e-8
+    else:
+        opt = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=1e-10, smoother=3,
+                           omega=1.0, coarse=3, coarse_sweeps=10)
+        ref, _ = O.solve(opt, rhs, x0)
+        s, info, rc = S.solve(opt, rhs, x0)
+        assert rc == 0 and rel(s, ref) < 1e-8, rel(s, ref)
