@@ -109,7 +109,7 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *      13 BiCGStab (Bcgs.C:111), 0 = one MG cycle
  *  [1] restart  [2] maxit  [3] res_norm_min (absolute)  [4] red_factor
  *  [5] smoother 1 Jacobi 2 SOR 3 SSOR  [6] pre  [7] post  [8] omega
- *  [9] coarse: 17 exact (FP64 dense inverse), 1/2/3 = Jacobi/SOR/SSOR sweeps
+ *  [9] coarse: 17 exact (FP64 dense inverse), 1/2 = Jacobi/SOR, other = SSOR sweeps
  *      until the coarse defect is reduced by [11] or [10] sweeps were made
  *      (MultiGrid3D.C:225-337; one workgroup where the level fits its LDS)
  *  [10] coarse maxit (SC_COARSE_MAXIT_SCALAR)

@@ -913,8 +913,9 @@ static void mg_cycle(System &Y, int i) {
   const double *o = Y.opt;
   const Space &S = Y.lev[i];
   if (i == 0) {
-    // CoarseSmooth (MultiGrid3D.C:225-337)
-    const int cs = (int)o[9];
+    // CoarseSmooth (MultiGrid3D.C:225-337); ids other than 1, 2, 17 take the
+    // switch's default branch, SSOR (as 6 does in the sequential build)
+    const int cs = ((int)o[9] == 1 || (int)o[9] == 2 || (int)o[9] == 17) ? (int)o[9] : 3;
     double res = defect(S, Y.E[0].data(), Y.sol[0].data(), Y.f[0].data(), Y.aux0[0].data());
     const double red = o[11] * res;
     int it = 0;

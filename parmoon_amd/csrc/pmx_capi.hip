@@ -478,7 +478,10 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   Level &lv = *Y.lev[i];
   Vecs<T> &v = vecs<T>(lv);
   if (i == 0) {
-    const int cs = (int)o[9];
+    // coarse smoother ids other than 1, 2, 17 take the default branch of
+    // MultiGrid3D.C's CoarseSmooth switch, SSOR (as 6 does in the
+    // sequential build)
+    const int cs = ((int)o[9] == 1 || (int)o[9] == 2 || (int)o[9] == 17) ? (int)o[9] : 3;
     if (cs == 17 && lv.ainv_n > 0) {
       gemv_coarse<T>(lv.ainv_n, (int)lv.L.N, lv.ainv.p, lv.iidx.p, v.f.p, v.x.p, st);
       boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);

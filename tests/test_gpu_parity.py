@@ -349,13 +349,14 @@ def test_step_length_control_cycle(request, which, mode):
 
 
 @pytest.mark.parametrize("which", ["q1sys", "q2sys"])
-@pytest.mark.parametrize("coarse", [1, 3])
+@pytest.mark.parametrize("coarse", [1, 3, 6])
 def test_iterative_coarse_solve(request, which, coarse):
     """Coarsest level smoothed until its defect drops by SC_COARSE_RED_FACTOR
     (0.1) or SC_COARSE_MAXIT (10) sweeps (TMultiGrid3D CoarseSmooth).  Damped
     Jacobi everywhere: one cycle equals the oracle's (the sweeps do not depend
-    on the row order); SSOR on the coarsest level: the FGMRES solutions (to a
-    relative residual of 1e-10) agree to 1e-8."""
+    on the row order); SSOR on the coarsest level (id 3, or 6, which the
+    switch's default turns into SSOR): the FGMRES solutions (to a relative
+    residual of 1e-10) agree to 1e-8."""
     S, O, prm = request.getfixturevalue(which)
     print(f"[case] iterative coarse solve {which} coarse {coarse}", flush=True)
     rhs = O.rhs()
@@ -371,7 +372,7 @@ def test_iterative_coarse_solve(request, which, coarse):
         assert rel(exact, ref) > 1e-8
     else:
         opt = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=1e-10, smoother=3,
-                           omega=1.0, coarse=3, coarse_sweeps=10)
+                           omega=1.0, coarse=coarse, coarse_sweeps=10)
         ref, _ = O.solve(opt, rhs, x0)
         s, info, rc = S.solve(opt, rhs, x0)
         assert rc == 0 and rel(s, ref) < 1e-8, rel(s, ref)
