@@ -376,3 +376,25 @@ def test_iterative_coarse_solve(request, which, coarse):
         ref, _ = O.solve(opt, rhs, x0)
         s, info, rc = S.solve(opt, rhs, x0)
         assert rc == 0 and rel(s, ref) < 1e-8, rel(s, ref)
+
+
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+def test_reference_input_file_solver(request, which):
+    """The solver settings of the reference's ReadIn/Input_CD3D.dat: fixed
+    point iteration (SC_SOLVER_SCALAR 11) preconditioned by a V-cycle with
+    smoother id 6 (the switch's default, SOR) and 5 pre- and post-sweeps,
+    coarse id 6 (SSOR) with at most 10 sweeps and reduction 0.1; solved to a
+    relative residual of 1e-10, the solution agrees with the oracle's run of
+    the same settings."""
+    S, O, prm = request.getfixturevalue(which)
+    print(f"[case] Input_CD3D.dat solver settings {which}", flush=True)
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    x0 = np.zeros(fine.N)
+    x0[fine.active:] = rhs[fine.active:]
+    opt = P.solver_opt(solver=11, maxit=200, res_norm_min=0.0, red_factor=1e-10, smoother=6, pre=5, post=5,
+                       omega=1.0, coarse=6, coarse_sweeps=10, coarse_red=0.1)
+    ref, oinfo = O.solve(opt, rhs, x0)
+    s, info, rc = S.solve(opt, rhs, x0)
+    assert rc == 0 and oinfo[1] <= 1e-10 * oinfo[2]
+    assert rel(s, ref) < 1e-8, rel(s, ref)
