@@ -90,6 +90,7 @@ struct Level {
   int ainv_n = 0;
   // learned post-smoother: 1/diag, correction and its image under A (FP32)
   DevBuf<float> dinv, lc, lac;
+  DevBuf<float> valT;  // transposed FP32 stencil values (residual-loss training)
   DevBuf<uint16_t> feat;              // network input features [N][8] (BF16)
   bool have_dinv = false;
 };
@@ -817,8 +818,15 @@ static int krylov(System &Y, T *x, const T *b, int maxit, double rmin, double re
 // u (FP32 lattice): c = mask(net(u)), loss = mean over interior nodes of
 // (u - D^-1 A c)^2 -- the part of the scaled defect that the correction
 // leaves.  Accumulates into net.grads; the loss goes to *loss_dev.
+// the level's transposed FP32 values for learned_res_step (from the current
+// FP32 values; call after they change)
+static void learned_transpose(System &Y, Level &lv) {
+  lv.valT.alloc(lv.L.nvals);
+  css_transpose_values(lv.D, lv.v32.val.p, lv.valT.p, Y.ctx->st);
+}
 static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float *u, float *loss_dev) {
   hipStream_t st = Y.ctx->st;
+  if (!lv.valT.p) throw std::logic_error("learned_res_step: transposed values missing");
   const int M = lv.L.M;
   Vecs<float> &v = lv.v32;
   learned_build_input(net, u, nullptr, 1.0, lv.feat.p, lv.L.N, st);
@@ -826,7 +834,7 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
   learned_mask(net.out, M, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   learned_res_loss(net, M, u, lv.dinv.p, lv.lac.p, v.aux.p, st);
-  css_transpose(lv.D, v.val.p, v.aux.p, v.d.p, st);
+  css_transpose(lv.D, lv.valT.p, v.aux.p, v.d.p, st);
   net_backward_from(net, M, v.d.p, st);
   learned_loss_sum(net, loss_dev, st);
 }
@@ -907,6 +915,8 @@ static void learned_train(System &Y, const double *to, double *ti) {
       if (!residual_loss) learned_target(v.x.p, M, Y.lscal.p, sq, -1.0f, dt.p + off[j] + (size_t)k * N, st);
     }
   }
+  if (residual_loss)
+    for (int j = j0; j < nl; j++) learned_transpose(Y, *Y.lev[L - j]);
   for (int k = 0; k < total; k++) {
     const int j = j0 + k % nt, kk = k / nt, smp = kk % ns;
     Level &lv = *Y.lev[L - j];
@@ -1671,6 +1681,7 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
     din.upload(in, lv.L.N, st);
     dl.alloc(1);
     net_zero_grads(net, st);
+    learned_transpose(Y, lv);
     learned_res_step(Y, lv, net, din.p, dl.p);
     if (grads)
       check(hipMemcpyAsync(grads, net.grads, sizeof(float) * net_total_params(), hipMemcpyDeviceToHost, st),

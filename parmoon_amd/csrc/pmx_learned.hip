@@ -922,60 +922,55 @@ __global__ __launch_bounds__(256) void k_res_loss(int M, const float *__restrict
   if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1] + sh[2] + sh[3]) * inv_count;
 }
 // y_j = sum over interior rows i of a_ij g_i, for interior j (0 on the boundary):
-// the transpose of the stored stencil operator restricted to interior nodes:
-// y_j = sum_i a_ij g_i over the interior rows i = j + d whose stencil holds
-// offset -d.  A workgroup takes 64 columns j of one residue class, so the
-// class of every neighbour row j + d, its stencil shape, the slot of -d and
-// the shift of the class-lattice indices are the same across the workgroup
-// (scalar work, S a compile-time constant); per lane only the neighbour's row
-// index and the bounds remain, without divisions.  Boundary entries of y are
-// set by the caller.
+// The transpose of the stored stencil operator restricted to interior rows,
+// stored in the same CSS layout: slot d of row j holds a_{j+d, j}, the entry
+// at offset -d of the interior row j + d (0 where j + d is not an interior
+// row or its stencil box lacks -d).  The FE pattern is structurally
+// symmetric, so every such entry has its slot in row j's box.  A workgroup
+// takes 64 rows j of one residue class, so the class of every neighbour row
+// j + d, its stencil shape, the slot of -d and the shift of the
+// class-lattice indices are the same across the workgroup (scalar work, S a
+// compile-time constant); per lane only the neighbour's row index and the
+// bounds remain.
 template <int E, int S>
-__global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose(const BlockInfo *__restrict__ blocks,
-                                                                 const DevSegment *__restrict__ segs,
-                                                                 const int *__restrict__ segidx, int M,
-                                                                 const float *__restrict__ val,
-                                                                 const float *__restrict__ g, float *__restrict__ y) {
+__global__ __launch_bounds__(kRowsPerBlock) void k_css_transpose_values(const BlockInfo *__restrict__ blocks,
+                                                                        const DevSegment *__restrict__ segs,
+                                                                        const int *__restrict__ segidx,
+                                                                        const float *__restrict__ val,
+                                                                        float *__restrict__ valT) {
   const BlockInfo bi = blocks[blockIdx.x];
   const DevSegment sj = segs[bi.seg];
   const int idx = bi.base + threadIdx.x;
-  const bool valid = idx < sj.count;
+  if (idx >= sj.count) return;
   const int rx = sj.x0 % S, ry = sj.y0 % S, rz = sj.z0 % S;
-  int i = 0, j = 0, k = 0;
-  if (valid) {
-    i = idx % sj.cx;
-    const int t = idx / sj.cx;
-    j = t % sj.cy;
-    k = t / sj.cy;
-  }
-  const int64_t p = (sj.x0 + S * i) + (int64_t)M * ((sj.y0 + S * j) + (int64_t)M * (sj.z0 + S * k));
-  float acc = 0.f;
+  const int i = idx % sj.cx, t = idx / sj.cx, j = t % sj.cy, k = t / sj.cy;
+  const int64_t vbj = sj.voff + (int64_t)(idx / kSlice) * ((int64_t)sj.ns * kSlice) + (idx % kSlice);
 #pragma unroll
   for (int dz = -E; dz <= E; dz++)
 #pragma unroll
     for (int dy = -E; dy <= E; dy++)
 #pragma unroll
       for (int dx = -E; dx <= E; dx++) {
+        if (dx < -sj.ex || dx > sj.ex || dy < -sj.ey || dy > sj.ey || dz < -sj.ez || dz > sj.ez) continue;
+        const int slot_j = (dx + sj.ex) + (2 * sj.ex + 1) * ((dy + sj.ey) + (2 * sj.ey + 1) * (dz + sj.ez));
+        float v = 0.f;
         const int si = segidx[((rx + dx + S) % S) + S * (((ry + dy + S) % S) + S * ((rz + dz + S) % S))];
-        if (si < 0) continue;
-        const DevSegment s = segs[si];
-        // column j sits at offset -d of row i = j + d
-        if (dx < -s.ex || dx > s.ex || dy < -s.ey || dy > s.ey || dz < -s.ez || dz > s.ez) continue;
-        const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
-        // class-lattice index of j + d in the neighbour's class (exact division)
-        const int ti = i + (sj.x0 + dx - s.x0) / S, tj = j + (sj.y0 + dy - s.y0) / S,
-                  tk = k + (sj.z0 + dz - s.z0) / S;
-        // loads are unconditional (row 0 of the class and node p when j + d is
-        // not an interior row, with weight 0), so the wavefront issues them
-        // without waiting per offset
-        const bool ok = valid && ti >= 0 && ti < s.cx && tj >= 0 && tj < s.cy && tk >= 0 && tk < s.cz;
-        const int ri = ok ? ti + s.cx * (tj + s.cy * tk) : 0;
-        const int64_t vb = s.voff + (int64_t)(ri / kSlice) * ((int64_t)s.ns * kSlice) + (ri % kSlice);
-        const int64_t gi = ok ? p + dx + (int64_t)M * (dy + (int64_t)M * dz) : 0;
-        const float prod = val[vb + (int64_t)slot * kSlice] * g[gi];
-        acc += ok ? prod : 0.f;
+        if (si >= 0) {
+          const DevSegment s = segs[si];
+          if (dx >= -s.ex && dx <= s.ex && dy >= -s.ey && dy <= s.ey && dz >= -s.ez && dz <= s.ez) {
+            const int slot = (-dx + s.ex) + (2 * s.ex + 1) * ((-dy + s.ey) + (2 * s.ey + 1) * (-dz + s.ez));
+            // class-lattice index of j + d in the neighbour's class (exact division)
+            const int ti = i + (sj.x0 + dx - s.x0) / S, tj = j + (sj.y0 + dy - s.y0) / S,
+                      tk = k + (sj.z0 + dz - s.z0) / S;
+            if (ti >= 0 && ti < s.cx && tj >= 0 && tj < s.cy && tk >= 0 && tk < s.cz) {
+              const int ri = ti + s.cx * (tj + s.cy * tk);
+              v = val[s.voff + (int64_t)(ri / kSlice) * ((int64_t)s.ns * kSlice) + (ri % kSlice) +
+                      (int64_t)slot * kSlice];
+            }
+          }
+        }
+        valT[vbj + (int64_t)slot_j * kSlice] = v;
       }
-  if (valid) y[p] = acc;
 }
 
 void learned_mask(const float *in, int M, float *out, hipStream_t st) {
@@ -990,14 +985,17 @@ void learned_res_loss(LearnedNet &net, int M, const float *u, const float *dinv,
 void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st) {
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, st, net.lpart, kLossBlocks, loss_dev);
 }
-void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st) {
-  if (D.S != (D.order == 2 ? 6 : 2)) throw std::logic_error("css_transpose: unexpected residue modulus");
+void css_transpose_values(const DevLayout &D, const float *val, float *valT, hipStream_t st) {
+  if (D.S != (D.order == 2 ? 6 : 2)) throw std::logic_error("css_transpose_values: unexpected residue modulus");
   if (D.order == 2)
-    hipLaunchKernelGGL((k_css_transpose<2, 6>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,
-                       D.segs, D.segidx, D.M, val, g, y);
+    hipLaunchKernelGGL((k_css_transpose_values<2, 6>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,
+                       D.blocks_all, D.segs, D.segidx, val, valT);
   else
-    hipLaunchKernelGGL((k_css_transpose<1, 2>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,
-                       D.segs, D.segidx, D.M, val, g, y);
+    hipLaunchKernelGGL((k_css_transpose_values<1, 2>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,
+                       D.blocks_all, D.segs, D.segidx, val, valT);
+}
+void css_transpose(const DevLayout &D, const float *valT, const float *g, float *y, hipStream_t st) {
+  css_apply<float>(D, 0, valT, g, nullptr, y, 0.0, -1, st);
   boundary<float>(D, 3, y, nullptr, nullptr, st);  // y = 0 on the boundary nodes
 }
 

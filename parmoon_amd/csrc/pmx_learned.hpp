@@ -118,8 +118,12 @@ void learned_mask(const float *in, int M, float *out, hipStream_t st);
 void learned_res_loss(LearnedNet &net, int M, const float *u, const float *dinv, const float *ac, float *g,
                       hipStream_t st);
 void learned_loss_sum(LearnedNet &net, float *loss_dev, hipStream_t st);
-// y = transpose of the interior stencil operator applied to g (0 on the boundary)
-void css_transpose(const DevLayout &D, const float *val, const float *g, float *y, hipStream_t st);
+// valT = the interior stencil operator's transpose in the CSS layout of val
+// (slot d of row j: a_{j+d, j}; D.nvals entries)
+void css_transpose_values(const DevLayout &D, const float *val, float *valT, hipStream_t st);
+// y = transpose of the interior stencil operator applied to g (0 on the
+// boundary), valT from css_transpose_values: a stencil product
+void css_transpose(const DevLayout &D, const float *valT, const float *g, float *y, hipStream_t st);
 // uniform(-1, 1) values on interior nodes of an M^3 lattice, 0 on the boundary
 void learned_random(float *x, int M, uint32_t seed, hipStream_t st);
 // t = sign * x * sqrt_count / *nrm on interior nodes, 0 on the boundary
