@@ -257,7 +257,7 @@ def main():
     ap.add_argument("--eval-seeds", type=int, default=4)
     args = ap.parse_args()
     if args.transient > 0:
-        dflt = dict(learned_levels=1, learned_skip=0, learned_steps=1, learned_pre=0, train_steps=1200)
+        dflt = dict(learned_levels=1, learned_skip=0, learned_steps=3, learned_pre=0, train_steps=1200)
     elif args.train_set > 0:
         dflt = dict(learned_levels=0, learned_skip=1, learned_steps=2, learned_pre=1, train_steps=16)
     else:
