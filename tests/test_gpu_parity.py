@@ -346,6 +346,8 @@ def test_step_length_control_cycle(request, which, mode):
     assert rc == 0
     assert rel(s, ref) < 1e-12, rel(s, ref)
     assert rel(plain, ref) > 1e-6
+    with pytest.raises(pa.PmxError):
+        S.set_step_length_control(3)
 
 
 @pytest.mark.parametrize("which", ["q1sys", "q2sys"])
