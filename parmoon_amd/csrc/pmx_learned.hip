@@ -239,8 +239,19 @@ __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y,
 // (outputs padded to 16 with zeros) and the transposed, tap-flipped weights
 // wt[16][27][cout] of the input-gradient convolution: wt[c][t][o] = W[o][26-t][c]
 // (tap 26-t is the opposite offset of tap t), rows c >= cin zero.
-__global__ void k_make_bf16_weights(const float *__restrict__ W, int cin, int cout, uint16_t *__restrict__ wb,
-                                    uint16_t *__restrict__ wt) {
+// per layer: offset of W in the parameter vector, cin, cout
+struct LayerShapes {
+  int off[kNetLayers], cin[kNetLayers], cout[kNetLayers];
+};
+// BF16 copies of every layer's weights (blockIdx.y = layer): wb = W padded to
+// 16 outputs [16][27][cin], wt = transposed, tap-flipped [16][27][cout]
+__global__ void k_make_bf16_weights(const float *__restrict__ params, LayerShapes ls, uint16_t *__restrict__ wbf,
+                                    uint16_t *__restrict__ wtbf, int wstride) {
+  const int l = blockIdx.y;
+  const float *__restrict__ W = params + ls.off[l];
+  const int cin = ls.cin[l], cout = ls.cout[l];
+  uint16_t *__restrict__ wb = wbf + (int64_t)l * wstride;
+  uint16_t *__restrict__ wt = wtbf + (int64_t)l * wstride;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 16 * kTaps * cin) {
     const int c = i % cin, t = (i / cin) % kTaps, o = i / (cin * kTaps);
@@ -490,10 +501,14 @@ void net_init(LearnedNet &net, uint32_t seed, hipStream_t st) {
 }
 
 void net_sync_bf16(LearnedNet &net, hipStream_t st) {
-  for (int l = 0; l < kNetLayers; l++)
-    hipLaunchKernelGGL(k_make_bf16_weights, dim3((kWStride + 255) / 256), dim3(256), 0, st,
-                       net.params + net_param_offset(l), net_cin(l), net_cout(l), net.wbf + l * kWStride,
-                       net.wtbf + l * kWStride);
+  LayerShapes ls;
+  for (int l = 0; l < kNetLayers; l++) {
+    ls.off[l] = net_param_offset(l);
+    ls.cin[l] = net_cin(l);
+    ls.cout[l] = net_cout(l);
+  }
+  hipLaunchKernelGGL(k_make_bf16_weights, dim3((kWStride + 255) / 256, kNetLayers), dim3(256), 0, st, net.params, ls,
+                     net.wbf, net.wtbf, kWStride);
 }
 
 bool net_reserve(LearnedNet &net, int64_t N, bool training) {
