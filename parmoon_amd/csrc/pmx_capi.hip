@@ -437,17 +437,22 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
 // alpha = max(0, (d, Ac) / (Ac, Ac)), so the level defect never grows in the
 // 2-norm (the step-length idea of TMGLevel3D::StepLengthControl,
 // MGLevel3D.C:908-960, applied to the network's correction).
-static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = true, bool carry = false) {
+static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = true, bool carry = false,
+                         bool x_zero = false) {
   hipStream_t st = Y.ctx->st;
   Vecs<float> &v = lv.v32;
   const int64_t N = lv.L.N;
   double *sc = Y.lscal.p, *part = Y.ctx->part.p;
   const double sq = std::sqrt((double)lv.L.nactive);
-  if (fresh) {
+  // x_zero (a fresh step on x = 0): the defect is f itself, bit for bit
+  const float *d0 = fresh && x_zero ? v.f.p : v.d.p;
+  if (fresh && !x_zero) {
     // defect of the interior rows; its Dirichlet entries meet zero weights
     // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
     css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
-    learned_scaled_input(net, lv.dinv.p, v.d.p, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
+  }
+  if (fresh) {
+    learned_scaled_input(net, lv.dinv.p, d0, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
   } else {
     // d, u = D^-1 d and the partials of ||u||^2 were left by the previous step
     learned_carried_input(net, lv.lc.p, N, part + 2 * kRedBlocks, sc, sq, lv.feat.p, st);
@@ -455,16 +460,19 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = tru
   net_forward_correction(net, lv.L.M, sc, sq, lv.lc.p, st);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   if (carry)
-    learned_update_carry(v.d.p, lv.lac.p, lv.lc.p, v.x.p, lv.dinv.p, lv.lc.p, N, part, part + 2 * kRedBlocks, st);
+    learned_update_carry(d0, v.d.p, lv.lac.p, lv.lc.p, v.x.p, lv.dinv.p, lv.lc.p, N, part, part + 2 * kRedBlocks,
+                         st);
   else
-    learned_update(v.d.p, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
+    learned_update(d0, lv.lac.p, lv.lc.p, v.x.p, N, part, st);
 }
 // `steps` learned steps on a level: the first from the level's defect, the
 // next ones from the defect the previous step updates (d - alpha A c: the
 // same defect to rounding, without a product with A and a pass over d);
 // carry_last: the last step leaves the updated defect too (interior rows)
-static void learned_smooth(System &Y, Level &lv, LearnedNet &net, int steps, bool carry_last = false) {
-  for (int k = 0; k < steps; k++) learned_post(Y, lv, net, k == 0, carry_last || k + 1 < steps);
+// x_zero: the level's x is 0 (a coarse level entered after its Reset)
+static void learned_smooth(System &Y, Level &lv, LearnedNet &net, int steps, bool carry_last = false,
+                           bool x_zero = false) {
+  for (int k = 0; k < steps; k++) learned_post(Y, lv, net, k == 0, carry_last || k + 1 < steps, k == 0 && x_zero);
 }
 
 // level i smooths with the network in this cycle
@@ -473,7 +481,8 @@ static bool learned_level(const System &Y, int i) {
   return Y.learn_active && i >= n - Y.learned_levels && i < n - Y.learned_skip;
 }
 template <typename T>
-static void mg_cycle(System &Y, int i, bool post = true) {
+// x_zero: x of level i is 0 on entry (the first visit after the Reset)
+static void mg_cycle(System &Y, int i, bool post = true, bool x_zero = false) {
   hipStream_t st = Y.ctx->st;
   const double *o = Y.opt;
   Level &lv = *Y.lev[i];
@@ -511,7 +520,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   if constexpr (std::is_same<T, float>::value) {
     if (Y.learned_pre && learned_level(Y, i)) {
       // the steps leave the defect of their last update in v.d
-      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]), true);
+      learned_smooth(Y, lv, Y.nets[i], std::max(1, (int)o[20]), true, x_zero);
       pre_done = true;
     }
   }
@@ -525,7 +534,7 @@ static void mg_cycle(System &Y, int i, bool post = true) {
   // (CorrectDefect)
   restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st, true);
   fill<T>(cv.x.p, (T)0, cl.L.N, st);                   // Reset
-  for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1);
+  for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1, true, j == 0);
   if ((int)o[12] < 1) Y.rec[i] = 1;
   // prolongation, CorrectNodes and x += SC_GMG_DAMP e
   prolongate_add<T>(cl.D, lv.D, cv.x.p, v.x.p, o[13], st);

@@ -808,11 +808,12 @@ __global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict
     x[i] += af * c[i];
 }
 // the step of k_learned_step_parts, and from the updated defect the next
-// step's input: d -= alpha Ac, u = dinv d (u may be c itself: each thread
-// reads c[i] before it writes u[i]) and the partials of ||u||^2 in
-// k_learned_input_sq's order (same grid)
+// step's input: d = d_in - alpha Ac (d_in may be d), u = dinv d (u may be c
+// itself: each thread reads c[i] before it writes u[i]) and the partials of
+// ||u||^2 in k_learned_input_sq's order (same grid)
 __global__ __launch_bounds__(kBlock) void k_learned_step_carry(float *__restrict__ x, const float *c,
-                                                               float *__restrict__ d, const float *__restrict__ ac,
+                                                               const float *d_in, float *d,
+                                                               const float *__restrict__ ac,
                                                                const float *__restrict__ dinv, float *u,
                                                                const double *__restrict__ pab,
                                                                const double *__restrict__ pbb, int np,
@@ -832,7 +833,7 @@ __global__ __launch_bounds__(kBlock) void k_learned_step_carry(float *__restrict
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const float ci = c[i];
     x[i] += af * ci;
-    const float dn = d[i] - af * ac[i];
+    const float dn = d_in[i] - af * ac[i];
     d[i] = dn;
     const float v = dinv[i] * dn;
     u[i] = v;
@@ -877,11 +878,11 @@ void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const dou
   hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, pu, (int)red_grid(n), sq, feat,
                      n, net.act[0], nrm);
 }
-void learned_update_carry(float *d, const float *ac, const float *c, float *x, const float *dinv, float *u, int64_t n,
-                          double *part, double *pu, hipStream_t st) {
+void learned_update_carry(const float *d_in, float *d, const float *ac, const float *c, float *x, const float *dinv,
+                          float *u, int64_t n, double *part, double *pu, hipStream_t st) {
   const unsigned g = red_grid(n);
-  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d, ac, part, part + kRedBlocks, n);
-  hipLaunchKernelGGL(k_learned_step_carry, dim3(g), dim3(kBlock), 0, st, x, c, d, ac, dinv, u, part,
+  hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kBlock), 0, st, d_in, ac, part, part + kRedBlocks, n);
+  hipLaunchKernelGGL(k_learned_step_carry, dim3(g), dim3(kBlock), 0, st, x, c, d_in, d, ac, dinv, u, part,
                      part + kRedBlocks, (int)g, pu, n);
 }
 void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,

@@ -97,11 +97,12 @@ void learned_update(const float *d, const float *ac, const float *c, float *x, i
 // left (*nrm = ||u||)
 void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const double *pu, double *nrm,
                            double sqrt_count, const uint16_t *feat, hipStream_t st);
-// learned_update, then from the updated defect d -= alpha ac the next step's
-// u = dinv d (u may alias c) and the partials of ||u||^2 into pu (kRedBlocks
-// doubles apart from part's 2 kRedBlocks), in two launches
-void learned_update_carry(float *d, const float *ac, const float *c, float *x, const float *dinv, float *u, int64_t n,
-                          double *part, double *pu, hipStream_t st);
+// learned_update with the defect d_in, then from the updated defect
+// d = d_in - alpha ac (d_in may be d) the next step's u = dinv d (u may alias
+// c) and the partials of ||u||^2 into pu (kRedBlocks doubles apart from
+// part's 2 kRedBlocks), in two launches
+void learned_update_carry(const float *d_in, float *d, const float *ac, const float *c, float *x, const float *dinv,
+                          float *u, int64_t n, double *part, double *pu, hipStream_t st);
 // net.act[0] = feat with channel 0 = bf16(u * sqrt_count / *nrm) (nrm: device
 // FP64 2-norm of u; nrm == nullptr: no scaling)
 void learned_build_input(LearnedNet &net, const float *u, const double *nrm, double sqrt_count, const uint16_t *feat,
