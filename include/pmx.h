@@ -168,9 +168,18 @@ int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0);
  * (B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol on active rows,
  * rhs on Dirichlet rows; system matrix M + tau th1 A on every level), Solve
  * with opt (as pmx_system_solve; info likewise), oldrhs = rhs.
+ * Stiffness forms: PMX_FORM_TCD_A, _GALERKIN, _SDFEM and the SUPG form
+ * PMX_FORM_TCD_SUPG_AK (MatricesAKRhsAssemble_SUPG, src/FE/TCD3D.C:128-266),
+ * whose time-derivative matrix K joins M in both places (M + K - tau th2 A,
+ * M + K + tau th1 A), as TSystemTCD2D wires its SUPG case
+ * (src/System/SystemTCD2D.C:294-297, 336-341); pmx_tcd_init starts that case
+ * from MatrixMRhsAssemble_SUPG (TCD3D.C:74-126) when prm[PMX_PRM_FORM] is
+ * PMX_FORM_TCD_SUPG_AK or PMX_FORM_MASS_SUPG.
  * td: [0] tau [1] theta1 [2] theta2 [3] theta3 [4] theta4
- * (SetTimeDiscParameters, src/FE/TimeDiscRout.C:59-140).  The solution stays
- * on the device (pmx_system_solution reads it). */
+ * (SetTimeDiscParameters, src/FE/TimeDiscRout.C:59-140); the step sets
+ * prm[PMX_PRM_TAU] = td[0] and prm[PMX_PRM_THETA1] = td[1] for the forms
+ * (TimeDB->CURRENTTIMESTEPLENGTH / THETA1).  The solution stays on the device
+ * (pmx_system_solution reads it). */
 int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const double *opt, double *info);
 
 /* ---- measurement ---------------------------------------------------------

@@ -49,11 +49,15 @@ enum pmx_example {
 
 /* Local assembling forms (the AssembleFct3D callbacks). */
 enum pmx_form {
-  PMX_FORM_GALERKIN = 1,   /* BilinearAssemble      src/FE/ConvDiff3D.C:87-146  */
-  PMX_FORM_SDFEM = 2,      /* BilinearAssemble_SD   src/FE/ConvDiff3D.C:150-214 */
-  PMX_FORM_MASS = 3,       /* MatrixMRhsAssemble    src/FE/TCD3D.C:37-72        */
-  PMX_FORM_TCD_A = 4,      /* MatrixARhsAssemble    src/FE/TCD3D.C:268-322      */
-  PMX_FORM_TCD_SUPG_AK = 5 /* MatricesAKRhsAssemble_SUPG src/FE/TCD3D.C:128-266 (A and K) */
+  PMX_FORM_GALERKIN = 1,    /* BilinearAssemble      src/FE/ConvDiff3D.C:87-146  */
+  PMX_FORM_SDFEM = 2,       /* BilinearAssemble_SD   src/FE/ConvDiff3D.C:150-214 */
+  PMX_FORM_MASS = 3,        /* MatrixMRhsAssemble    src/FE/TCD3D.C:37-72        */
+  PMX_FORM_TCD_A = 4,       /* MatrixARhsAssemble    src/FE/TCD3D.C:268-322      */
+  PMX_FORM_TCD_SUPG_AK = 5, /* MatricesAKRhsAssemble_SUPG src/FE/TCD3D.C:128-266 (A and K) */
+  PMX_FORM_MASS_SUPG = 6,   /* MatrixMRhsAssemble_SUPG src/FE/TCD3D.C:74-126: Galerkin mass,
+                               rhs f (phi + delta b.grad phi)                    */
+  PMX_FORM_RHS_SUPG = 7,    /* RhsAssemble_SUPG      src/FE/TCD3D.C:374-421 (rhs only) */
+  PMX_FORM_RHS = 8          /* RhsAssemble           src/FE/TCD3D.C:423-446 (rhs only) */
 };
 
 #endif

@@ -339,6 +339,17 @@ static double sdfem_delta(const double *prm, double hK, double eps, double b1, d
   }
   return delta;
 }
+// RhsAssemble_SUPG's own parameter (src/FE/TCD3D.C:396-408); b = 0 (0/0 in
+// the reference) gives 0
+static double rhs_supg_tau(double hK, double eps, const double b[3]) {
+  const double bb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  if (!(bb > 0)) return 0.0;
+  const double pe = hK * bb / (2 * eps);
+  double tau;
+  if (pe < 1e-2) tau = pe / (6 * bb);
+  else tau = (1 / std::tanh(pe) - 1 / pe) / (2 * bb);
+  return tau * hK;
+}
 
 // ===========================================================================
 // FE space: ConstructSpace restatement (src/FE/FESpace3D.C:485-2080)
@@ -568,12 +579,15 @@ static void build_space(Space &S, const Tree &T, int depth, int order) {
 // ===========================================================================
 // Assembly: TAssembleMat3D::Assemble3D cell loop (src/System/AssembleMat3D.C:415-521)
 // ===========================================================================
+// Forms without local matrices (RhsAssemble, RhsAssemble_SUPG) leave E and
+// E2 as they are.
 static int assemble(const Space &S, const double *prm, double *E, double *rhs, double *E2) {
   const int form = (int)prm[PMX_PRM_FORM];
   const bool linear = prm[PMX_PRM_LINEAR] != 0;
+  const bool rhs_only = form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG;
   const int nnz = S.rowptr[S.N];
-  std::memset(E, 0, sizeof(double) * nnz);
-  if (E2) std::memset(E2, 0, sizeof(double) * nnz);
+  if (!rhs_only) std::memset(E, 0, sizeof(double) * nnz);
+  if (E2 && !rhs_only) std::memset(E2, 0, sizeof(double) * nnz);
   std::memset(rhs, 0, sizeof(double) * S.N);
   const Quad Q = gauss_hexa(rule_points(S.order, linear));
   const int nq = (int)Q.w.size(), nb = S.nb;
@@ -641,6 +655,25 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
           Lr[a] += mult * f[a] * c.f;
           for (int b = 0; b < nb; b++) L[a * nb + b] += mult * f[b] * f[a];
         }
+      } else if (form == PMX_FORM_MASS_SUPG) {
+        // MatrixMRhsAssemble_SUPG (TCD3D.C:74-126): the Galerkin mass matrix and
+        // the rhs f (phi + tau b.grad phi), tau from the unscaled coefficients
+        const double dl = sdfem_delta(prm, hK, c.eps, c.b[0], c.b[1], c.b[2]);
+        for (int a = 0; a < nb; a++) {
+          const double bgv = c.b[0] * gx[a] + c.b[1] * gy[a] + c.b[2] * gz[a];
+          Lr[a] += mult * (f[a] + dl * bgv) * c.f;
+          for (int b = 0; b < nb; b++) L[a * nb + b] += mult * f[b] * f[a];
+        }
+      } else if (form == PMX_FORM_RHS) {
+        // RhsAssemble (TCD3D.C:423-446)
+        for (int a = 0; a < nb; a++) Lr[a] += mult * f[a] * c.f;
+      } else if (form == PMX_FORM_RHS_SUPG) {
+        // RhsAssemble_SUPG (TCD3D.C:374-421)
+        const double tau_r = rhs_supg_tau(hK, c.eps, c.b);
+        for (int a = 0; a < nb; a++) {
+          const double bgv = c.b[0] * gx[a] + c.b[1] * gy[a] + c.b[2] * gz[a];
+          Lr[a] += mult * (f[a] + tau_r * bgv) * c.f;
+        }
       } else if (form == PMX_FORM_TCD_SUPG_AK) {
         // MatricesAKRhsAssemble_SUPG (TCD3D.C:128-266), delta from tau*theta1-scaled
         // coefficients and the max-norm of b
@@ -667,6 +700,10 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
     const int *dof = &S.global[S.begin[i]];
     for (int a = 0; a < nb; a++) {
       const int l = dof[a];
+      if (rhs_only) {
+        if (l < S.active) rhs[l] += Lr[a];
+        continue;
+      }
       if (l < S.active) {
         for (int b = 0; b < nb; b++) {
           int n = S.rowptr[l];
@@ -1314,13 +1351,25 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
   const int L = (int)Y->lev.size() - 1;
   std::vector<double> prm(prm_in, prm_in + PMX_PRM_LEN);
   const double tau = td[0], tend = td[1], th1 = td[2], th2 = td[3], th3 = td[4], th4 = td[5];
+  // TimeDB's CURRENTTIMESTEPLENGTH and THETA1, read by the SUPG forms
+  prm[PMX_PRM_TAU] = tau;
+  prm[PMX_PRM_THETA1] = th1;
   const int nlev = L + 1;
-  std::vector<std::vector<double>> M(nlev), A(nlev), rhsA(nlev);
+  // stiffness form: prm[FORM] (TCD_A when it names none); the SUPG form
+  // MatricesAKRhsAssemble_SUPG starts from MatrixMRhsAssemble_SUPG and adds its
+  // time-derivative matrix K to M, as TSystemTCD2D::AssembleSystMat does with
+  // MatAdd(M, S, 1) (src/System/SystemTCD2D.C:294-297) and RestoreMassMat
+  // takes it out again (:336-341)
+  int sform = (int)prm_in[PMX_PRM_FORM];
+  if (sform != PMX_FORM_GALERKIN && sform != PMX_FORM_SDFEM && sform != PMX_FORM_TCD_SUPG_AK) sform = PMX_FORM_TCD_A;
+  const bool supg = sform == PMX_FORM_TCD_SUPG_AK;
+  std::vector<std::vector<double>> M(nlev), A(nlev), K(nlev), rhsA(nlev);
   // AssembleMRhs at the start time
-  prm[PMX_PRM_FORM] = PMX_FORM_MASS;
+  prm[PMX_PRM_FORM] = supg ? PMX_FORM_MASS_SUPG : PMX_FORM_MASS;
   for (int l = 0; l < nlev; l++) {
     M[l].assign(Y->lev[l].rowptr.back(), 0.0);
     A[l].assign(Y->lev[l].rowptr.back(), 0.0);
+    if (supg) K[l].assign(Y->lev[l].rowptr.back(), 0.0);
     rhsA[l].assign(Y->lev[l].N, 0.0);
     assemble(Y->lev[l], prm.data(), M[l].data(), rhsA[l].data(), nullptr);
   }
@@ -1338,8 +1387,9 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
     t += tau;
     prm[PMX_PRM_TIME] = t;
     // AssembleARhs
-    prm[PMX_PRM_FORM] = PMX_FORM_TCD_A;
-    for (int l = 0; l < nlev; l++) assemble(Y->lev[l], prm.data(), A[l].data(), rhsA[l].data(), nullptr);
+    prm[PMX_PRM_FORM] = sform;
+    for (int l = 0; l < nlev; l++)
+      assemble(Y->lev[l], prm.data(), A[l].data(), rhsA[l].data(), supg ? K[l].data() : nullptr);
     const double *rhs = rhsA[L].data();
     std::memcpy(sol + NA, rhs + NA, sizeof(double) * (N - NA));
     // AssembleSystMat
@@ -1348,6 +1398,7 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
     daxpy(NA, tau * th4, rhs, B.data());
     const int nact = S.rowptr[NA];
     daxpy(nact, -tau * th2, A[L].data(), M[L].data());
+    if (supg) daxpy(nact, 1.0, K[L].data(), M[L].data());
     // MatVectActive
     for (int i = 0; i < NA; i++) {
       double s = 0;
@@ -1361,6 +1412,7 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
       const Space &Sl = Y->lev[l];
       const double fac = (l == L) ? (tau * th2 + tau * th1) : tau * th1;
       daxpy(Sl.rowptr[Sl.active], fac, A[l].data(), M[l].data());
+      if (supg && l < L) daxpy(Sl.rowptr[Sl.active], 1.0, K[l].data(), M[l].data());
       Y->E[l] = M[l];
     }
     std::memcpy(oldrhs.data(), rhs, sizeof(double) * N);
@@ -1370,6 +1422,7 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
     for (int l = 0; l < nlev; l++) {
       const Space &Sl = Y->lev[l];
       daxpy(Sl.rowptr[Sl.active], -tau * th1, A[l].data(), M[l].data());
+      if (supg) daxpy(Sl.rowptr[Sl.active], -1.0, K[l].data(), M[l].data());
     }
     orc::errors(S, prm.data(), sol, e);
     if (m > 1) {

@@ -110,7 +110,7 @@ struct System {
   DevBuf<double> kc;                  // Q2 cell matrices (finest level's cells x 27 x 28)
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
-  DevBuf<double> oldrhs, oldsol, tmp1, tmp2;  // time stepping (finest level)
+  DevBuf<double> oldrhs, oldsol, tmp1, tmp2, tmp3;  // time stepping (finest level)
   bool tcd_ready = false;
   DevBuf<double> krylov64;
   DevBuf<float> krylov32;
@@ -1256,12 +1256,20 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
     if (!prm) throw std::invalid_argument("prm is NULL");
     hipStream_t st = Y.ctx->st;
     const int form = (int)prm[PMX_PRM_FORM];
-    if (form < PMX_FORM_GALERKIN || form > PMX_FORM_TCD_SUPG_AK) throw std::invalid_argument("unknown form");
+    if (form < PMX_FORM_GALERKIN || form > PMX_FORM_RHS) throw std::invalid_argument("unknown form");
     Y.prm.upload(prm, PMX_PRM_LEN, st);
     Y.example = (int)prm[PMX_PRM_EXAMPLE];
-    Y.form = form;
     ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
     const int nq = Y.nq;
+    if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
+      // Assemble3D with no matrices: only the right-hand side of every level
+      for (auto &lp : Y.lev)
+        assemble(lp->D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
+                 lp->rhs.p, st);
+      check(hipStreamSynchronize(st), "assemble");
+      return PMX_OK;
+    }
+    Y.form = form;
     Y.two_matrices = form == PMX_FORM_TCD_SUPG_AK;
     for (auto &lp : Y.lev) {
       Level &lv = *lp;
@@ -1281,18 +1289,22 @@ int pmx_system_assemble(pmx_system *sys, const double *prm) {
 // 3DPrograms/TCD3D_ParMooN.C:565-663.  The mass matrix is kept apart, so the
 // system matrix M + tau theta1 A is rebuilt each step and no RestoreMassMat
 // pass is needed.
+// mass: the mass form into lv.mass; otherwise the stiffness form into the
+// operator values (and K of the SUPG form into lv.val2)
 static void assemble_into(System &Y, const double *prm, bool mass) {
   hipStream_t st = Y.ctx->st;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
   Y.example = (int)prm[PMX_PRM_EXAMPLE];
   Y.form = (int)prm[PMX_PRM_FORM];
+  Y.two_matrices = !mass && Y.form == PMX_FORM_TCD_SUPG_AK;
   ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
   const int nq = Y.nq;
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (mass) lv.mass.alloc(lv.L.nvals);
-    assemble(lv.D, Y.prm.p, Y.example, (int)prm[PMX_PRM_FORM], Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
-             mass ? lv.mass.p : lv.v64.val.p, nullptr, lv.rhs.p, st);
+    if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
+    assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
+             mass ? lv.mass.p : lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
   }
 }
 
@@ -1302,11 +1314,14 @@ int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0) {
     if (!prm) throw std::invalid_argument("prm is NULL");
     hipStream_t st = Y.ctx->st;
     std::vector<double> p(prm, prm + PMX_PRM_LEN);
-    p[PMX_PRM_FORM] = PMX_FORM_MASS;
+    // the SUPG time discretisation starts from MatrixMRhsAssemble_SUPG (its
+    // rhs carries the streamline term), the others from MatrixMRhsAssemble
+    const int f = (int)prm[PMX_PRM_FORM];
+    p[PMX_PRM_FORM] = (f == PMX_FORM_TCD_SUPG_AK || f == PMX_FORM_MASS_SUPG) ? PMX_FORM_MASS_SUPG : PMX_FORM_MASS;
     assemble_into(Y, p.data(), true);  // AssembleMRhs (SystemTCD3D.C:208-227)
     Level &lv = *Y.lev.back();
     const int64_t N = lv.L.N;
-    for (auto *b : {&Y.oldrhs, &Y.oldsol, &Y.tmp1, &Y.tmp2}) b->alloc(N);
+    for (auto *b : {&Y.oldrhs, &Y.oldsol, &Y.tmp1, &Y.tmp2, &Y.tmp3}) b->alloc(N);
     check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
     if (sol0) to_lattice<double>(Y, lv, sol0, Y.sol.p);
     else interpolate(lv.D, Y.prm.p, Y.sol.p, st);
@@ -1321,23 +1336,35 @@ int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const dou
     if (!prm || !td || !opt) throw std::invalid_argument("NULL argument");
     if (!Y.tcd_ready) throw std::logic_error("pmx_tcd_init first");
     const int form = (int)prm[PMX_PRM_FORM];
-    if (form != PMX_FORM_TCD_A && form != PMX_FORM_GALERKIN && form != PMX_FORM_SDFEM)
-      throw std::invalid_argument("time stepping takes a one-matrix stiffness form");
+    if (form != PMX_FORM_TCD_A && form != PMX_FORM_GALERKIN && form != PMX_FORM_SDFEM &&
+        form != PMX_FORM_TCD_SUPG_AK)
+      throw std::invalid_argument("time stepping takes a stiffness form (TCD_A, GALERKIN, SDFEM, TCD_SUPG_AK)");
     hipStream_t st = Y.ctx->st;
     const double tau = td[0], th1 = td[1], th2 = td[2], th3 = td[3], th4 = td[4];
     Level &lv = *Y.lev.back();
     const int64_t N = lv.L.N;
     check(hipMemcpyAsync(Y.oldsol.p, Y.sol.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
-    assemble_into(Y, prm, false);                                // AssembleARhs (SystemTCD3D.C:229-246)
+    // the SUPG form reads the step length and theta1 (TimeDB's
+    // CURRENTTIMESTEPLENGTH and THETA1, TCD3D.C:145-147) from td
+    std::vector<double> pa(prm, prm + PMX_PRM_LEN);
+    pa[PMX_PRM_TAU] = tau;
+    pa[PMX_PRM_THETA1] = th1;
+    assemble_into(Y, pa.data(), false);                          // AssembleARhs (SystemTCD3D.C:229-246)
     boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // sol Dirichlet = rhs
-    // AssembleSystMat: B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol
+    // AssembleSystMat: B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol;
+    // the SUPG form adds its time-derivative part K to M in both the rhs and
+    // the system matrix, as the 2D system does (SystemTCD2D.C:294-297; its
+    // RestoreMassMat takes it out again, :336-341; here M is kept apart)
+    const bool supg = Y.two_matrices;
     css_apply<double>(lv.D, 0, lv.mass.p, Y.oldsol.p, nullptr, Y.tmp1.p, 0.0, -1, st);
     css_apply<double>(lv.D, 0, lv.v64.val.p, Y.oldsol.p, nullptr, Y.tmp2.p, 0.0, -1, st);
-    tcd_rhs(lv.D, Y.oldrhs.p, lv.rhs.p, Y.tmp1.p, Y.tmp2.p, tau * th3, tau * th4, tau * th2, Y.b.p, st);
-    // system matrices M + tau th1 A on every level
+    if (supg) css_apply<double>(lv.D, 0, lv.val2.p, Y.oldsol.p, nullptr, Y.tmp3.p, 0.0, -1, st);
+    tcd_rhs(lv.D, Y.oldrhs.p, lv.rhs.p, Y.tmp1.p, Y.tmp2.p, supg ? Y.tmp3.p : nullptr, tau * th3, tau * th4,
+            tau * th2, Y.b.p, st);
+    // system matrices M (+ K) + tau th1 A on every level
     for (auto &lp : Y.lev) {
       Level &l = *lp;
-      mass_plus(l.v64.val.p, l.mass.p, tau * th1, l.L.nvals, st);
+      mass_plus(l.v64.val.p, l.mass.p, supg ? l.val2.p : nullptr, tau * th1, l.L.nvals, st);
       l.have_val32 = false;
       l.have_dinv = false;
     }

@@ -156,4 +156,17 @@ __host__ __device__ inline double sdfem_delta(const double *prm, double hK, doub
   return delta;
 }
 
+// The stabilisation parameter of RhsAssemble_SUPG (src/FE/TCD3D.C:396-408),
+// which has its own formula: Pe = hK |b| / (2 eps), tau = hK (coth Pe - 1/Pe)
+// / (2 |b|), and hK Pe / (6 |b|) for Pe < 1e-2.  With b = 0 the reference
+// divides 0 by 0 (a NaN right-hand side); here the parameter is 0 there, which
+// is the value every finite b.grad phi = 0 multiplies to.
+__host__ __device__ inline double rhs_supg_tau(double hK, double eps, double b1, double b2, double b3) {
+  const double bb = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+  if (!(bb > 0)) return 0.0;
+  const double pe = hK * bb / (2 * eps);
+  const double tau = pe < 1e-2 ? pe / (6 * bb) : (1 / tanh(pe) - 1 / pe) / (2 * bb);
+  return tau * hK;
+}
+
 }  // namespace pmx

@@ -459,8 +459,12 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const QpCoeffs c = example_coeffs(prm, X, Y, Z);
   const double hK = sqrt(3.0) * h;  // THexahedron::GetDiameter of a cube
   double delta = 0;
-  if (form == PMX_FORM_SDFEM) {
+  if (form == PMX_FORM_SDFEM || form == PMX_FORM_MASS_SUPG) {
+    // BilinearAssemble_SD and MatrixMRhsAssemble_SUPG (TCD3D.C:103-104):
+    // Compute_SDFEM_delta of the unscaled coefficients
     delta = sdfem_delta(prm, hK, c.eps, c.b0, c.b1, c.b2);
+  } else if (form == PMX_FORM_RHS_SUPG) {
+    delta = rhs_supg_tau(hK, c.eps, c.b0, c.b1, c.b2);
   } else if (form == PMX_FORM_TCD_SUPG_AK) {
     const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
     delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
@@ -498,7 +502,7 @@ __host__ __device__ inline bool coeffs_inline(int example) {
 // coordinate lies in one cell along it, every other node in two), so the
 // loops are uniform across the wavefront and the basis tables are read with
 // scalar loads; TWO selects the second matrix of the SUPG TCD form.
-template <int ORDER, int EX, int EY, int EZ, bool TWO, bool INL>
+template <int ORDER, int EX, int EY, int EZ, bool TWO, bool INL, bool RONLY>
 __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M, int n, int nq, int form,
                                              const double *__restrict__ prm, const double *__restrict__ qref,
                                              const double *__restrict__ qp, const double *__restrict__ tab,
@@ -546,9 +550,17 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
           const double fa = Ta[0], gax = g * Ta[1], gay = g * Ta[2], gaz = g * Ta[3];
           const double bga = b0 * gax + b1 * gay + b2 * gaz;
           double u0, ux, uy, uz, k0 = 0;
-          if (form == PMX_FORM_MASS) {
+          if (RONLY) {
+            // RhsAssemble / RhsAssemble_SUPG: f (phi_a + delta b.grad phi_a)
+            // (delta = 0 for the Galerkin rhs)
+            frhs += w * ff * (fa + dl * bga);
+            continue;
+          }
+          if (form == PMX_FORM_MASS || form == PMX_FORM_MASS_SUPG) {
+            // the mass matrix; the rhs is f (phi_a + delta b.grad phi_a) with
+            // delta = 0 for MatrixMRhsAssemble (TCD3D.C:64, 117)
             u0 = fa; ux = uy = uz = 0;
-            frhs += w * ff * fa;
+            frhs += w * ff * (fa + dl * bga);
           } else {
             const double sa = fa + dl * bga;
             u0 = cc * sa;
@@ -567,6 +579,7 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
             if (TWO) acc2[b] += k0 * Tb[0];
           }
         }
+        if (RONLY) continue;
         // add the row into the CSS slots of node p
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -580,7 +593,7 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   rhs[p] = frhs;
 }
 
-template <int ORDER, bool TWO, bool INL>
+template <int ORDER, bool TWO, bool INL, bool RONLY>
 __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__restrict__ blocks,
                                                      const DevSegment *__restrict__ segs, int M, int n, int nq,
                                                      const double *__restrict__ prm, const double *__restrict__ qref,
@@ -593,7 +606,7 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__r
   if (idx >= s.count) return;
   const int form = (int)prm[PMX_PRM_FORM];
 #define PMX_AROW(O, EX, EY, EZ) \
-  assemble_row<O, EX, EY, EZ, TWO, INL>(s, idx, M, n, nq, form, prm, qref, qp, tab, val, val2, rhs)
+  assemble_row<O, EX, EY, EZ, TWO, INL, RONLY>(s, idx, M, n, nq, form, prm, qref, qp, tab, val, val2, rhs)
   if (ORDER == 1) {
     PMX_AROW(1, 1, 1, 1);
     return;
@@ -1199,7 +1212,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int f
       if (MASS) {
         d0 = r0 ? w : 0.0;
         d1 = d2 = d3 = 0.0;
-        rr = ff * d0;
+        // right-hand-side weights w f (1, g delta b): delta = 0 for
+        // MatrixMRhsAssemble, Compute_SDFEM_delta for MatrixMRhsAssemble_SUPG
+        const double be = s1 * b0 + s2 * b1 + s3 * b2;
+        rr = ff * (r0 ? w : w * g * dl * be);
       } else {
         const double be = s1 * b0 + s2 * b1 + s3 * b2;  // b_comp (0 for comp 0)
         const double sc = r0 ? w : w * g * dl * be;
@@ -1302,6 +1318,23 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
   if (!inl)
     hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
                        nq, D.n, qpbuf);
+  if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
+    // right-hand-side forms (no local matrices): the matrices are left as they are
+#define PMX_RHS(O, IN)                                                                                     \
+  hipLaunchKernelGGL((k_assemble<O, false, IN, true>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,        \
+                     D.blocks_all, D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, nullptr, nullptr, rhs)
+    if (D.order == 1) {
+      if (inl) PMX_RHS(1, true);
+      else PMX_RHS(1, false);
+    } else {
+      if (inl) PMX_RHS(2, true);
+      else PMX_RHS(2, false);
+    }
+#undef PMX_RHS
+    hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
+                       D.nboundary, D.M, prm_dev, rhs);
+    return;
+  }
   check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
   if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   if (kc && D.order == 2 && !val2) {
@@ -1311,7 +1344,7 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
 #define PMX_CELL(IN, MS)                                                                                   \
   hipLaunchKernelGGL((k_cell_mfma<IN, MS>), dim3(grid), dim3(kCellThreads), lds, st, D.n, nq, form, prm_dev, qref,   \
                      qpbuf, tab, kc)
-    const bool ms = form == PMX_FORM_MASS;
+    const bool ms = form == PMX_FORM_MASS || form == PMX_FORM_MASS_SUPG;
     if (inl) {
       if (ms) PMX_CELL(true, true);
       else PMX_CELL(true, false);
@@ -1327,7 +1360,8 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
     return;
   }
 #define PMX_ASM(O, TW, IN)                                                                                   \
-  hipLaunchKernelGGL((k_assemble<O, TW, IN>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all,      \
+  hipLaunchKernelGGL((k_assemble<O, TW, IN, false>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,             \
+                     D.blocks_all,                                                                                \
                      D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, val, val2, rhs)
 #define PMX_ASM2(O, TW) \
   do {                   \
@@ -1435,33 +1469,38 @@ __global__ void k_interpolate(const double *__restrict__ prm, int M, double *__r
     x[i] = example_exact(prm, a * h, b * h, c * h);
   }
 }
-// val = mass + c * val (MatAdd of Blas2.C:128-150 on the stored stencil values)
-__global__ void k_mass_plus(double *__restrict__ val, const double *__restrict__ mass, double c, int64_t n) {
+// val = mass (+ k) + c * val (MatAdd of Blas2.C:128-150 on the stored stencil
+// values; k: the time-derivative part of the SUPG form, or nullptr)
+__global__ void k_mass_plus(double *__restrict__ val, const double *__restrict__ mass,
+                            const double *__restrict__ k, double c, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    val[i] = mass[i] + c * val[i];
+    val[i] = (k ? mass[i] + k[i] : mass[i]) + c * val[i];
 }
 // right-hand side of TSystemTCD3D::AssembleSystMat (SystemTCD3D.C:255-283):
-// active rows B = t3 oldrhs + t4 rhs + (M - t2 A) oldsol, Dirichlet rows B = rhs
+// active rows B = t3 oldrhs + t4 rhs + (M (+ K) - t2 A) oldsol, Dirichlet rows
+// B = rhs (k_old = K oldsol of the SUPG form, SystemTCD2D.C:294-297, or nullptr)
 __global__ void k_tcd_rhs(int M, const double *__restrict__ oldrhs, const double *__restrict__ rhs,
-                          const double *__restrict__ m_old, const double *__restrict__ a_old, double t3, double t4,
-                          double t2, double *__restrict__ B) {
+                          const double *__restrict__ m_old, const double *__restrict__ a_old,
+                          const double *__restrict__ k_old, double t3, double t4, double t2,
+                          double *__restrict__ B) {
   const int64_t N = (int64_t)M * M * M;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
     const int a = (int)(i % M), b = (int)((i / M) % M), c = (int)(i / ((int64_t)M * M));
     const bool interior = a > 0 && b > 0 && c > 0 && a < M - 1 && b < M - 1 && c < M - 1;
-    B[i] = interior ? (t3 * oldrhs[i] + t4 * rhs[i]) + (m_old[i] - t2 * a_old[i]) : rhs[i];
+    const double mk = k_old ? m_old[i] + k_old[i] : m_old[i];
+    B[i] = interior ? (t3 * oldrhs[i] + t4 * rhs[i]) + (mk - t2 * a_old[i]) : rhs[i];
   }
 }
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st) {
   hipLaunchKernelGGL(k_interpolate, dim3(grid_for(D.N)), dim3(kBlock), 0, st, prm_dev, D.M, x);
 }
-void mass_plus(double *val, const double *mass, double c, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_mass_plus, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, c, n);
+void mass_plus(double *val, const double *mass, const double *k, double c, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_mass_plus, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, k, c, n);
 }
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
-             double t3, double t4, double t2, double *B, hipStream_t st) {
-  hipLaunchKernelGGL(k_tcd_rhs, dim3(grid_for(D.N)), dim3(kBlock), 0, st, D.M, oldrhs, rhs, m_old, a_old, t3, t4,
-                     t2, B);
+             const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st) {
+  hipLaunchKernelGGL(k_tcd_rhs, dim3(grid_for(D.N)), dim3(kBlock), 0, st, D.M, oldrhs, rhs, m_old, a_old, k_old,
+                     t3, t4, t2, B);
 }
 
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st) {

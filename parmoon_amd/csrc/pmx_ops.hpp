@@ -120,11 +120,13 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
               const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st);
 constexpr int64_t kCellMatrixDoubles = 27 * 28;  // cell buffer entries per Q2 cell
 // time stepping (TSystemTCD3D): nodal interpolation of the example's initial
-// function, val = mass + c val, and the Crank-Nicolson / theta right-hand side
+// function, val = mass (+ k) + c val, and the Crank-Nicolson / theta
+// right-hand side (k, k_old: the SUPG time-derivative matrix K and K oldsol,
+// nullptr for the Galerkin forms)
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st);
-void mass_plus(double *val, const double *mass, double c, int64_t n, hipStream_t st);
+void mass_plus(double *val, const double *mass, const double *k, double c, int64_t n, hipStream_t st);
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
-             double t3, double t4, double t2, double *B, hipStream_t st);
+             const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st);
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st);
 void backsolve(const double *H, const double *s, double *y, int k, int ld, hipStream_t st);
 template <typename T, typename X>

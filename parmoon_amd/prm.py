@@ -14,6 +14,7 @@ FOURIER, GRF, GRF_SIGMA, BLOB = 16, 136, 392, 393
 
 EX_LAPLACE, EX_SINE_CD, EX_CURL_FOURIER, EX_HEAT_AMC, EX_LOGNORMAL, EX_ROTATING = range(6)
 FORM_GALERKIN, FORM_SDFEM, FORM_MASS, FORM_TCD_A, FORM_TCD_SUPG_AK = 1, 2, 3, 4, 5
+FORM_MASS_SUPG, FORM_RHS_SUPG, FORM_RHS = 6, 7, 8
 
 
 def base(example, eps=1.0, form=FORM_GALERKIN, b=(0.0, 0.0, 0.0), c=0.0, time=0.0,

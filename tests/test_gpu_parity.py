@@ -41,7 +41,27 @@ CASES = [
     ("rotating-Q2-supgAK", 2, 2, lambda: dict_supg(P.rotating())),
     ("lognormal-Q1-sdfem", 1, 3, lambda: P.lognormal(nref=3, peclet=10.0)),
     ("rotating-Q1-supgAK", 1, 3, lambda: dict_supg(P.rotating())),
+    ("config1-Q1-sdfem-type0", 1, 3, lambda: with_sdfem_type0(P.config1())),
+    ("config1-Q2-sdfem-type0-inline", 2, 2, lambda: with_sdfem_type0(P.config1())),
+    ("curl-Q2-sdfem-type0", 2, 2, lambda: with_sdfem_type0(P.config2(nref=2))),
+    ("rotating-Q1-mass-supg", 1, 3, lambda: with_form(P.rotating(), P.FORM_MASS_SUPG)),
+    ("config1-Q2-mass-supg-inline", 2, 2, lambda: with_form(P.config1(), P.FORM_MASS_SUPG)),
+    ("curl-Q2-mass-supg", 2, 2, lambda: with_form(P.config2(nref=2), P.FORM_MASS_SUPG)),
 ]
+
+
+def with_form(p, form):
+    p = p.copy()
+    p[P.FORM] = form
+    return p
+
+
+def with_sdfem_type0(p):
+    """SDFEM_TYPE = 0 (ConvDiff.C:568-590) with delta1 != delta0 so both branches differ."""
+    p = p.copy()
+    p[P.SDFEM_TYPE] = 0
+    p[P.DELTA1] = 0.6
+    return p
 
 
 def dict_supg(p):
@@ -70,6 +90,34 @@ def test_assembly_matches_oracle(ctx, name, order, nref, mk):
     assert rel(rhs, out[1]) < 1e-12
     if two:
         assert row_rel_err(out[2], S.entries(which=1), rp) < 1e-12
+    S.close()
+
+
+RHS_CASES = [
+    ("config1-Q1-rhs-supg", 1, 3, lambda: P.config1()),
+    ("config1-Q2-rhs-supg-inline", 2, 2, lambda: P.config1()),
+    ("curl-Q2-rhs-supg", 2, 2, lambda: P.config2(nref=2)),
+    ("lognormal-Q1-rhs-supg", 1, 3, lambda: P.lognormal(nref=3, peclet=10.0)),
+]
+
+
+@pytest.mark.parametrize("name,order,nref,mk", RHS_CASES, ids=[c[0] for c in RHS_CASES])
+@pytest.mark.parametrize("form", [P.FORM_RHS, P.FORM_RHS_SUPG])
+def test_rhs_forms_match_oracle(ctx, name, order, nref, mk, form):
+    """RhsAssemble / RhsAssemble_SUPG (TCD3D.C:374-446): the right-hand side of
+    every level is re-assembled and the matrices stay as they were."""
+    prm = mk()
+    print(f"[case] rhs form {form} {name} order={order} nref={nref}", flush=True)
+    S = pa.System(ctx, nref - 1, 2, order)
+    S.assemble(prm)
+    E_before = S.entries()
+    prm_r = with_form(prm, form)
+    S.assemble(prm_r)
+    assert np.array_equal(S.entries(), E_before)
+    for lvl, nr in ((0, nref - 1), (1, nref)):
+        sp = orc.Space(nr, order)
+        _, r_ref = sp.assemble(prm_r)
+        assert rel(S.rhs(lvl), r_ref) < 1e-12, lvl
     S.close()
 
 

@@ -299,3 +299,200 @@ def test_step_length_control_is_the_minimal_residual_step():
     S.set_slc(0)
     assert np.abs(u2 - u1).max() > 1e-8 * np.abs(u1).max()
     assert np.linalg.norm(rhs - A @ u2) <= np.linalg.norm(rhs - A @ x0)
+
+
+# ------------------------------------- closed forms with right-hand sides (f = 1)
+def exact_1d_vectors(order, h):
+    """int phi_i and int phi_i' over [0, h] of the 1D Lagrange basis."""
+    from numpy.polynomial import polynomial as Pl
+    nodes = np.linspace(0, h, order + 1)
+    m, d = [], []
+    for i in range(order + 1):
+        c = np.array([1.0])
+        for j in range(order + 1):
+            if j != i:
+                c = Pl.polymul(c, np.array([-nodes[j], 1.0]) / (nodes[i] - nodes[j]))
+        integ = lambda q: Pl.polyval(h, Pl.polyint(q)) - Pl.polyval(0, Pl.polyint(q))
+        m.append(integ(c))
+        d.append(integ(Pl.polyder(c)))
+    return np.array(m), np.array(d)
+
+
+def assemble_1d_vec(order, n, e):
+    v = np.zeros(order * n + 1)
+    for k in range(n):
+        v[order * k:order * k + order + 1] += e
+    return v
+
+
+def kron3v(vx, vy, vz):
+    return np.kron(vz, np.kron(vy, vx))
+
+
+def closed_form_parts(order, nref):
+    """Global 1D matrices and vectors of the brick lattice (test index first)."""
+    n = 2 ** nref
+    h = 1.0 / n
+    Me, Ke, Ce = exact_1d(order, h)
+    M, K, Cv = (assemble_1d(order, n, X) for X in (Me, Ke, Ce))
+    Ct = assemble_1d(order, n, Ce.T)
+    me, de = exact_1d_vectors(order, h)
+    m, d = assemble_1d_vec(order, n, me), assemble_1d_vec(order, n, de)
+    return h, M, K, Cv, Ct, m, d
+
+
+def sd_terms(M, K, Cv, Ct, b, c):
+    """Galerkin operator pieces and the streamline terms: G = (b.grad u + c u, v)
+    without diffusion, S = (b.grad u, b.grad v), T = (u, b.grad v)."""
+    mixed = lambda k, l: kron3(*[K if (k == l == a) else (Cv if a == k else (Ct if a == l else M))
+                                 for a in range(3)])
+    D = [kron3(Cv, M, M), kron3(M, Cv, M), kron3(M, M, Cv)]
+    T = [kron3(Ct, M, M), kron3(M, Ct, M), kron3(M, M, Ct)]
+    G = sum(b[k] * D[k] for k in range(3)) + c * kron3(M, M, M)
+    S = sum(b[k] * b[l] * mixed(k, l) for k in range(3) for l in range(3))
+    Tb = sum(b[k] * T[k] for k in range(3))
+    return G, S, Tb
+
+
+def rhs_streamline(m, d, b):
+    """int b.grad phi_a (f = 1) as a sum of Kronecker products."""
+    return b[0] * kron3v(d, m, m) + b[1] * kron3v(m, d, m) + b[2] * kron3v(m, m, d)
+
+
+def delta_closed(sdfem_type, hK, eps, nb, delta0, delta1):
+    """Compute_SDFEM_delta's two cases in their textbook form (ConvDiff.C:568-611):
+    type 0 (Roos-Stynes-Tobiska): delta0 h / |b| in the convection-dominated
+    regime eps < h|b|, delta1 h^2 / eps otherwise; type 2: the 1D Green's
+    function parameter h/(2|b|) (coth(Pe) - 1/Pe), Pe = h|b|/(2 eps)."""
+    if sdfem_type == 0:
+        return delta0 * hK / nb if eps < hK * nb else delta1 * hK * hK / eps
+    pe = nb * hK / (2 * eps)
+    return delta0 * hK * (np.cosh(pe) / np.sinh(pe) - 1 / pe) / (2 * nb)
+
+
+@pytest.mark.parametrize("order,nref", [(1, 2), (2, 1)])
+@pytest.mark.parametrize("case", ["sd_type2", "sd_type0_conv", "sd_type0_diff", "mass_supg", "rhs",
+                                  "rhs_supg", "tcd_supg_ak"])
+def test_assembly_closed_form_rhs(order, nref, case):
+    """Constant eps, b, c and f = 1 (the log-normal example with sigma = 0): the
+    matrices and right-hand sides of the SD, SUPG-mass, rhs-only and SUPG A/K
+    forms as Kronecker products of exact 1D integrals.  The streamline terms
+    carry delta (BilinearAssemble_SD, MatrixMRhsAssemble_SUPG), RhsAssemble_SUPG's
+    tau, or Compute_SDFEM_delta of the theta1*tau-scaled coefficients times
+    theta1*tau (MatricesAKRhsAssemble_SUPG, TCD3D.C:167-181,206-208); theta1*tau
+    is not 1 here, so a missing scale factor shows."""
+    eps, b, c = 0.01, np.array([0.3, -0.7, 1.1]), 0.4
+    sdfem_type, delta0, delta1 = 2, 0.7, 1.3
+    if case == "sd_type0_conv":
+        sdfem_type = 0
+    if case == "sd_type0_diff":
+        sdfem_type, eps = 0, 2.0
+    form = {"mass_supg": P.FORM_MASS_SUPG, "rhs": P.FORM_RHS, "rhs_supg": P.FORM_RHS_SUPG,
+            "tcd_supg_ak": P.FORM_TCD_SUPG_AK}.get(case, P.FORM_SDFEM)
+    prm = P.base(P.EX_LOGNORMAL, eps=eps, form=form, b=b, c=c, sdfem_type=sdfem_type, delta0=delta0,
+                 delta1=delta1)
+    prm[P.GRF_SIGMA] = 0.0
+    prm[P.TAU], prm[P.THETA1] = 0.02, 0.5
+    sp = orc.Space(nref, order)
+    h, M, K, Cv, Ct, m, d = closed_form_parts(order, nref)
+    hK, nb = np.sqrt(3) * h, np.linalg.norm(b)
+    G, S, Tb = sd_terms(M, K, Cv, Ct, b, c)
+    lap = eps * (kron3(K, M, M) + kron3(M, K, M) + kron3(M, M, K))
+    mass, one, stream = kron3(M, M, M), kron3v(m, m, m), rhs_streamline(m, d, b)
+    dl = delta_closed(sdfem_type, hK, eps, nb, delta0, delta1)
+    E2ref = None
+    if form == P.FORM_SDFEM:
+        Aref, rref = lap + G + dl * (S + c * Tb), one + dl * stream
+    elif form == P.FORM_MASS_SUPG:
+        Aref, rref = mass, one + dl * stream
+    elif form == P.FORM_RHS:
+        Aref, rref = None, one
+    elif form == P.FORM_RHS_SUPG:
+        # RhsAssemble_SUPG's tau equals the type-2 delta with delta0 = 1 and
+        # nu = 1 when Pe >= 1e-2 (here Pe = 1.2 or 2.3)
+        tr = delta_closed(2, hK, eps, nb, 1.0, 1.0)
+        Aref, rref = None, one + tr * stream
+    else:
+        # tilde delta of the scaled coefficients times theta1 tau: the same
+        # number as delta of the unscaled ones (Pe is scale invariant)
+        tau = dl
+        Aref, rref, E2ref = lap + G + tau * (S + c * Tb), one + tau * stream, tau * Tb
+    Mlat = order * 2 ** nref + 1
+    lat = lattice_index(sp, Mlat)
+    act = np.arange(sp.N) < sp.active
+    E0 = np.full(sp.nnz, 7.0)  # rhs-only forms leave the matrix as it is
+    E2 = np.zeros(sp.nnz)
+    r = np.zeros(sp.N)
+    rc = orc.lib().orc_assemble(sp.h, orc.P(prm), orc.P(E0), orc.P(r), orc.P(E2))
+    assert rc == 0
+    if Aref is None:
+        assert np.all(E0 == 7.0)
+    else:
+        A = Aref[np.ix_(lat, lat)]
+        Ao = csr(sp, E0).toarray()
+        assert np.abs(Ao[act] - A[act]).max() <= 1e-13 * np.abs(A[act]).max()
+    if E2ref is not None:
+        K2 = E2ref[np.ix_(lat, lat)]
+        Ko = csr(sp, E2).toarray()
+        assert np.abs(Ko[act] - K2[act]).max() <= 1e-13 * np.abs(K2[act]).max()
+    rr = rref[lat]
+    assert np.abs(r[act] - rr[act]).max() <= 1e-13 * np.abs(rr[act]).max()
+    assert np.all(r[~act] == 0.0)  # g = 0 on the Dirichlet nodes
+
+
+def test_rhs_supg_small_peclet():
+    """RhsAssemble_SUPG below Pe = 1e-2 uses the series limit of
+    (coth Pe - 1/Pe)/(2|b|) h, i.e. tau = h^2 / (12 eps)."""
+    order, nref = 1, 2
+    eps, b = 50.0, np.array([0.3, -0.7, 1.1])
+    prm = P.base(P.EX_LOGNORMAL, eps=eps, form=P.FORM_RHS_SUPG, b=b)
+    prm[P.GRF_SIGMA] = 0.0
+    sp = orc.Space(nref, order)
+    h, M, K, Cv, Ct, m, d = closed_form_parts(order, nref)
+    hK = np.sqrt(3) * h
+    assert hK * np.linalg.norm(b) / (2 * eps) < 1e-2
+    _, r = sp.assemble(prm)
+    rr = (kron3v(m, m, m) + hK * hK / (12 * eps) * rhs_streamline(m, d, b))[lattice_index(sp, 2 ** nref + 1)]
+    act = np.arange(sp.N) < sp.active
+    assert np.abs(r[act] - rr[act]).max() <= 1e-14 * np.abs(rr[act]).max()
+
+
+def test_tcd_supg_step_closed_form():
+    """One Crank-Nicolson step of the SUPG time discretisation (constant
+    coefficients, f = 1): the oracle's loop solves
+    (M + K + tau th1 A) u1 = tau th3 f0 + tau th4 f1 + (M + K - tau th2 A) u0
+    with M the mass matrix, A and K the SUPG stiffness and time-derivative
+    matrices and f0 (MatrixMRhsAssemble_SUPG) = f1 (MatricesAKRhsAssemble_SUPG)
+    the streamline-augmented rhs; the right side is checked against the same
+    system built from the closed forms and solved by scipy."""
+    order, nref = 1, 2
+    eps, b, c, tau = 0.01, np.array([0.3, -0.7, 1.1]), 0.4, 0.05
+    prm = P.base(P.EX_LOGNORMAL, eps=eps, form=P.FORM_TCD_SUPG_AK, b=b, c=c)
+    prm[P.GRF_SIGMA] = 0.0
+    prm[P.TAU], prm[P.THETA1] = tau, 0.5
+    S = orc.System(nref, 1, order)
+    fine = S.levels[-1]
+    h, M, K, Cv, Ct, m, d = closed_form_parts(order, nref)
+    hK, nb = np.sqrt(3) * h, np.linalg.norm(b)
+    G, Sm, Tb = sd_terms(M, K, Cv, Ct, b, c)
+    lap = eps * (kron3(K, M, M) + kron3(M, K, M) + kron3(M, M, K))
+    dl = delta_closed(2, hK, eps, nb, 1.0, 1.0)
+    A = lap + G + dl * (Sm + c * Tb)
+    Km = dl * Tb
+    Mm = kron3(M, M, M)
+    f = kron3v(m, m, m) + dl * rhs_streamline(m, d, b)
+    lat = lattice_index(fine, order * 2 ** nref + 1)
+    A, Km, Mm, f = A[np.ix_(lat, lat)], Km[np.ix_(lat, lat)], Mm[np.ix_(lat, lat)], f[lat]
+    rng = np.random.default_rng(4)
+    u0 = rng.standard_normal(fine.N)
+    u0[fine.active:] = 0.0
+    NA = fine.active
+    th = 0.5
+    rhs = tau * th * f + tau * th * f + (Mm + Km - tau * th * A) @ u0
+    Lhs = Mm + Km + tau * th * A
+    u1 = u0.copy()
+    u1[:NA] = np.linalg.solve(Lhs[:NA, :NA], rhs[:NA])
+    opt = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-15, smoother=3, omega=1.0)
+    s, err = S.tcd3d(prm, [tau, tau * 1.5, th, th, th, th], opt, u0, max_steps=1)
+    assert len(err) == 1
+    assert np.abs(s - u1).max() <= 1e-11 * np.abs(u1).max()
