@@ -923,11 +923,20 @@ struct System {
   // SC_STEP_LENGTH_CONTROL_FINE_SCALAR (1) / _ALL_SCALAR (2) / off (0)
   int slc = 0;
   std::vector<std::vector<double>> old_sol, old_def;
+  // optional row order of the SOR/SSOR sweeps per level (empty: lexicographic,
+  // as TMGLevel3D; a multicolour order reproduces a device sweep row for row)
+  std::vector<std::vector<int>> order;
 };
 
 static void mg_smooth(System &Y, int l, int kind, int n, double omega) {
-  for (int k = 0; k < n; k++)
-    smooth(Y.lev[l], Y.E[l].data(), kind, omega, Y.sol[l].data(), Y.f[l].data(), Y.aux1[l].data());
+  const bool ordered = kind != 1 && l < (int)Y.order.size() && !Y.order[l].empty();
+  for (int k = 0; k < n; k++) {
+    if (ordered)
+      smooth_ordered(Y.lev[l], Y.E[l].data(), kind, omega, Y.sol[l].data(), Y.f[l].data(), Y.order[l].data(),
+                     (int)Y.order[l].size());
+    else
+      smooth(Y.lev[l], Y.E[l].data(), kind, omega, Y.sol[l].data(), Y.f[l].data(), Y.aux1[l].data());
+  }
 }
 // TMGLevel3D::StepLengthControl (src/FE/MGLevel3D.C:908-960): with
 // x = u - uold and y = A x, omega = (def, y) / (y, y); omega = Parameters[0]
@@ -967,7 +976,7 @@ static void mg_cycle(System &Y, int i) {
         dense_lu_solve(n, a.data(), b.data());
         Y.sol[0] = b;
       } else {
-        smooth(S, Y.E[0].data(), cs, o[8], Y.sol[0].data(), Y.f[0].data(), Y.aux1[0].data());
+        mg_smooth(Y, 0, cs, 1, o[8]);
       }
       res = defect(S, Y.E[0].data(), Y.sol[0].data(), Y.f[0].data(), Y.aux0[0].data());
       it++;
@@ -1075,11 +1084,14 @@ static int fgmres(System &Y, const double *rhs, double *sol, double *info) {
         return j;
       }
     }
-    for (int a = m - 1; a >= 0; a--) {
+    // UpdateGmresIterate over the columns this cycle filled (all restart
+    // columns in FgmresIte.C:563, which reads unfilled ones when maxit ends
+    // the cycle early)
+    for (int a = i - 1; a >= 0; a--) {
       s[a] /= H[a][a];
       for (int b = a - 1; b >= 0; b--) s[b] -= H[b][a] * s[a];
     }
-    for (int a = 0; a < m; a++) daxpy(N, s[a], z[a].data(), sol);
+    for (int a = 0; a < i; a++) daxpy(N, s[a], z[a].data(), sol);
     res = defect(S, E, sol, rhs, v[0].data());
     res0 = beta = res;
   }
@@ -1333,6 +1345,12 @@ void orc_system_set_slc(void *sys, int mode) {
   Y.old_sol.resize(Y.lev.size());
   Y.old_def.resize(Y.lev.size());
 }
+// row order of the SOR/SSOR sweeps of a level (n = 0: lexicographic again)
+void orc_system_set_order(void *sys, int level, const int *order, int n) {
+  System &Y = *static_cast<System *>(sys);
+  Y.order.resize(Y.lev.size());
+  Y.order[level].assign(order, order + n);
+}
 void orc_errors(void *sp, const double *prm, const double *sol, double *err) { errors(SP(sp), prm, sol, err); }
 // which = 0: exact solution values at the nodes (TFEFunction3D::Interpolate)
 void orc_interpolate(void *sp, const double *prm, int which, double *sol) {
@@ -1344,7 +1362,8 @@ void orc_interpolate(void *sp, const double *prm, int which, double *sol) {
   }
 }
 
-// 3DPrograms/TCD3D_ParMooN.C:565-753 with TSystemTCD3D (src/System/SystemTCD3D.C:208-428)
+// 3DPrograms/TCD3D_ParMooN.C:565-753 with TSystemTCD3D (src/System/SystemTCD3D.C:208-428).
+// err_out: per step {time, L2 error, H1-semi error, L2(0,t;L2), L2(0,t;H1), solver iterations}
 int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const double *opt,
                   double *sol, double *err_out, int max_steps) {
   auto *Y = static_cast<System *>(sys);
@@ -1429,8 +1448,8 @@ int orc_tcd3d_run(void *sys, const double *prm_in, const double *td, const doubl
       acc[0] += (e[0] * e[0] + e_old[0] * e_old[0]) * tau / 2.0;
       acc[1] += (e[1] * e[1] + e_old[1] * e_old[1]) * tau / 2.0;
     }
-    double *o = err_out + 5 * (m - 1);
-    o[0] = t; o[1] = e[0]; o[2] = e[1]; o[3] = std::sqrt(acc[0]); o[4] = std::sqrt(acc[1]);
+    double *o = err_out + 6 * (m - 1);
+    o[0] = t; o[1] = e[0]; o[2] = e[1]; o[3] = std::sqrt(acc[0]); o[4] = std::sqrt(acc[1]); o[5] = info[0];
     e_old[0] = e[0]; e_old[1] = e[1];
   }
   return m;

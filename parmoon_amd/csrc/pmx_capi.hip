@@ -110,6 +110,7 @@ struct System {
   DevBuf<double> kc;                  // Q2 cell matrices (finest level's cells x 27 x 28)
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
+  DevBuf<double> sol_try, r_try;      // FP32 form: the corrected iterate and its defect before acceptance
   DevBuf<double> oldrhs, oldsol, tmp1, tmp2, tmp3;  // time stepping (finest level)
   bool tcd_ready = false;
   DevBuf<double> krylov64;
@@ -139,6 +140,7 @@ struct System {
   // (SC_STEP_LENGTH_CONTROL_ALL_SCALAR)
   int slc = 0;
   bool learn_active = false;
+  bool x_zero_shortcut = true;  // see mg_cycle
   int fallbacks = 0;
   DevBuf<double> lscal;               // norm, (d, Ac), (Ac, Ac)
   ~System() {
@@ -534,7 +536,11 @@ static void mg_cycle(System &Y, int i, bool post = true, bool x_zero = false) {
   // (CorrectDefect)
   restrict_<T>(cl.D, lv.D, cv.f.p, v.d.p, st, true);
   fill<T>(cv.x.p, (T)0, cl.L.N, st);                   // Reset
-  for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1, true, j == 0);
+  // the first visit of the coarse level starts from its Reset (x = 0); the
+  // learned pre-smoothing there takes d = f (restrict_ with zero_surface
+  // leaves the coarse f zero on the Dirichlet surface, so the Dirichlet
+  // entries of that defect are the ones A x = 0 would give)
+  for (int j = 0; j < Y.rec[i]; j++) mg_cycle<T>(Y, i - 1, true, j == 0 && Y.x_zero_shortcut);
   if ((int)o[12] < 1) Y.rec[i] = 1;
   // prolongation, CorrectNodes and x += SC_GMG_DAMP e
   prolongate_add<T>(cl.D, lv.D, cv.x.p, v.x.p, o[13], st);
@@ -693,8 +699,11 @@ static int fgmres(System &Y, T *x, const T *b, int m, int maxit, double rmin, do
       }
       if (r > 1e10 * res0) { *status = PMX_ERR_DIVERGED; *res_out = r; return j; }
     }
-    backsolve(H, s, yv, m - 1, ld, st);
-    multi_axpy<T, T>(x, Z, yv, m, N, N, st);
+    // the columns of this cycle (i < m when the iteration limit ended it; the
+    // reference back-substitutes all restart columns there, FgmresIte.C:563,
+    // and reads Hessenberg columns the cycle did not fill)
+    backsolve(H, s, yv, i - 1, ld, st);
+    multi_axpy<T, T>(x, Z, yv, i, N, N, st);
     fine_defect<T>(Y, x, b, V);
     dot<T>(V, V, N, part, beta, 1, st);
     bet = read_scalar(Y, beta);
@@ -978,11 +987,23 @@ static int solve_device(System &Y, double *info) {
   Level &lv = *Y.lev.back();
   const int64_t N = lv.L.N;
   const int prec = (int)Y.opt[15];
+  // ILU (id 4 of MultiGrid3D.C's switches, TMGLevel3D::ILU) is not built;
+  // every other id follows the switches' branches (default: SOR / SSOR)
+  if ((int)Y.opt[5] == 4 || (int)Y.opt[9] == 4)
+    throw std::invalid_argument("smoother id 4 (ILU) is not implemented");
+  // PMX_NO_XZERO=1: a coarse level's first learned step forms its defect with
+  // the operator instead of taking d = f on x = 0 (the two agree bit for bit)
+  const char *nxz = std::getenv("PMX_NO_XZERO");
+  const bool xz = !(nxz && nxz[0] == '1');
+  if (xz != Y.x_zero_shortcut) {
+    Y.x_zero_shortcut = xz;
+    Y.invalidate_graphs();
+  }
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
   check(hipEventRecord(e0, st), "event");
-  int status = PMX_OK, its = 0, outer = 0;
+  int status = PMX_OK, its = 0, outer = 0, fp64_fallback = 0;
   double res = 0, res0 = 0;
   const int maxit = (int)Y.opt[2];
   const double rel_target = Y.opt[16];
@@ -1013,6 +1034,8 @@ static int solve_device(System &Y, double *info) {
       Y.learn_active = true;
     }
     Y.r64.alloc(N);
+    Y.sol_try.alloc(N);
+    Y.r_try.alloc(N);
     DevBuf<float> r32, e32;
     r32.alloc(N);
     e32.alloc(N);
@@ -1029,19 +1052,45 @@ static int solve_device(System &Y, double *info) {
       int st_in;
       const double want = std::max(inner_red, 0.5 * target / res);
       its += krylov<float>(Y, e32.p, r32.p, maxit - its, 0.0, want, &ri, &ri0, &st_in);
-      add_convert<double, float>(Y.sol.p, e32.p, N, st);
-      fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
-      dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
-      const double rn = read_scalar(Y, nr);
       outer++;
-      if (Y.learn_active && rn >= res) {
-        // the cycles with the learned post-smoother did not reduce the FP64 residual
+      // the correction is accepted only if the inner solve ended with a
+      // finite estimate and the FP64 residual of the corrected iterate is
+      // finite and smaller; otherwise it is discarded
+      double rn = HUGE_VAL;
+      const bool inner_ok = st_in != PMX_ERR_DIVERGED && std::isfinite(ri);
+      if (inner_ok) {
+        copy<double>(Y.sol_try.p, Y.sol.p, N, st);
+        add_convert<double, float>(Y.sol_try.p, e32.p, N, st);
+        fine_defect<double>(Y, Y.sol_try.p, Y.b.p, Y.r_try.p);
+        dot<double>(Y.r_try.p, Y.r_try.p, N, Y.ctx->part.p, nr, 1, st);
+        rn = read_scalar(Y, nr);
+      }
+      if (inner_ok && std::isfinite(rn) && rn < res) {
+        std::swap(Y.sol.p, Y.sol_try.p);
+        std::swap(Y.r64.p, Y.r_try.p);
+        res = rn;
+        continue;
+      }
+      if (Y.learn_active) {
+        // the cycles with the learned smoother did not reduce the FP64
+        // residual: the same correction again with the plain smoother
         Y.learn_active = false;
         Y.fallbacks++;
+        continue;
       }
-      if (!(rn == rn) || rn > 1e10 * res0) { status = PMX_ERR_DIVERGED; res = rn; break; }
-      res = rn;
-      if (st_in == PMX_ERR_DIVERGED) { status = PMX_ERR_DIVERGED; break; }
+      if (its >= maxit) break;
+      // the FP32 solve itself failed: the rest of the solve in FP64 from the
+      // last accepted iterate (FP64 operator, cycle and Krylov vectors)
+      fp64_fallback = 1;
+      double r64res, r64res0;
+      int st64;
+      its += krylov<double>(Y, Y.sol.p, Y.b.p, std::max(1, maxit - its), target, 0.0, &r64res, &r64res0, &st64);
+      fine_defect<double>(Y, Y.sol.p, Y.b.p, Y.r64.p);
+      dot<double>(Y.r64.p, Y.r64.p, N, Y.ctx->part.p, nr, 1, st);
+      res = read_scalar(Y, nr);
+      outer++;
+      if (st64 == PMX_ERR_DIVERGED || !std::isfinite(res)) status = PMX_ERR_DIVERGED;
+      break;
     }
     if (res > target && status == PMX_OK) status = PMX_ERR_NOCONV;
   }
@@ -1054,6 +1103,7 @@ static int solve_device(System &Y, double *info) {
   if (info) {
     info[0] = its; info[1] = res; info[2] = res0; info[3] = outer; info[4] = ms * 1e-3;
     info[5] = prec == 2 ? Y.fallbacks : 0;
+    info[6] = fp64_fallback;
   }
   Y.learn_active = false;
   return status;

@@ -51,6 +51,7 @@ def lib():
         L.orc_system_solve.argtypes = [vp, vp, vp, vp, vp]
         L.orc_system_solve.restype = ip
         L.orc_system_set_slc.argtypes = [vp, ip]
+        L.orc_system_set_order.argtypes = [vp, ip, vp, ip]
         L.orc_errors.argtypes = [vp, vp, vp, vp]
         L.orc_interpolate.argtypes = [vp, vp, ip, vp]
         L.orc_tcd3d_run.argtypes = [vp, vp, vp, vp, vp, vp, ip]
@@ -184,6 +185,18 @@ class System:
     def set_slc(self, mode):
         self.L.orc_system_set_slc(self.h, int(mode))
 
+    def set_order(self, level, order):
+        """Row order of the SOR/SSOR sweeps on a level (None: lexicographic)."""
+        o = np.zeros(0, np.int32) if order is None else np.ascontiguousarray(order, np.int32)
+        self.L.orc_system_set_order(self.h, level % len(self.levels), P(o) if o.size else None, int(o.size))
+
+    def set_multicolour(self):
+        """Every level sweeps in the device's multicolour order: colour by colour
+        ((x, y, z) mod (order + 1) of the lattice node, numbered as
+        pmx_space_colours), rows of a colour in any order (they are independent)."""
+        for l, sp in enumerate(self.levels):
+            self.set_order(l, multicolour_order(sp))
+
     def solve(self, opt, rhs, sol):
         o = np.zeros(24)
         o[:len(opt)] = opt
@@ -195,10 +208,21 @@ class System:
     def tcd3d(self, prm, tdisc, opt, sol, max_steps=100000):
         o = np.zeros(24)
         o[:len(opt)] = opt
-        err = np.zeros(5 * max_steps)
+        err = np.zeros(6 * max_steps)
         s = np.array(sol, float)
         m = self.L.orc_tcd3d_run(self.h, P(prm), P(np.asarray(tdisc, float)), P(o), P(s), P(err), max_steps)
-        return s, err[:5 * m].reshape(m, 5)
+        return s, err[:6 * m].reshape(m, 6)
+
+
+def multicolour_order(sp):
+    """Active rows sorted by the colour of their lattice node: (x, y, z) mod K
+    with K = order + 1, colour index cx + K (cy + K cz) (the numbering of
+    node_colour in parmoon_amd/csrc/pmx_mesh.cpp)."""
+    M = sp.order * 2 ** sp.nref + 1
+    ijk = np.rint(sp.dof_coords()[:sp.active] * (M - 1)).astype(int)
+    K = sp.order + 1
+    col = (ijk[:, 0] % K) + K * ((ijk[:, 1] % K) + K * (ijk[:, 2] % K))
+    return np.argsort(col, kind="stable").astype(np.int32)
 
 
 def dense_lu_solve(a, b):

@@ -43,6 +43,7 @@ def test_forward_vs_oracle(small, M):
     u = rng.standard_normal((M, M, M, 16)).astype(np.float32)
     out = small.learned_forward(u)
     ref = LR.forward(p.astype(np.float64), u.astype(np.float64))
+    print(f"forward rel err {nrel(out, ref):.3e}", flush=True)
     assert nrel(out, ref) < 5e-3, nrel(out, ref)
 
 
@@ -61,6 +62,7 @@ def test_gradient_vs_oracle(small, M):
     for l in range(4):
         a, b = g[offs[l]:offs[l + 1]], g_ref[offs[l]:offs[l + 1]]
         cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+        print(f"gradient layer {l} rel err {nrel(a, b):.3e} cos {cos:.6f}", flush=True)
         assert nrel(a, b) < 3e-2 and cos > 0.999, (l, nrel(a, b), cos)
 
 
@@ -259,4 +261,58 @@ def test_learned_coarse_levels(ctx):
     assert np.max(np.abs(x_l - x64)) <= 1e-7 * np.max(np.abs(x64))
     n_e, x_e = run(eager=1, **coarse)
     assert n_e == n_l and np.max(np.abs(x_e - x_l)) <= 1e-12 * np.max(np.abs(x_l))
+    S.close()
+
+
+def test_fallback_to_the_plain_smoother(ctx):
+    """The plain smoother takes over when learned cycles stop reducing the
+    residual: with networks whose output is zero on every level above the
+    coarsest (learned steps there change nothing: the step length of a zero
+    correction is 0) and replacing both pre- and post-smoothing, the cycle is
+    a bare coarse-grid correction; the FGMRES estimate stalls, the solve
+    switches to SSOR (info[5] >= 1) and still reaches 1e-10."""
+    print("[case] learned fallback", flush=True)
+    S = pa.System(ctx, 1, 3, 2)
+    S.assemble(P.config2(nref=3))
+    zero = np.zeros(S.learned_nparams(), np.float32)
+    for lvl in (1, 2):
+        S.learned_params(zero, level=lvl)
+    base = dict(solver=16, restart=30, maxit=300, res_norm_min=0.0, red_factor=0.0, smoother=3, pre=2, post=2,
+                omega=1.0, rel_target=1e-10)
+    info, rc = S.solve_resident(P.solver_opt(**dict(base, precision=2, learned_levels=2, learned_skip=0,
+                                                    learned_pre=1, learned_steps=1)), check=False)
+    print(f"fallback solve: rc {rc} iterations {int(info[0])} fallbacks {int(info[5])}", flush=True)
+    assert rc == 0 and info[5] >= 1, info
+    assert info[1] <= 1e-10 * info[2] * 1.0001
+    x = S.solution()
+    info64, rc64 = S.solve_resident(P.solver_opt(**dict(base, precision=0)))
+    assert rc64 == 0
+    x64 = S.solution()
+    assert np.max(np.abs(x - x64)) <= 1e-7 * np.max(np.abs(x64))
+    S.close()
+
+
+def test_learned_coarse_zero_start_shortcut(ctx):
+    """A coarse level entered after its Reset (x = 0) starts its learned
+    smoothing from d = f instead of forming f - A x: the solve gives the same
+    bits with the shortcut switched off (PMX_NO_XZERO=1)."""
+    print("[case] learned x_zero shortcut", flush=True)
+    S = pa.System(ctx, 1, 3, 2)
+    S.assemble(P.config2(nref=3))
+    S.learned_init(3)
+    base = dict(solver=16, restart=30, maxit=300, res_norm_min=0.0, red_factor=0.0, smoother=3, pre=2, post=2,
+                omega=1.0, rel_target=1e-10, precision=2, learned_levels=2, learned_skip=1, learned_pre=1,
+                learned_steps=2)
+    S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=4, steps=50, lr=2e-3, seed=1, levels=2,
+                    data=1, skip=1)
+    info_a, rc_a = S.solve_resident(P.solver_opt(**base))
+    xa = S.solution()
+    os.environ["PMX_NO_XZERO"] = "1"
+    try:
+        info_b, rc_b = S.solve_resident(P.solver_opt(**base))
+        xb = S.solution()
+    finally:
+        del os.environ["PMX_NO_XZERO"]
+    assert rc_a == 0 and rc_b == 0
+    assert info_a[0] == info_b[0] and np.array_equal(xa, xb)
     S.close()

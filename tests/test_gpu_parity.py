@@ -278,15 +278,18 @@ def test_graph_replays_match_eager(q2sys):
             S.matvec(g)  # eager work between replays
 
 
-@pytest.mark.parametrize("order,nref", [(1, 4), (2, 4)])
-def test_wg_sweep_matches_launch_per_colour(ctx, order, nref):
+WG_CASES = [(o, n, p, sm) for o, n in ((1, 4), (2, 4)) for p, sm in ((0, 2), (0, 3), (1, 3), (1, 2))]
+
+
+@pytest.mark.parametrize("order,nref,precision,smoother", WG_CASES)
+def test_wg_sweep_matches_launch_per_colour(ctx, order, nref, precision, smoother):
     """SOR and SSOR sweeps run in one workgroup (all colours, workgroup
     barriers between them) give the launch-per-colour iterate bit for bit:
     FGMRES-GMG solves with the single-workgroup sweep forced on every level
     against none, FP64 and FP32.  The finest levels here (17^3 Q1: 422 rows per
     colour, 33^3 Q2: 1103) need several rows per lane and mix residue classes
     of different stencil shape inside a wavefront."""
-    print(f"[case] wg sweep order={order} nref={nref}", flush=True)
+    print(f"[case] wg sweep order={order} nref={nref} precision={precision} smoother={smoother}", flush=True)
     prm = P.config2(nref=nref) if order == 2 else P.config1b()
     S = pa.System(ctx, nref - 2, 3, order)
     S.assemble(prm)
@@ -294,18 +297,53 @@ def test_wg_sweep_matches_launch_per_colour(ctx, order, nref):
     n, na = int(S.dims(-1)[0]), int(S.dims(-1)[1])
     sol0 = np.zeros(n)
     sol0[na:] = rhs[na:]
-    # (FP32 cycles with forward SOR overflow on the Q2 SUPG problem, in both
-    # variants alike, so that case is left out there)
-    cases = [(0, 2), (0, 3), (1, 3)] + ([(1, 2)] if order == 1 else [])
-    for precision, smoother in cases:
-        base = dict(solver=16, restart=20, maxit=60, res_norm_min=0.0, red_factor=0.0, rel_target=1e-10,
-                    smoother=smoother, omega=1.0, precision=precision)
-        a, ia, rca = S.solve(P.solver_opt(wg_sweep_rows=-1, **base), rhs, sol0, check=False)
-        b, ib, rcb = S.solve(P.solver_opt(wg_sweep_rows=1 << 30, **base), rhs, sol0, check=False)
-        assert rca == 0 and rcb == 0, (precision, smoother, rca, rcb)
-        assert ia[0] == ib[0], (precision, smoother, ia[0], ib[0])
-        assert np.array_equal(a, b), (precision, smoother, float(np.max(np.abs(a - b))))
+    # (the FP32 form with forward SOR on the Q2 SUPG problem needs more than
+    # 60 iterations; it used to end in NaN when the iteration limit cut an
+    # FGMRES cycle short, see test_fgmres_limit_inside_a_cycle)
+    base = dict(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=0.0, rel_target=1e-10,
+                smoother=smoother, omega=1.0, precision=precision)
+    a, ia, rca = S.solve(P.solver_opt(wg_sweep_rows=-1, **base), rhs, sol0, check=False)
+    b, ib, rcb = S.solve(P.solver_opt(wg_sweep_rows=1 << 30, **base), rhs, sol0, check=False)
+    assert rca == 0 and rcb == 0, (precision, smoother, rca, rcb)
+    assert ia[0] == ib[0], (precision, smoother, ia[0], ib[0])
+    assert np.array_equal(a, b), (precision, smoother, float(np.max(np.abs(a - b))))
     S.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fgmres_limit_inside_a_cycle(q2sys, precision):
+    """An iteration limit that ends an FGMRES cycle before its restart length
+    updates the iterate with the columns the cycle filled: a finite iterate
+    whose FP64 residual is below the initial one, for every limit (the FP32
+    form's inner solves get the remaining budget, which ends cycles early)."""
+    S, O, prm = q2sys
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    sol0 = np.zeros_like(rhs)
+    sol0[fine.active:] = rhs[fine.active:]
+    last = None
+    for maxit in (1, 3, 7, 13, 20, 27):
+        print(f"[case] fgmres limit {maxit} precision {precision}", flush=True)
+        opt = P.solver_opt(solver=16, restart=20, maxit=maxit, res_norm_min=0.0, red_factor=0.0, rel_target=1e-10,
+                           smoother=2, omega=1.0, precision=precision)
+        s, info, rc = S.solve(opt, rhs, sol0, check=False)
+        assert rc in (0, 4) and np.all(np.isfinite(s)), (maxit, rc, info)
+        assert np.isfinite(info[1]) and info[1] < info[2], (maxit, info)
+        r, res = S.defect(s, rhs)
+        assert abs(res - info[1]) <= 1e-6 * res
+        last = res
+    assert last < 1e-3 * info[2]
+
+
+def test_ilu_smoother_is_rejected(q1sys):
+    """Smoother id 4 (ILU in MultiGrid3D.C's switches) is not built: the solve
+    reports an invalid argument instead of substituting another smoother."""
+    S, O, prm = q1sys
+    rhs = O.rhs()
+    for kw in (dict(smoother=4), dict(coarse=4)):
+        _, _, rc = S.solve(P.solver_opt(**dict(dict(solver=16, maxit=10, smoother=3), **kw)), rhs,
+                           np.zeros_like(rhs), check=False)
+        assert rc == 1, kw
 
 
 def test_coarse_inverse_follows_the_operator(ctx):

@@ -496,3 +496,26 @@ def test_tcd_supg_step_closed_form():
     s, err = S.tcd3d(prm, [tau, tau * 1.5, th, th, th, th], opt, u0, max_steps=1)
     assert len(err) == 1
     assert np.abs(s - u1).max() <= 1e-11 * np.abs(u1).max()
+
+
+def test_fgmres_limit_inside_a_cycle():
+    """FGMRES stopped by maxit before the end of a restart cycle updates the
+    iterate with the columns that cycle filled (FgmresIte.C:563 would
+    back-substitute all restart columns): finite, and the residual falls with
+    the limit."""
+    S = orc.System(1, 3, 1)
+    S.assemble(P.config1b())
+    fine = S.levels[-1]
+    rhs = S.rhs()
+    sol0 = np.zeros(fine.N)
+    sol0[fine.active:] = rhs[fine.active:]
+    r0 = fine.defect(S.entries(), sol0, rhs)[1]
+    prev = r0
+    for maxit in (1, 3, 7, 13):
+        opt = P.solver_opt(solver=16, restart=10, maxit=maxit, res_norm_min=0.0, red_factor=1e-14, smoother=3,
+                           omega=1.0)
+        s, info = S.solve(opt, rhs, sol0)
+        assert np.all(np.isfinite(s))
+        r = fine.defect(S.entries(), s, rhs)[1]
+        assert r < prev
+        prev = r
