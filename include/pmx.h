@@ -187,8 +187,12 @@ int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const dou
  * HIP events: 0 stencil SpMV, 1 defect, 2 Jacobi sweep, 3 SSOR sweep,
  * 4 assembly (all kernels of one level), 5 prolongation, 6 restriction,
  * 7 one MG V-cycle, 8 one learned post-smoothing step (FP32), 9 the network's
- * forward convolutions alone on the level's lattice (FP32/BF16).
- * precision 0 FP64, 1 FP32.  out: ms per launch. */
+ * forward convolutions alone on the level's lattice (FP32/BF16); these are
+ * timed with two events around the `reps` repetitions (ms per repetition).
+ * 10 and 11 give the average duration of a single dispatch, from events
+ * recorded by the dispatch itself (hipExtLaunchKernelGGL), without the gaps
+ * between dispatches: 10 one colour launch of an SSOR sweep (launch per
+ * colour), 11 one SpMV.  precision 0 FP64, 1 FP32.  out: ms. */
 int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms);
 
 /* ---- learned post-smoother (no counterpart in the reference; it replaces

@@ -1647,6 +1647,35 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
         }
       }
     };
+    if (kernel == 10 || kernel == 11) {
+      // durations of the individual dispatches (not the gaps between them):
+      // 10 the colour launches of SSOR sweeps (launch per colour forced), 11 SpMV
+      Y.opt[21] = -1;
+      auto run = [&]() {
+        if (kernel == 10) {
+          if (precision) level_smooth<float>(Y, lv, 3, 1.0, 1);
+          else level_smooth<double>(Y, lv, 3, 1.0, 1);
+        } else {
+          if (precision) css_apply<float>(lv.D, 0, lv.v32.val.p, lv.v32.x.p, nullptr, lv.v32.d.p, 0.0, -1, st);
+          else css_apply<double>(lv.D, 0, lv.v64.val.p, lv.v64.x.p, nullptr, lv.v64.d.p, 0.0, -1, st);
+        }
+      };
+      run();
+      DispatchTimer tm;
+      set_dispatch_timer(&tm);
+      try {
+        for (int r = 0; r < reps; r++) run();
+      } catch (...) {
+        set_dispatch_timer(nullptr);
+        throw;
+      }
+      set_dispatch_timer(nullptr);
+      check(hipStreamSynchronize(st), "sync");
+      *ms = tm.used ? tm.total_ms() / tm.used : 0.0;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return PMX_OK;
+    }
     if (kernel >= 8) {
       if (!precision) throw std::invalid_argument("the learned smoother runs in FP32");
       if (!Y.nets[li].ready) throw std::logic_error("learned smoother: train or load parameters first");

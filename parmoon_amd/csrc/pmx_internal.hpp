@@ -35,7 +35,10 @@ constexpr int kMaxSegs = 216;        // residue classes of a level: at most 6^3
 constexpr int kMaxColours = 27;
 constexpr int64_t kSweepLdsBytes = 48 * 1024;  // largest x staged in LDS by the single-workgroup sweep
 constexpr int kSweepWgRows = 512;    // default: single-workgroup sweep up to this many rows per colour
-constexpr int kSlice = 16;        // rows per value slice (see Layout)
+#ifndef PMX_SLICE
+#define PMX_SLICE 16
+#endif
+constexpr int kSlice = PMX_SLICE;  // rows per value slice (see Layout)
 constexpr int kRedBlocks = 1024;  // partial sums of the deterministic reductions
 
 struct Segment {

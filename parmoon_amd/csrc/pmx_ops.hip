@@ -1,6 +1,7 @@
 // pmx_ops.hip -- operator, transfer, assembly and Krylov kernels (gfx950) and
 // their stream-ordered launchers.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdlib>
@@ -1015,6 +1016,38 @@ void mgs_arnoldi(T *w, const T *V, int64_t n, int k1, double *h, T *vnext, doubl
   hipLaunchKernelGGL(k_norm_scale<T>, dim3(g), dim3(kBlock), 0, st, w, pa, g, h + k1, vnext, n);
 }
 
+// Per-dispatch timing of the row kernels: while a timer is installed on the
+// calling thread, css_apply launches through hipExtLaunchKernelGGL with a
+// start and a stop event of the timer around each dispatch (the events take
+// the dispatch's own begin and end, not the gaps between dispatches).
+static thread_local DispatchTimer *g_timer = nullptr;
+void set_dispatch_timer(DispatchTimer *t) { g_timer = t; }
+double DispatchTimer::total_ms() const {
+  double s = 0;
+  for (int k = 0; k < used; k++) {
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]), "dispatch timer");
+    s += ms;
+  }
+  return s;
+}
+DispatchTimer::~DispatchTimer() {
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+}
+static void timer_events(hipEvent_t *e0, hipEvent_t *e1) {
+  DispatchTimer &t = *g_timer;
+  if (2 * t.used + 1 >= (int)t.ev.size()) {
+    for (int k = 0; k < 2; k++) {
+      hipEvent_t e;
+      check(hipEventCreate(&e), "event");
+      t.ev.push_back(e);
+    }
+  }
+  *e0 = t.ev[2 * t.used];
+  *e1 = t.ev[2 * t.used + 1];
+  t.used++;
+}
+
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st) {
@@ -1022,8 +1055,14 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
   const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
 #define PMX_ROWS(MODE, R)                                                                                      \
-  hipLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val, x, f, \
-                     y, omega)
+  if (g_timer) {                                                                                               \
+    hipEvent_t e0, e1;                                                                                         \
+    timer_events(&e0, &e1);                                                                                    \
+    hipExtLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk,       \
+                          D.segs, D.M, val, x, f, y, omega);                                                   \
+  } else                                                                                                       \
+    hipLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val, x, \
+                       f, y, omega)
 #define PMX_ROWS_R(MODE)                                                                                       \
   switch (row_lanes(D)) {                                                                                      \
     case 1: PMX_ROWS(MODE, 1); break;                                                                          \

@@ -31,6 +31,20 @@ struct DevLayout {
 
 void check(hipError_t e, const char *what);
 
+// event pairs of timed dispatches (see set_dispatch_timer)
+struct DispatchTimer {
+  std::vector<hipEvent_t> ev;
+  int used = 0;
+  double total_ms() const;  // sum of the dispatch durations (synchronise first)
+  DispatchTimer() = default;
+  DispatchTimer(const DispatchTimer &) = delete;
+  DispatchTimer &operator=(const DispatchTimer &) = delete;
+  ~DispatchTimer();
+};
+// while installed (non-null), the row kernels of css_apply record each
+// dispatch between two events of the timer; nullptr uninstalls
+void set_dispatch_timer(DispatchTimer *t);
+
 template <typename T> void fill(T *x, T v, int64_t n, hipStream_t st);
 template <typename T> void copy(T *y, const T *x, int64_t n, hipStream_t st);
 template <typename T>

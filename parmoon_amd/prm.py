@@ -143,3 +143,12 @@ def solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-10, red_factor=
               coarse_sweeps, coarse_red, cycle, gmg_damp, bicg_flexible, precision, rel_target, inner_red, eager,
               learned_levels, learned_steps, wg_sweep_rows, learned_skip, learned_pre]
     return o
+
+
+if __name__ == "__main__":
+    # python -m parmoon_amd.prm config2 FILE [NREF]: the benchmark's parameter
+    # block as PRM_LEN raw doubles (read by tests/cpp/kernel_probe)
+    import sys
+    if len(sys.argv) < 3 or sys.argv[1] != "config2":
+        sys.exit("usage: python -m parmoon_amd.prm config2 FILE [NREF]")
+    config2(nref=int(sys.argv[3]) if len(sys.argv) > 3 else 6).astype(np.float64).tofile(sys.argv[2])

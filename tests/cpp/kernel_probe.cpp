@@ -1,8 +1,13 @@
-// Short run for hardware-counter passes (rocprofv3 --pmc): one 64^3 Q2 SUPG
-// system, 6 FP32 stencil SpMV launches and 3 FP32 SSOR sweeps (54 colour
-// launches each) on the finest level, then 5 device copies of 1 GiB
-// (hipMemcpy device to device, the calibration kernel of known size).
-// Prints the time per SpMV launch and per sweep.
+// Short run for hardware-counter passes (rocprofv3 --pmc) on the 64^3 Q2 SUPG
+// system of the benchmark: the parameter block comes from a file of
+// PMX_PRM_LEN doubles (written by `python -m parmoon_amd.prm config2 FILE`,
+// the curl-Fourier convection of the benchmark).  Five levels 4^3 .. 64^3,
+// then on the finest level: 6 FP32 stencil SpMV launches, 3 FP32 SSOR sweeps
+// (53 colour launches each; the first SpMV and sweep are untimed) and 5
+// device copies of 1 GiB (hipMemcpy device to device, the calibration kernel
+// of known size).  Prints the average dispatch durations.
+//
+//   kernel_probe PRM_FILE
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -10,40 +15,39 @@
 
 #include "pmx.h"
 
-int main() {
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::printf("usage: kernel_probe PRM_FILE\n");
+    return 2;
+  }
+  std::vector<double> prm(PMX_PRM_LEN, 0.0);
+  FILE *fp = std::fopen(argv[1], "rb");
+  if (!fp || std::fread(prm.data(), sizeof(double), PMX_PRM_LEN, fp) != (size_t)PMX_PRM_LEN) {
+    std::printf("cannot read %d doubles from %s\n", PMX_PRM_LEN, argv[1]);
+    return 2;
+  }
+  std::fclose(fp);
   pmx_ctx *ctx = nullptr;
   pmx_system *sys = nullptr;
   if (pmx_ctx_create(0, &ctx) || pmx_system_create(ctx, 2, 5, 2, &sys)) {
     std::printf("setup failed: %s\n", pmx_last_error());
     return 1;
   }
-  std::vector<double> prm(PMX_PRM_LEN, 0.0);
-  prm[PMX_PRM_EXAMPLE] = PMX_EX_SINE_CD;
-  prm[PMX_PRM_EPS] = 1e-4;
-  prm[PMX_PRM_B0] = 0.2672612419124244;
-  prm[PMX_PRM_B1] = 0.5345224838248488;
-  prm[PMX_PRM_B2] = 0.8017837257372732;
-  prm[PMX_PRM_FORM] = PMX_FORM_SDFEM;
-  prm[PMX_PRM_SDFEM_TYPE] = 2;
-  prm[PMX_PRM_DELTA0] = 1.0;
-  prm[PMX_PRM_DELTA1] = 1.0;
-  prm[PMX_PRM_LINEAR] = 1;
-  prm[PMX_PRM_POLYDEG] = 2;
   if (pmx_system_assemble(sys, prm.data())) {
     std::printf("assemble failed: %s\n", pmx_last_error());
     return 1;
   }
   double ms = 0;
-  if (pmx_time_kernel(sys, 0, -1, 1, 5, &ms)) {
+  if (pmx_time_kernel(sys, 11, -1, 1, 5, &ms)) {
     std::printf("time_kernel failed: %s\n", pmx_last_error());
     return 1;
   }
-  std::printf("fp32 spmv %.4f ms per launch\n", ms);
-  if (pmx_time_kernel(sys, 3, -1, 1, 2, &ms)) {
+  std::printf("fp32 spmv %.4f ms per dispatch\n", ms);
+  if (pmx_time_kernel(sys, 10, -1, 1, 2, &ms)) {
     std::printf("time_kernel failed: %s\n", pmx_last_error());
     return 1;
   }
-  std::printf("fp32 ssor sweep %.4f ms\n", ms);
+  std::printf("fp32 ssor colour launch %.4f ms per dispatch\n", ms);
   const size_t bytes = size_t(1) << 30;
   void *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) return 1;

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "..", "oracle", "liborc.so")
+# PMX_ORACLE_LIB selects another build of the oracle (the sanitizer build)
+LIB_PATH = os.environ.get("PMX_ORACLE_LIB") or os.path.join(_HERE, "..", "oracle", "liborc.so")
 PRM_LEN = 512
 
 _lib = None

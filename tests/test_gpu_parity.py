@@ -335,6 +335,42 @@ def test_fgmres_limit_inside_a_cycle(q2sys, precision):
     assert last < 1e-3 * info[2]
 
 
+@pytest.mark.parametrize("cycle", [2, 0])
+@pytest.mark.parametrize("which", ["q1sys", "q2sys"])
+def test_w_and_f_cycles_match_oracle(request, which, cycle):
+    """W (SC_MG_CYCLE_SCALAR = 2) and F (0) cycles, TMultiGrid3D::SetRecursion
+    (MultiGrid3D.C:718-729) and the F-cycle's recursion reset (:620): one
+    damped-Jacobi cycle (independent of the row order) from a random right-hand
+    side equals the oracle's cycle to 1e-12 and differs from the V-cycle; on the
+    Q1 system FGMRES with the cycle reaches the oracle's solution to 1e-10
+    (damped Jacobi does not converge on the Q2 SUPG operator, in the oracle
+    either)."""
+    S, O, prm = request.getfixturevalue(which)
+    fine = O.levels[-1]
+    rng = np.random.default_rng(10 + cycle)
+    rhs = rng.standard_normal(fine.N)
+    sol0 = np.zeros(fine.N)
+    sol0[fine.active:] = rhs[fine.active:]
+    print(f"[case] cycle {cycle} {which}", flush=True)
+    one = dict(solver=0, smoother=1, omega=0.8, pre=2, post=2, coarse=17)
+    g, _, rc = S.solve(P.solver_opt(cycle=cycle, **one), rhs, sol0)
+    o, _ = O.solve(P.solver_opt(cycle=cycle, **one), rhs, sol0)
+    v, _ = O.solve(P.solver_opt(cycle=1, **one), rhs, sol0)
+    assert rc == 0 and rel(g, o) < 1e-12
+    assert rel(o, v) > 1e-6
+    if which != "q1sys":
+        return
+    rhs = O.rhs()
+    sol0[fine.active:] = rhs[fine.active:]
+    sol0[:fine.active] = 0.0
+    r0 = fine.defect(O.entries(), sol0, rhs)[1]
+    full = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=1e-12 * r0, red_factor=0.0, smoother=1,
+                        omega=0.8, cycle=cycle)
+    g, info, rc = S.solve(full, rhs, sol0)
+    o, _ = O.solve(full, rhs, sol0)
+    assert rc == 0 and rel(g, o) < 1e-10
+
+
 def test_ilu_smoother_is_rejected(q1sys):
     """Smoother id 4 (ILU in MultiGrid3D.C's switches) is not built: the solve
     reports an invalid argument instead of substituting another smoother."""

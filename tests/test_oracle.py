@@ -519,3 +519,23 @@ def test_fgmres_limit_inside_a_cycle():
         r = fine.defect(S.entries(), s, rhs)[1]
         assert r < prev
         prev = r
+
+
+def test_config1_as_specified_stagnates():
+    """Config 1 exactly as stated (32^3 Q1, SUPG delta0 = 0.25, V(2,2) damped
+    Jacobi omega = 0.8, FGMRES(20), 8^3 level solved exactly): the residual
+    stays at 0.997 of the initial one (40 iterations here; 400 give the same
+    0.9969), the consequence of the smoother's spectral radius > 1."""
+    prm = P.config1()
+    O = orc.System(3, 3, 1)
+    O.assemble(prm)
+    fine = O.levels[-1]
+    rhs = O.rhs()
+    sol0 = np.zeros(fine.N)
+    sol0[fine.active:] = rhs[fine.active:]
+    r0 = fine.defect(O.entries(), sol0, rhs)[1]
+    opt = P.solver_opt(solver=16, restart=20, maxit=40, res_norm_min=0.0, red_factor=0.0, smoother=1, omega=0.8,
+                       coarse=17)
+    s, info = O.solve(opt, rhs, sol0)
+    r = fine.defect(O.entries(), s, rhs)[1] / r0
+    assert 0.99 < r < 1.0, r
