@@ -1,9 +1,10 @@
 """Learned post-smoother on the GPU (libpmx through the C-ABI) against the oracle.
 
 Forward: BF16 operands, FP32 accumulation on MFMA vs the oracle with the same
-BF16 rounding and exact sums: relative 2-norm error <= 5e-3.  Gradients: the
-device also rounds the backward flow to BF16, so <= 3e-2 per layer (relative
-2-norm) with cosine similarity >= 0.999.  Adam: FP32 update of the same
+BF16 rounding and exact sums: relative 2-norm error <= 1e-3 (observed up to
+2.9e-4 on the 33^3 lattice).  Gradients: the device also rounds the backward
+flow to BF16, so <= 1e-2 per layer (relative 2-norm; observed up to 4.6e-3)
+with cosine similarity >= 0.9999.  Adam: FP32 update of the same
 gradient, 1e-6 relative.  Post-smoothing step vs the oracle's FP64 restatement
 of it: update within 3e-2 (relative 2-norm), defect never larger than before.
 """
@@ -44,7 +45,7 @@ def test_forward_vs_oracle(small, M):
     out = small.learned_forward(u)
     ref = LR.forward(p.astype(np.float64), u.astype(np.float64))
     print(f"forward rel err {nrel(out, ref):.3e}", flush=True)
-    assert nrel(out, ref) < 5e-3, nrel(out, ref)
+    assert nrel(out, ref) < 1e-3, nrel(out, ref)
 
 
 @pytest.mark.parametrize("M", [6, 17])
@@ -63,7 +64,7 @@ def test_gradient_vs_oracle(small, M):
         a, b = g[offs[l]:offs[l + 1]], g_ref[offs[l]:offs[l + 1]]
         cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
         print(f"gradient layer {l} rel err {nrel(a, b):.3e} cos {cos:.6f}", flush=True)
-        assert nrel(a, b) < 3e-2 and cos > 0.999, (l, nrel(a, b), cos)
+        assert nrel(a, b) < 1e-2 and cos > 0.9999, (l, nrel(a, b), cos)
 
 
 def test_adam_step(small):

@@ -330,7 +330,9 @@ def test_fgmres_limit_inside_a_cycle(q2sys, precision):
         assert rc in (0, 4) and np.all(np.isfinite(s)), (maxit, rc, info)
         assert np.isfinite(info[1]) and info[1] < info[2], (maxit, info)
         r, res = S.defect(s, rhs)
-        assert abs(res - info[1]) <= 1e-6 * res
+        # (the FP64 form reports the Givens estimate, which follows the true
+        # residual to rounding: 1e-14 of the initial residual near convergence)
+        assert abs(res - info[1]) <= 1e-6 * res + 1e-14 * info[2], (maxit, res, info)
         last = res
     assert last < 1e-3 * info[2]
 
