@@ -90,6 +90,17 @@ int pmx_system_rhs(pmx_system *sys, int level, double *rhs);
  * entries, 1 = FP32 copy of the entries. */
 int pmx_matvec(pmx_system *sys, int level, int precision, const double *x, double *y);
 int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, double *r, double *res);
+/* No counterpart in the reference: the allocation-free variant of pmx_defect
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  x, b, r are
+ * device arrays of the level's N nodes in lattice order (node (i, j, k) of the
+ * M^3 lattice at i + M (j + M k), M = pmx_system_dims[5]), res one device
+ * double: r = b - A x (Dirichlet rows r = b - x), *res = ||r||_2.  Ordered on
+ * the context's stream, no synchronisation, no allocation: the norm's partial
+ * sums live in a static device array of 4096 doubles per GPU (one per 2048
+ * rows), so levels of at most 8,388,608 rows are accepted (PMX_ERR_ARG
+ * beyond).  Calls that overlap on one GPU share that array and race. */
+int pmx_defect_noalloc(pmx_system *sys, int level, const double *x, const double *b, double *r, double *res);
 /* TMGLevel3D::Jacobi (kind 1) / SOR (2) / SSOR (3) (src/FE/MGLevel3D.C:280-563),
  * SOR/SSOR in multicolour order; `sweeps` sweeps, sol updated in place. */
 int pmx_smooth(pmx_system *sys, int level, int kind, double omega, int sweeps, double *sol, const double *f);

@@ -50,6 +50,7 @@ def lib():
         L.pmx_system_rhs.argtypes = [vp, ip, vp]
         L.pmx_matvec.argtypes = [vp, ip, ip, vp, vp]
         L.pmx_defect.argtypes = [vp, ip, vp, vp, vp, vp]
+        L.pmx_defect_noalloc.argtypes = [vp, ip, vp, vp, vp, vp]
         L.pmx_smooth.argtypes = [vp, ip, ip, dp, ip, vp, vp]
         L.pmx_prolongate.argtypes = [vp, ip, vp, vp]
         L.pmx_restrict.argtypes = [vp, ip, vp, vp]
@@ -205,6 +206,13 @@ class System:
         _ok(lib().pmx_defect(self.h, level, _p(np.ascontiguousarray(x, float)),
                              _p(np.ascontiguousarray(b, float)), _p(r), _p(res)), "pmx_defect")
         return r, float(res[0])
+
+    def defect_noalloc(self, x_ptr, b_ptr, r_ptr, res_ptr, level=-1):
+        """pmx_defect_noalloc on device arrays (addresses of level-sized FP64
+        lattice-ordered vectors and of one FP64 result); ordered on the
+        context's stream, no synchronisation."""
+        _ok(lib().pmx_defect_noalloc(self.h, level, ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr),
+                                     ctypes.c_void_p(r_ptr), ctypes.c_void_p(res_ptr)), "pmx_defect_noalloc")
 
     def smooth(self, kind, omega, sweeps, sol, f, level=-1):
         s = np.array(sol, float)

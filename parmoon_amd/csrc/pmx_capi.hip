@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -1156,6 +1157,39 @@ using namespace pmx;
 struct pmx_ctx { Ctx c; };
 struct pmx_system { System s; };
 
+// ---- scratch of the allocation-free entry point (pmx_defect_noalloc) ------
+// The partial sums of the defect norm live in this file-scope device array
+// (one copy per GPU, held until the process ends) instead of memory the
+// library allocates: one partial per kDotChunk rows, so levels up to
+// kNoAllocMaxRows rows fit.  The array's name is a host-side stand-in: host
+// code uses the address hipGetSymbolAddress returns for the current device,
+// kept per device, never the name itself.
+namespace pmx {
+constexpr int kNoAllocParts = 4096;
+constexpr int64_t kNoAllocMaxRows = (int64_t)kNoAllocParts * kDotChunk;  // 8,388,608
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocParts];
+static double *g_scratch_ptr[kNoAllocMaxDevices] = {nullptr};
+static std::mutex g_scratch_mutex;
+// the device address of `scratch` on the context's device (looked up once per
+// device, with that device current)
+static double *noalloc_scratch(const Ctx &c) {
+  if (c.device < 0 || c.device >= kNoAllocMaxDevices) throw std::invalid_argument("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[c.device]) {
+    int cur = 0;
+    check(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != c.device) check(hipSetDevice(c.device), "hipSetDevice");
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != c.device) (void)hipSetDevice(cur);
+    check(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[c.device] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[c.device];
+}
+}  // namespace pmx
+
 #define PMX_TRY(...)                                         \
   try {                                                      \
     __VA_ARGS__;                                                    \
@@ -1484,6 +1518,22 @@ int pmx_defect(pmx_system *sys, int level, const double *x, const double *b, dou
     dot<double>(lv.v64.d.p, lv.v64.d.p, lv.L.N, Y.ctx->part.p, nr, 1, st);
     if (res) *res = read_scalar(Y, nr);
     from_lattice<double>(Y, lv, lv.v64.d.p, r);
+  })
+}
+int pmx_defect_noalloc(pmx_system *sys, int level, const double *x, const double *b, double *r, double *res) {
+  PMX_TRY({
+    if (!sys || !x || !b || !r || !res) throw std::invalid_argument("NULL argument");
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    Level &lv = level_of(Y, level);
+    if (lv.L.N > kNoAllocMaxRows)
+      throw std::invalid_argument("pmx_defect_noalloc: the level has more than 8,388,608 rows (the scratch limit)");
+    double *part = noalloc_scratch(*Y.ctx);
+    hipStream_t st = Y.ctx->st;
+    css_apply<double>(lv.D, 1, lv.v64.val.p, x, b, r, 0.0, -1, st);
+    boundary<double>(lv.D, 1, r, x, b, st);
+    dot_chunked(r, r, lv.L.N, part, res, 1, st);
+    check(hipGetLastError(), "pmx_defect_noalloc launch");
   })
 }
 int pmx_smooth(pmx_system *sys, int level, int kind, double omega, int sweeps, double *sol, const double *f) {

@@ -885,6 +885,26 @@ template <typename T>
 void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_scale_inv<T>, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, nrm, n);
 }
+// dot product with one workgroup per kDotChunk elements (no grid-stride
+// loop): the number of partial sums grows with n, ceil(n / kDotChunk)
+__global__ __launch_bounds__(kBlock) void k_dot_chunk(const double *__restrict__ x, const double *__restrict__ y,
+                                                      double *__restrict__ part, int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * kDotChunk;
+  double s = 0;
+#pragma unroll
+  for (int k = 0; k < kDotChunk / kBlock; k++) {
+    const int64_t i = base + k * kBlock + threadIdx.x;
+    if (i < n) s += x[i] * y[i];
+  }
+  s = block_sum<double>(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+void dot_chunked(const double *x, const double *y, int64_t n, double *part, double *out, int do_sqrt,
+                 hipStream_t st) {
+  const int64_t nb = (n + kDotChunk - 1) / kDotChunk;
+  hipLaunchKernelGGL(k_dot_chunk, dim3((unsigned)nb), dim3(kBlock), 0, st, x, y, part, n);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, st, part, (int)nb, out, do_sqrt);
+}
 template <typename T>
 void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st) {
   int g = (int)((n + kBlock - 1) / kBlock);

@@ -54,6 +54,11 @@ template <typename T, typename U> void add_convert(T *y, const U *x, int64_t n, 
 template <typename T> void scale_inv(T *y, const T *x, const double *nrm, int64_t n, hipStream_t st);
 template <typename T>
 void dot(const T *x, const T *y, int64_t n, double *part, double *out, int do_sqrt, hipStream_t st);
+// out = (x, y) (sqrt with do_sqrt) with one partial sum per kDotChunk
+// elements in part (ceil(n / kDotChunk) doubles), summed in a fixed order
+constexpr int kDotChunk = 2048;
+void dot_chunked(const double *x, const double *y, int64_t n, double *part, double *out, int do_sqrt,
+                 hipStream_t st);
 // e = u - uold
 template <typename T> void axpy_diff(T *e, const T *u, const T *uold, int64_t n, hipStream_t st);
 // TMGLevel3D::StepLengthControl's update: omega = num_nom[0] / num_nom[1]
