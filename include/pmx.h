@@ -79,8 +79,28 @@ int pmx_space_colours(pmx_system *sys, int level, int *colour);
  * prm[PMX_PRM_EXAMPLE] (CoeffFct3D, include/General/Constants.h:148).
  * The result stays on the device. */
 int pmx_system_assemble(pmx_system *sys, const double *prm);
+/* The same with the caller's coefficient and boundary callbacks instead of a
+ * built-in functor: the host-function form of CoeffFct3D, BoundCondFunct3D and
+ * BoundValueFunct3D (include/General/Constants.h:128-129,148; BoundCond& and
+ * double& become int* and double*, which ParMooN's example functions match in
+ * the ABI).  coeffs is called once per cell with the cell's n quadrature
+ * points (X, Y, Z) and fills coeffs[q][0..5] = eps, b1, b2, b3, c, f
+ * (param[q] are zeros), as TDiscreteForm3D::GetLocalForms calls it
+ * (DiscreteForm3D.C:212-301); the records go to the device and the forms run
+ * there.  cond (optional) must return DIRICHLET (0) at every boundary node;
+ * value (optional, else 0) gives the Dirichlet rhs at each boundary node.
+ * bdcomp numbers the cube's planes as data/Wuerfel.PRM: 0 z = 0, 1 x = 0,
+ * 2 y = 0, 3 z = 1, 4 x = 1, 5 y = 1 (the first that holds the node).  The
+ * host evaluation costs n_q x n_cells callback points per level. */
+typedef void pmx_coeff_fct(int n_points, double *X, double *Y, double *Z, double **param, double **coeffs);
+typedef void pmx_bound_cond_fct(int bdcomp, double x, double y, double z, int *cond);
+typedef void pmx_bound_value_fct(int bdcomp, double x, double y, double z, double *value);
+int pmx_system_assemble_cb(pmx_system *sys, const double *prm, pmx_coeff_fct *coeffs, pmx_bound_cond_fct *cond,
+                           pmx_bound_value_fct *value);
 /* TSquareMatrix3D::GetEntries() of a level in CSR order; which = 0 the main
- * matrix, 1 the second matrix of two-matrix forms (SUPG K of TCD3D.C:128-266). */
+ * matrix (the operator of the solves), 1 the second matrix of two-matrix
+ * forms (SUPG K of TCD3D.C:128-266), 2 the mass matrix and 3 the stiffness
+ * matrix A of the time stepping. */
 int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries);
 /* right-hand side of a level (Rhs_array[level]) */
 int pmx_system_rhs(pmx_system *sys, int level, double *rhs);
@@ -144,6 +164,16 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  * residual [2] initial residual [3] outer corrections [4] seconds (device
  * events) [5] learned-smoother fallbacks [6..7] reserved. */
 int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, double *sol, double *info);
+/* TItMethod::Iterate(TSquareMatrix **A, TMatrix **B, double *sol, double *rhs)
+ * (include/FE/ItMethod.h:105) on a caller's matrix: the Krylov method of
+ * opt[0] works with the caller's CSR matrix of the finest level (n = N,
+ * RowPtr / KCol / Entries in the reference numbering, FP64, uploaded and
+ * permuted to the device order) and is preconditioned by the system's own
+ * assembled MG levels (TMultiGridScaIte).  With opt[15] >= 1 the FP64 defect
+ * correction uses the caller's matrix and the inner FP32 solves the system's
+ * operator.  sol, rhs, info as in pmx_system_solve. */
+int pmx_iterate(pmx_system *sys, int n, const int *rowptr, const int *kcol, const double *entries,
+                const double *opt, const double *rhs, double *sol, double *info);
 /* Step length control of the MG cycle (TMGLevel3D::StepLengthControl,
  * src/FE/MGLevel3D.C:908-960, applied as in TMultiGrid3D::Cycle,
  * src/FE/MultiGrid3D.C:465-471, 562-566, 671-679): on the levels it covers,
@@ -192,6 +222,24 @@ int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0);
  * (TimeDB->CURRENTTIMESTEPLENGTH / THETA1).  The solution stays on the device
  * (pmx_system_solution reads it). */
 int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const double *opt, double *info);
+/* The same step in TSystemTCD3D's phases (SystemTCD3D.C:229-330), for hosts
+ * that drive the time loop of 3DPrograms/TCD3D_ParMooN.C:565-640 call by call:
+ *  - AssembleARhs (:229-246): oldsol = sol, the stiffness form (and K of the
+ *    SUPG form) and the rhs at prm[PMX_PRM_TIME] on every level, sol's
+ *    Dirichlet values from the rhs; td as in pmx_tcd_step;
+ *  - AssembleSystMat (:248-300): the right-hand side B and the system matrix
+ *    M (+ K) + tau th1 A on every level, then oldrhs = rhs (the loop's copy);
+ *    PMX_ERR_STATE if the system is still assembled (the reference exits);
+ *  - Solve with opt (as pmx_system_solve; the solution stays on the device);
+ *  - RestoreMassMat (:312-330): M, K and A are kept apart, so nothing is
+ *    subtracted; the state returns to "not assembled" (PMX_ERR_STATE if it was
+ *    not assembled, where the reference exits).
+ * Between the phases pmx_matrix_entries reads the matrices: which 0 the
+ * operator (the system matrix once assembled), 1 K, 2 M, 3 A. */
+int pmx_tcd_assemble_arhs(pmx_system *sys, const double *prm, const double *td);
+int pmx_tcd_assemble_systmat(pmx_system *sys);
+int pmx_tcd_solve(pmx_system *sys, const double *opt, double *info);
+int pmx_tcd_restore_massmat(pmx_system *sys);
 
 /* ---- measurement ---------------------------------------------------------
  * Times `reps` back-to-back launches of one kernel on the context stream with

@@ -10,6 +10,7 @@
 //                                                         (include/System/SystemCD3D.h:129-140)
 //   TSystemTCD3D       AssembleMRhs, AssembleARhs, AssembleSystMat, Solve,
 //                      RestoreMassMat                     (include/System/SystemTCD3D.h)
+//   TAssembleMat3D     Init, Reset, Assemble3D, DeAllocate (include/System/AssembleMat3D.h)
 //
 // Differences: parameters come from an explicit block (pmx_prm.h) and an
 // option vector (pmx.h) instead of TDatabase; errors throw pmx::Error
@@ -151,12 +152,61 @@ class TSystemCD3D {
   std::vector<double> prm_;
 };
 
+// TAssembleMat3D (include/System/AssembleMat3D.h:53-91, Init / Reset /
+// Assemble3D / DeAllocate): the local form of a TDiscreteForm3D (a pmx_form id
+// in prm[PMX_PRM_FORM]) with the caller's CoeffFct3D, BoundCondFunct3D and
+// BoundValueFunct3D (pmx_system_assemble_cb), assembled on the device for all
+// levels of a system; Assemble3D fills the caller's entries of one level (the
+// matrix of TSquareStructure3D's RowPtr / KCol, and the second matrix of a
+// two-matrix form) and its rhs in place, as the reference accumulates into
+// caller-owned TSquareMatrix3D and rhs arrays.
+class TAssembleMat3D {
+ public:
+  TAssembleMat3D(pmx_system *system, int level, const double *prm, pmx_coeff_fct *coeffs,
+                 pmx_bound_cond_fct *cond, pmx_bound_value_fct *value, double *entries, double *rhs,
+                 double *entries2 = nullptr)
+      : h_(system), level_(level), prm_(prm, prm + PMX_PRM_LEN), coeffs_(coeffs), cond_(cond), value_(value),
+        entries_(entries), entries2_(entries2), rhs_(rhs) {
+    long long d[8];
+    check(pmx_system_dims(h_, level, d), "pmx_system_dims");
+    n_dof_ = d[0];
+    n_entries_ = d[3];
+  }
+  void Init() {}
+  // zero the caller's matrices and rhs (TAssembleMat3D::Reset)
+  void Reset() {
+    for (long long i = 0; i < n_entries_; i++) {
+      entries_[i] = 0.0;
+      if (entries2_) entries2_[i] = 0.0;
+    }
+    for (long long i = 0; i < n_dof_; i++) rhs_[i] = 0.0;
+  }
+  void Assemble3D() {
+    check(pmx_system_assemble_cb(h_, prm_.data(), coeffs_, cond_, value_), "pmx_system_assemble_cb");
+    check(pmx_matrix_entries(h_, level_, 0, entries_), "pmx_matrix_entries");
+    if (entries2_) check(pmx_matrix_entries(h_, level_, 1, entries2_), "pmx_matrix_entries");
+    check(pmx_system_rhs(h_, level_, rhs_), "pmx_system_rhs");
+  }
+  void DeAllocate() {}
+
+ private:
+  pmx_system *h_;
+  int level_;
+  std::vector<double> prm_;
+  pmx_coeff_fct *coeffs_;
+  pmx_bound_cond_fct *cond_;
+  pmx_bound_value_fct *value_;
+  double *entries_, *entries2_, *rhs_;
+  long long n_dof_ = 0, n_entries_ = 0;
+};
+
 // TSystemTCD3D with the time loop of 3DPrograms/TCD3D_ParMooN.C:565-663:
 //   AssembleMRhs();                       once
 //   per step: AssembleARhs(); AssembleSystMat(); Solve(sol); RestoreMassMat();
-// The device performs AssembleARhs + AssembleSystMat + Solve as one step
-// (pmx_tcd_step) when Solve is called; RestoreMassMat has nothing to undo
-// because the mass matrix is kept apart.
+// each call runs its phase on the device (pmx_tcd_assemble_arhs,
+// pmx_tcd_assemble_systmat, pmx_tcd_solve, pmx_tcd_restore_massmat), so the
+// rhs (after AssembleARhs) and the matrices (GetEntries) can be read between
+// them; RestoreMassMat subtracts nothing because M, K and A are kept apart.
 class TSystemTCD3D {
  public:
   TSystemTCD3D(Context &ctx, int N_levels, int nref_coarsest, int order, double *sol)
@@ -174,36 +224,39 @@ class TSystemTCD3D {
   // mass matrix and rhs at the current time; sol (the initial solution) is
   // taken as it is, with the Dirichlet values of the rhs
   void AssembleMRhs() { check(pmx_tcd_init(h_, prm_.data(), sol_), "pmx_tcd_init"); }
-  // time and theta of the coming step (TDatabase::TimeDB CURRENTTIME,
-  // CURRENTTIMESTEPLENGTH, THETA1..4 after SetTimeDiscParameters)
-  void AssembleARhs(double time, double tau, double th1, double th2, double th3, double th4) {
+  // the stiffness matrix and rhs at `time` with the step's tau and theta
+  // (TDatabase::TimeDB CURRENTTIME, CURRENTTIMESTEPLENGTH, THETA1..4 after
+  // SetTimeDiscParameters); rhs (optional): the assembled rhs of the finest
+  // level in the reference numbering
+  void AssembleARhs(double time, double tau, double th1, double th2, double th3, double th4,
+                    double *rhs = nullptr) {
     prm_[PMX_PRM_TIME] = time;
-    td_[0] = tau; td_[1] = th1; td_[2] = th2; td_[3] = th3; td_[4] = th4;
-    pending_ = true;
+    const double td[5] = {tau, th1, th2, th3, th4};
+    check(pmx_tcd_assemble_arhs(h_, prm_.data(), td), "pmx_tcd_assemble_arhs");
+    if (rhs) check(pmx_system_rhs(h_, -1, rhs), "pmx_system_rhs");
   }
-  void AssembleSystMat() {
-    if (!pending_) throw Error(PMX_ERR_STATE, "AssembleARhs first");
-  }
+  void AssembleSystMat() { check(pmx_tcd_assemble_systmat(h_), "pmx_tcd_assemble_systmat"); }
   int Solve(double *info = nullptr) {
-    if (!pending_) throw Error(PMX_ERR_STATE, "AssembleARhs first");
     double inf[8];
-    const int rc = pmx_tcd_step(h_, prm_.data(), td_, opt_.data(), inf);
-    pending_ = false;
+    const int rc = pmx_tcd_solve(h_, opt_.data(), inf);
     if (info)
       for (int i = 0; i < 8; i++) info[i] = inf[i];
-    check(rc, "pmx_tcd_step");
+    check(rc, "pmx_tcd_solve");
     check(pmx_system_solution(h_, sol_), "pmx_system_solution");
     return (int)inf[0];
   }
-  void RestoreMassMat() {}
+  void RestoreMassMat() { check(pmx_tcd_restore_massmat(h_), "pmx_tcd_restore_massmat"); }
+  // entries of a level's matrix: 0 the operator (the system matrix between
+  // AssembleSystMat and RestoreMassMat), 1 K, 2 M, 3 A
+  void GetEntries(int level, int which, double *entries) {
+    check(pmx_matrix_entries(h_, level, which, entries), "pmx_matrix_entries");
+  }
   pmx_system *handle() const { return h_; }
 
  private:
   pmx_system *h_ = nullptr;
   double *sol_;
   std::vector<double> prm_, opt_;
-  double td_[5] = {0};
-  bool pending_ = false;
 };
 
 }  // namespace pmx

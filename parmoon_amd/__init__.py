@@ -17,6 +17,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpmx.so")
 
 PRM_LEN = 512
+# the callback types of pmx_system_assemble_cb (CoeffFct3D, BoundCondFunct3D, BoundValueFunct3D)
+COEFF_FCT = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+                             ctypes.POINTER(ctypes.POINTER(ctypes.c_double)))
+BOUND_COND_FCT = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_int))
+BOUND_VALUE_FCT = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_double))
 LEARNED_IN = 16  # input channels of the learned smoother's network
 _lib = None
 
@@ -56,6 +64,8 @@ def lib():
         L.pmx_restrict.argtypes = [vp, ip, vp, vp]
         L.pmx_csr_matvec.argtypes = [vp, ip, vp, vp, vp, vp, vp]
         L.pmx_system_solve.argtypes = [vp, vp, vp, vp, vp]
+        L.pmx_iterate.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
+        L.pmx_system_assemble_cb.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
         L.pmx_system_set_step_length_control.argtypes = [vp, ctypes.c_int]
         L.pmx_system_solution.argtypes = [vp, vp]
@@ -63,6 +73,10 @@ def lib():
         L.pmx_time_kernel.argtypes = [vp, ip, ip, ip, ip, vp]
         L.pmx_tcd_init.argtypes = [vp, vp, vp]
         L.pmx_tcd_step.argtypes = [vp, vp, vp, vp, vp]
+        L.pmx_tcd_assemble_arhs.argtypes = [vp, vp, vp]
+        L.pmx_tcd_assemble_systmat.argtypes = [vp]
+        L.pmx_tcd_solve.argtypes = [vp, vp, vp]
+        L.pmx_tcd_restore_massmat.argtypes = [vp]
         L.pmx_learned_nparams.argtypes = [vp]
         L.pmx_learned_init.argtypes = [vp, ctypes.c_uint]
         L.pmx_learned_params.argtypes = [vp, ip, vp]
@@ -242,6 +256,49 @@ class System:
             _ok(rc, "pmx_system_solve")
         return s, info, rc
 
+    def iterate(self, rowptr, kcol, entries, opt, rhs, sol, check=True):
+        """pmx_iterate: the Krylov method of opt on the caller's CSR matrix of
+        the finest level, preconditioned by the system's MG levels."""
+        o = np.zeros(24)
+        o[:len(opt)] = opt
+        info = np.zeros(8)
+        s = np.array(sol, float)
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        rc = lib().pmx_iterate(self.h, len(rp) - 1, _p(rp), _p(np.ascontiguousarray(kcol, np.int32)),
+                               _p(np.ascontiguousarray(entries, float)), _p(o), _p(np.ascontiguousarray(rhs, float)),
+                               _p(s), _p(info))
+        if check:
+            _ok(rc, "pmx_iterate")
+        return s, info, rc
+
+    def assemble_cb(self, prm_block, coeffs, cond=None, value=None):
+        """pmx_system_assemble_cb with Python callables:
+        coeffs(X, Y, Z) -> array (n, 6) of eps, b1, b2, b3, c, f at the points;
+        cond(bdcomp, x, y, z) -> int; value(bdcomp, x, y, z) -> float."""
+        def cf(n, X, Y, Z, param, cptr):
+            x = np.ctypeslib.as_array(X, (n,))
+            y = np.ctypeslib.as_array(Y, (n,))
+            z = np.ctypeslib.as_array(Z, (n,))
+            c = np.asarray(coeffs(x, y, z), float)
+            for i in range(n):
+                row = cptr[i]
+                for k in range(6):
+                    row[k] = c[i, k]
+
+        def bc(comp, x, y, z, out):
+            out[0] = int(cond(comp, x, y, z))
+
+        def bv(comp, x, y, z, out):
+            out[0] = float(value(comp, x, y, z))
+
+        f1 = COEFF_FCT(cf)
+        f2 = BOUND_COND_FCT(bc) if cond else None
+        f3 = BOUND_VALUE_FCT(bv) if value else None
+        p = np.ascontiguousarray(prm_block, float)
+        _ok(lib().pmx_system_assemble_cb(self.h, _p(p), ctypes.cast(f1, ctypes.c_void_p),
+                                         ctypes.cast(f2, ctypes.c_void_p) if f2 else None,
+                                         ctypes.cast(f3, ctypes.c_void_p) if f3 else None), "pmx_system_assemble_cb")
+
     def set_step_length_control(self, mode):
         """MG step length control (TMGLevel3D::StepLengthControl): 0 off,
         1 finest level, 2 every level above the coarsest."""
@@ -288,6 +345,25 @@ class System:
         if check:
             _ok(rc, "pmx_tcd_step")
         return info, rc
+
+    def tcd_assemble_arhs(self, prm_block, td):
+        _ok(lib().pmx_tcd_assemble_arhs(self.h, _p(np.ascontiguousarray(prm_block, float)),
+                                        _p(np.ascontiguousarray(td, float))), "pmx_tcd_assemble_arhs")
+
+    def tcd_assemble_systmat(self):
+        _ok(lib().pmx_tcd_assemble_systmat(self.h), "pmx_tcd_assemble_systmat")
+
+    def tcd_solve(self, opt, check=True):
+        o = np.zeros(24)
+        o[:len(opt)] = opt
+        info = np.zeros(8)
+        rc = lib().pmx_tcd_solve(self.h, _p(o), _p(info))
+        if check:
+            _ok(rc, "pmx_tcd_solve")
+        return info, rc
+
+    def tcd_restore_massmat(self):
+        _ok(lib().pmx_tcd_restore_massmat(self.h), "pmx_tcd_restore_massmat")
 
     # ---- learned post-smoother -------------------------------------------
     @staticmethod

@@ -15,6 +15,7 @@
 
 #include "../../include/pmx.h"
 #include "pmx_internal.hpp"
+#include "pmx_functors.hpp"
 #include "pmx_learned.hpp"
 #include "pmx_ops.hpp"
 
@@ -79,6 +80,7 @@ struct Level {
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
   DevBuf<double> mass;                // time stepping: mass matrix M
+  DevBuf<double> stiff;               // time stepping: stiffness matrix A (the system matrix is in v64.val)
   DevBuf<double> rhs;                 // assembled rhs (lattice order)
   Vecs<double> v64;
   Vecs<float> v32;
@@ -112,8 +114,17 @@ struct System {
   // Krylov workspace
   DevBuf<double> sol, b, r64;         // finest level, FP64 (lattice)
   DevBuf<double> sol_try, r_try;      // FP32 form: the corrected iterate and its defect before acceptance
+  // a caller's finest-level CSR matrix (lattice-permuted) as the FP64 Krylov
+  // operator of pmx_iterate; the MG cycle keeps the system's own levels
+  DevBuf<int> ucsr_rp, ucsr_kc;
+  DevBuf<double> ucsr_val;
+  bool use_ucsr = false;
   DevBuf<double> oldrhs, oldsol, tmp1, tmp2, tmp3;  // time stepping (finest level)
   bool tcd_ready = false;
+  // phase of the time step (TSystemTCD3D's call order): 0 before
+  // AssembleARhs, 1 after it, 2 system matrix assembled (SystMatAssembled)
+  int tcd_phase = 0;
+  double td[5] = {0};
   DevBuf<double> krylov64;
   DevBuf<float> krylov32;
   DevBuf<double> scal;                // H, cs, sn, s, y, beta, res
@@ -626,16 +637,29 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(z, v.x.p, lv.L.N, st);
 }
 
-// y = A x on the finest level (stencil + Dirichlet identity rows)
+// y = A x on the finest level (stencil + Dirichlet identity rows); in FP64
+// the caller's CSR matrix instead while pmx_iterate runs
 template <typename T>
 static void fine_matvec(System &Y, const T *x, T *y) {
   Level &lv = *Y.lev.back();
+  if constexpr (std::is_same<T, double>::value) {
+    if (Y.use_ucsr) {
+      csr_spmv<double>((int)lv.L.N, Y.ucsr_rp.p, Y.ucsr_kc.p, Y.ucsr_val.p, x, y, Y.ctx->st);
+      return;
+    }
+  }
   css_apply<T>(lv.D, 0, vecs<T>(lv).val.p, x, nullptr, y, 0.0, -1, Y.ctx->st);
   boundary<T>(lv.D, 0, y, x, nullptr, Y.ctx->st);
 }
 template <typename T>
 static void fine_defect(System &Y, const T *x, const T *b, T *r) {
   Level &lv = *Y.lev.back();
+  if constexpr (std::is_same<T, double>::value) {
+    if (Y.use_ucsr) {
+      csr_defect<double>((int)lv.L.N, Y.ucsr_rp.p, Y.ucsr_kc.p, Y.ucsr_val.p, x, b, r, Y.ctx->st);
+      return;
+    }
+  }
   css_apply<T>(lv.D, 1, vecs<T>(lv).val.p, x, b, r, 0.0, -1, Y.ctx->st);
   boundary<T>(lv.D, 1, r, x, b, Y.ctx->st);
 }
@@ -1334,47 +1358,158 @@ int pmx_space_colours(pmx_system *sys, int level, int *colour) {
   })
 }
 
-int pmx_system_assemble(pmx_system *sys, const double *prm) {
-  PMX_TRY({
-    System &Y = sys->s;
-    if (!prm) throw std::invalid_argument("prm is NULL");
-    hipStream_t st = Y.ctx->st;
-    const int form = (int)prm[PMX_PRM_FORM];
-    if (form < PMX_FORM_GALERKIN || form > PMX_FORM_RHS) throw std::invalid_argument("unknown form");
-    Y.prm.upload(prm, PMX_PRM_LEN, st);
-    Y.example = (int)prm[PMX_PRM_EXAMPLE];
-    ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
-    const int nq = Y.nq;
-    if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
-      // Assemble3D with no matrices: only the right-hand side of every level
-      for (auto &lp : Y.lev)
-        assemble(lp->D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
-                 lp->rhs.p, st);
-      check(hipStreamSynchronize(st), "assemble");
-      return PMX_OK;
-    }
+namespace pmx {
+// The coefficient callbacks of a host caller (CoeffFct3D, BoundCondFunct3D,
+// BoundValueFunct3D of include/General/Constants.h:128-129,148).
+struct HostCallbacks {
+  pmx_coeff_fct *coeffs = nullptr;
+  pmx_bound_cond_fct *cond = nullptr;
+  pmx_bound_value_fct *value = nullptr;
+};
+// boundary component of a unit-cube node as data/Wuerfel.PRM numbers the
+// planes: 0 z = 0, 1 x = 0, 2 y = 0, 3 z = 1, 4 x = 1, 5 y = 1 (the first
+// one that holds the node)
+static int cube_component(double x, double y, double z) {
+  if (z == 0.0) return 0;
+  if (x == 0.0) return 1;
+  if (y == 0.0) return 2;
+  if (z == 1.0) return 3;
+  if (x == 1.0) return 4;
+  return 5;
+}
+// The quadrature-point records of every cell of a level from the caller's
+// CoeffFct3D, called once per cell with the cell's points as
+// TDiscreteForm3D::GetLocalForms does (DiscreteForm3D.C:212-301); coeffs[q]
+// holds 20 doubles (eps, b1, b2, b3, c, f at 0..5), param[q] 20 zeros.
+static void host_qp_records(const Level &lv, const double *prm, int form, bool linear, pmx_coeff_fct *cf,
+                            std::vector<double> &out) {
+  std::vector<double> qref, tab;
+  int nq;
+  make_tables(lv.L.order, linear, qref, tab, nq);
+  const int n = lv.L.n;
+  const double h = 1.0 / n;
+  out.assign((size_t)n * n * n * nq * 8, 0.0);
+  std::vector<double> X(nq), Yc(nq), Z(nq), cbuf((size_t)nq * 20), pbuf((size_t)nq * 20, 0.0);
+  std::vector<double *> cptr(nq), pptr(nq);
+  for (int q = 0; q < nq; q++) {
+    cptr[q] = &cbuf[(size_t)q * 20];
+    pptr[q] = &pbuf[(size_t)q * 20];
+  }
+  for (int cz = 0; cz < n; cz++)
+    for (int cy = 0; cy < n; cy++)
+      for (int cx = 0; cx < n; cx++) {
+        for (int q = 0; q < nq; q++) {
+          X[q] = (cx * h + 0.5 * h) + 0.5 * h * qref[4 * q];
+          Yc[q] = (cy * h + 0.5 * h) + 0.5 * h * qref[4 * q + 1];
+          Z[q] = (cz * h + 0.5 * h) + 0.5 * h * qref[4 * q + 2];
+        }
+        std::fill(cbuf.begin(), cbuf.end(), 0.0);
+        cf(nq, X.data(), Yc.data(), Z.data(), pptr.data(), cptr.data());
+        const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        for (int q = 0; q < nq; q++) {
+          const double *c = cptr[q];
+          QpCoeffs k;
+          k.eps = c[0]; k.b0 = c[1]; k.b1 = c[2]; k.b2 = c[3]; k.c = c[4]; k.f = c[5];
+          qp_record(prm, form, h, qref[4 * q + 3], k, &out[((size_t)cell * nq + q) * 8]);
+        }
+      }
+}
+// TSystemCD3D::Assemble of every level with the device form of prm and the
+// built-in coefficient functor of prm, or the caller's callbacks (cb)
+static void system_assemble(System &Y, const double *prm, const HostCallbacks *cb) {
+  hipStream_t st = Y.ctx->st;
+  const int form = (int)prm[PMX_PRM_FORM];
+  if (form < PMX_FORM_GALERKIN || form > PMX_FORM_RHS) throw std::invalid_argument("unknown form");
+  const bool linear = prm[PMX_PRM_LINEAR] != 0;
+  Y.prm.upload(prm, PMX_PRM_LEN, st);
+  Y.example = (int)prm[PMX_PRM_EXAMPLE];
+  ensure_tables(Y, linear);
+  const int nq = Y.nq;
+  const bool rhs_only = form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG;
+  if (!rhs_only) {
     Y.form = form;
     Y.two_matrices = form == PMX_FORM_TCD_SUPG_AK;
-    for (auto &lp : Y.lev) {
-      Level &lv = *lp;
+  }
+  std::vector<double> rec, bval;
+  DevBuf<double> dval;
+  for (auto &lp : Y.lev) {
+    Level &lv = *lp;
+    if (cb) {
+      // the records of the level's quadrature points from the caller's
+      // coefficients (the buffer is sized for the finest level)
+      host_qp_records(lv, prm, form, linear, cb->coeffs, rec);
+      check(hipMemcpyAsync(Y.qpbuf.p, rec.data(), sizeof(double) * rec.size(), hipMemcpyHostToDevice, st),
+            "upload qp records");
+    }
+    if (rhs_only) {
+      // Assemble3D with no matrices: only the right-hand side
+      assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
+               lv.rhs.p, st, cb != nullptr);
+    } else {
       if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
-               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
+               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr);
       lv.have_val32 = false;
       lv.have_dinv = false;
     }
-    check(hipStreamSynchronize(st), "assemble");
+    if (cb) {
+      // Dirichlet rows: the caller's BoundValue at each boundary node (the
+      // nodal functionals of AssembleMat3D.C:1031-1096); BoundCondition
+      // must report DIRICHLET there, the only condition the cube's
+      // structure holds
+      const int M = lv.L.M;
+      const double hn = 1.0 / (M - 1);
+      bval.resize(lv.L.boundary.size());
+      for (size_t b = 0; b < lv.L.boundary.size(); b++) {
+        const int p = lv.L.boundary[b];
+        const double x = (p % M) * hn, y = ((p / M) % M) * hn, z = (p / (M * M)) * hn;
+        const int comp = cube_component(x, y, z);
+        if (cb->cond) {
+          int c = -1;
+          cb->cond(comp, x, y, z, &c);
+          if (c != 0) throw std::invalid_argument("BoundCondition: only DIRICHLET (0) boundaries are supported");
+        }
+        double v = 0.0;
+        if (cb->value) cb->value(comp, x, y, z, &v);
+        bval[b] = v;
+      }
+      dval.upload(bval.data(), bval.size(), st);
+      scatter(lv.rhs.p, lv.boundary.p, dval.p, (int)bval.size(), st);
+      check(hipStreamSynchronize(st), "boundary values");
+    }
+  }
+  check(hipStreamSynchronize(st), "assemble");
+  if (!rhs_only) {
     coarse_inverse(Y, *Y.lev.front());
     Y.invalidate_graphs();
     Y.assembled = true;
+  }
+}
+}  // namespace pmx
+
+int pmx_system_assemble(pmx_system *sys, const double *prm) {
+  PMX_TRY({
+    if (!sys || !prm) throw std::invalid_argument("NULL argument");
+    system_assemble(sys->s, prm, nullptr);
+  })
+}
+int pmx_system_assemble_cb(pmx_system *sys, const double *prm, pmx_coeff_fct *coeffs, pmx_bound_cond_fct *cond,
+                           pmx_bound_value_fct *value) {
+  PMX_TRY({
+    if (!sys || !prm || !coeffs) throw std::invalid_argument("NULL argument (prm and coeffs are required)");
+    HostCallbacks cb;
+    cb.coeffs = coeffs;
+    cb.cond = cond;
+    cb.value = value;
+    system_assemble(sys->s, prm, &cb);
   })
 }
 // ---- time stepping: TSystemTCD3D (src/System/SystemTCD3D.C:208-428) driven as
 // 3DPrograms/TCD3D_ParMooN.C:565-663.  The mass matrix is kept apart, so the
 // system matrix M + tau theta1 A is rebuilt each step and no RestoreMassMat
 // pass is needed.
-// mass: the mass form into lv.mass; otherwise the stiffness form into the
-// operator values (and K of the SUPG form into lv.val2)
+// mass: the mass form into lv.mass; otherwise the stiffness form into
+// lv.stiff (and K of the SUPG form into lv.val2)
 static void assemble_into(System &Y, const double *prm, bool mass) {
   hipStream_t st = Y.ctx->st;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
@@ -1386,9 +1521,10 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
     if (mass) lv.mass.alloc(lv.L.nvals);
+    else lv.stiff.alloc(lv.L.nvals);
     if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
     assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
-             mass ? lv.mass.p : lv.v64.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
+             mass ? lv.mass.p : lv.stiff.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
   }
 }
 
@@ -1412,51 +1548,111 @@ int pmx_tcd_init(pmx_system *sys, const double *prm, const double *sol0) {
     boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // Dirichlet values of the rhs
     check(hipStreamSynchronize(st), "tcd init");
     Y.tcd_ready = true;
+    Y.tcd_phase = 0;
+  })
+}
+// ---- the phases of a time step (TSystemTCD3D, SystemTCD3D.C:229-330, in
+// the order of the time loop of 3DPrograms/TCD3D_ParMooN.C:565-640)
+namespace pmx {
+// AssembleARhs at prm[PMX_PRM_TIME]: oldsol = sol (the loop's memcpy), the
+// stiffness form (and K of the SUPG form) and the rhs on every level, the
+// Dirichlet values of sol from the rhs
+static void tcd_arhs(System &Y, const double *prm, const double *td) {
+  if (!Y.tcd_ready) throw std::logic_error("pmx_tcd_init first");
+  if (Y.tcd_phase == 2) throw std::logic_error("the system matrix has to be restored before AssembleARhs");
+  const int form = (int)prm[PMX_PRM_FORM];
+  if (form != PMX_FORM_TCD_A && form != PMX_FORM_GALERKIN && form != PMX_FORM_SDFEM &&
+      form != PMX_FORM_TCD_SUPG_AK)
+    throw std::invalid_argument("time stepping takes a stiffness form (TCD_A, GALERKIN, SDFEM, TCD_SUPG_AK)");
+  hipStream_t st = Y.ctx->st;
+  for (int k = 0; k < 5; k++) Y.td[k] = td[k];
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  check(hipMemcpyAsync(Y.oldsol.p, Y.sol.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+  // the SUPG form reads the step length and theta1 (TimeDB's
+  // CURRENTTIMESTEPLENGTH and THETA1, TCD3D.C:145-147) from td
+  std::vector<double> pa(prm, prm + PMX_PRM_LEN);
+  pa[PMX_PRM_TAU] = td[0];
+  pa[PMX_PRM_THETA1] = td[1];
+  assemble_into(Y, pa.data(), false);
+  boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // sol Dirichlet = rhs
+  Y.tcd_phase = 1;
+}
+// AssembleSystMat: B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol
+// on active rows, rhs on Dirichlet rows, system matrix M + tau th1 A on every
+// level; the SUPG form adds its time-derivative part K to M in both, as the
+// 2D system does (SystemTCD2D.C:294-297).  Then oldrhs = rhs (the loop's
+// memcpy after AssembleSystMat) and the coarse inverse of the new operator.
+static void tcd_systmat(System &Y) {
+  if (Y.tcd_phase == 2) throw std::logic_error("the system is assembled: RestoreMassMat first");
+  if (Y.tcd_phase != 1) throw std::logic_error("AssembleARhs first");
+  hipStream_t st = Y.ctx->st;
+  const double tau = Y.td[0], th1 = Y.td[1], th2 = Y.td[2], th3 = Y.td[3], th4 = Y.td[4];
+  Level &lv = *Y.lev.back();
+  const int64_t N = lv.L.N;
+  const bool supg = Y.two_matrices;
+  css_apply<double>(lv.D, 0, lv.mass.p, Y.oldsol.p, nullptr, Y.tmp1.p, 0.0, -1, st);
+  css_apply<double>(lv.D, 0, lv.stiff.p, Y.oldsol.p, nullptr, Y.tmp2.p, 0.0, -1, st);
+  if (supg) css_apply<double>(lv.D, 0, lv.val2.p, Y.oldsol.p, nullptr, Y.tmp3.p, 0.0, -1, st);
+  tcd_rhs(lv.D, Y.oldrhs.p, lv.rhs.p, Y.tmp1.p, Y.tmp2.p, supg ? Y.tmp3.p : nullptr, tau * th3, tau * th4,
+          tau * th2, Y.b.p, st);
+  for (auto &lp : Y.lev) {
+    Level &l = *lp;
+    system_matrix(l.v64.val.p, l.mass.p, supg ? l.val2.p : nullptr, l.stiff.p, tau * th1, l.L.nvals, st);
+    l.have_val32 = false;
+    l.have_dinv = false;
+  }
+  check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+  coarse_inverse(Y, *Y.lev.front());
+  Y.assembled = true;
+  Y.tcd_phase = 2;
+}
+}  // namespace pmx
+
+int pmx_tcd_assemble_arhs(pmx_system *sys, const double *prm, const double *td) {
+  PMX_TRY({
+    if (!sys || !prm || !td) throw std::invalid_argument("NULL argument");
+    tcd_arhs(sys->s, prm, td);
+    check(hipStreamSynchronize(sys->s.ctx->st), "AssembleARhs");
+  })
+}
+int pmx_tcd_assemble_systmat(pmx_system *sys) {
+  PMX_TRY({
+    if (!sys) throw std::invalid_argument("NULL argument");
+    tcd_systmat(sys->s);
+    check(hipStreamSynchronize(sys->s.ctx->st), "AssembleSystMat");
+  })
+}
+int pmx_tcd_solve(pmx_system *sys, const double *opt, double *info) {
+  PMX_TRY({
+    if (!sys || !opt) throw std::invalid_argument("NULL argument");
+    System &Y = sys->s;
+    if (Y.tcd_phase != 2) throw std::logic_error("AssembleSystMat first");
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    const int rc = solve_device(Y, info);
+    if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+int pmx_tcd_restore_massmat(pmx_system *sys) {
+  PMX_TRY({
+    if (!sys) throw std::invalid_argument("NULL argument");
+    System &Y = sys->s;
+    // M, K and A are kept apart, so nothing is subtracted; the system matrix
+    // is no longer assembled (SystemTCD3D.C:312-330)
+    if (Y.tcd_phase != 2) throw std::logic_error("the system is not assembled: nothing to restore");
+    Y.tcd_phase = 0;
   })
 }
 int pmx_tcd_step(pmx_system *sys, const double *prm, const double *td, const double *opt, double *info) {
   PMX_TRY({
     System &Y = sys->s;
     if (!prm || !td || !opt) throw std::invalid_argument("NULL argument");
-    if (!Y.tcd_ready) throw std::logic_error("pmx_tcd_init first");
-    const int form = (int)prm[PMX_PRM_FORM];
-    if (form != PMX_FORM_TCD_A && form != PMX_FORM_GALERKIN && form != PMX_FORM_SDFEM &&
-        form != PMX_FORM_TCD_SUPG_AK)
-      throw std::invalid_argument("time stepping takes a stiffness form (TCD_A, GALERKIN, SDFEM, TCD_SUPG_AK)");
-    hipStream_t st = Y.ctx->st;
-    const double tau = td[0], th1 = td[1], th2 = td[2], th3 = td[3], th4 = td[4];
-    Level &lv = *Y.lev.back();
-    const int64_t N = lv.L.N;
-    check(hipMemcpyAsync(Y.oldsol.p, Y.sol.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
-    // the SUPG form reads the step length and theta1 (TimeDB's
-    // CURRENTTIMESTEPLENGTH and THETA1, TCD3D.C:145-147) from td
-    std::vector<double> pa(prm, prm + PMX_PRM_LEN);
-    pa[PMX_PRM_TAU] = tau;
-    pa[PMX_PRM_THETA1] = th1;
-    assemble_into(Y, pa.data(), false);                          // AssembleARhs (SystemTCD3D.C:229-246)
-    boundary<double>(lv.D, 2, Y.sol.p, nullptr, lv.rhs.p, st);  // sol Dirichlet = rhs
-    // AssembleSystMat: B = tau th3 oldrhs + tau th4 rhs + (M - tau th2 A) oldsol;
-    // the SUPG form adds its time-derivative part K to M in both the rhs and
-    // the system matrix, as the 2D system does (SystemTCD2D.C:294-297; its
-    // RestoreMassMat takes it out again, :336-341; here M is kept apart)
-    const bool supg = Y.two_matrices;
-    css_apply<double>(lv.D, 0, lv.mass.p, Y.oldsol.p, nullptr, Y.tmp1.p, 0.0, -1, st);
-    css_apply<double>(lv.D, 0, lv.v64.val.p, Y.oldsol.p, nullptr, Y.tmp2.p, 0.0, -1, st);
-    if (supg) css_apply<double>(lv.D, 0, lv.val2.p, Y.oldsol.p, nullptr, Y.tmp3.p, 0.0, -1, st);
-    tcd_rhs(lv.D, Y.oldrhs.p, lv.rhs.p, Y.tmp1.p, Y.tmp2.p, supg ? Y.tmp3.p : nullptr, tau * th3, tau * th4,
-            tau * th2, Y.b.p, st);
-    // system matrices M (+ K) + tau th1 A on every level
-    for (auto &lp : Y.lev) {
-      Level &l = *lp;
-      mass_plus(l.v64.val.p, l.mass.p, supg ? l.val2.p : nullptr, tau * th1, l.L.nvals, st);
-      l.have_val32 = false;
-      l.have_dinv = false;
-    }
-    check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
-    coarse_inverse(Y, *Y.lev.front());
-    Y.assembled = true;
+    if (Y.tcd_phase == 2) Y.tcd_phase = 0;  // a step after a bare solve restores first
+    tcd_arhs(Y, prm, td);
+    tcd_systmat(Y);
     std::memcpy(Y.opt, opt, sizeof(Y.opt));
     const int rc = solve_device(Y, info);
+    Y.tcd_phase = 0;
     if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
   })
 }
@@ -1466,12 +1662,14 @@ int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
     System &Y = sys->s;
     if (!Y.assembled) throw std::logic_error("assemble first");
     Level &lv = level_of(Y, level);
-    if (which == 1 && !lv.val2.p) throw std::logic_error("no second matrix assembled");
+    const double *src = which == 0 ? lv.v64.val.p : which == 1 ? lv.val2.p : which == 2 ? lv.mass.p
+                      : which == 3 ? lv.stiff.p : nullptr;
+    if (which < 0 || which > 3) throw std::invalid_argument("which: 0 operator, 1 K, 2 mass, 3 stiffness");
+    if (!src) throw std::logic_error("that matrix has not been assembled");
     ensure_csr_dev(Y, lv);
     DevBuf<double> out;
     out.alloc(lv.Nb.kcol.size());
-    css_to_csr<double>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, which ? lv.val2.p : lv.v64.val.p,
-                       out.p, Y.ctx->st);
+    css_to_csr<double>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, src, out.p, Y.ctx->st);
     check(hipMemcpyAsync(entries, out.p, sizeof(double) * out.n, hipMemcpyDeviceToHost, Y.ctx->st), "d2h");
     check(hipStreamSynchronize(Y.ctx->st), "sync");
   })
@@ -1598,6 +1796,51 @@ int pmx_system_solve(pmx_system *sys, const double *opt, const double *rhs, doub
     to_lattice<double>(Y, lv, rhs, Y.b.p);
     to_lattice<double>(Y, lv, sol, Y.sol.p);
     const int rc = solve_device(Y, info);
+    from_lattice<double>(Y, lv, Y.sol.p, sol);
+    if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
+  })
+}
+int pmx_iterate(pmx_system *sys, int n, const int *rowptr, const int *kcol, const double *entries,
+                const double *opt, const double *rhs, double *sol, double *info) {
+  PMX_TRY({
+    if (!sys || !rowptr || !kcol || !entries || !opt || !rhs || !sol) throw std::invalid_argument("NULL argument");
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first: the MG levels precondition the caller's matrix");
+    Level &lv = *Y.lev.back();
+    const int64_t N = lv.L.N;
+    if (n != N) throw std::invalid_argument("the matrix does not have the finest level's size");
+    // the caller's rows and columns in lattice order
+    const int64_t nnz = rowptr[n];
+    if (rowptr[0] != 0 || nnz < n) throw std::invalid_argument("malformed RowPtr");
+    std::vector<int> rp(N + 1), kc(nnz);
+    std::vector<double> v(nnz);
+    rp[0] = 0;
+    for (int64_t p = 0; p < N; p++) {
+      const int r = lv.Nb.lat2dof[p];
+      int64_t o = rp[p];
+      for (int k = rowptr[r]; k < rowptr[r + 1]; k++, o++) {
+        if (kcol[k] < 0 || kcol[k] >= n) throw std::invalid_argument("column index out of range");
+        kc[o] = lv.Nb.dof2lat[kcol[k]];
+        v[o] = entries[k];
+      }
+      rp[p + 1] = (int)o;
+    }
+    hipStream_t st = Y.ctx->st;
+    Y.ucsr_rp.upload(rp.data(), rp.size(), st);
+    Y.ucsr_kc.upload(kc.data(), kc.size(), st);
+    Y.ucsr_val.upload(v.data(), v.size(), st);
+    std::memcpy(Y.opt, opt, sizeof(Y.opt));
+    to_lattice<double>(Y, lv, rhs, Y.b.p);
+    to_lattice<double>(Y, lv, sol, Y.sol.p);
+    Y.use_ucsr = true;
+    int rc;
+    try {
+      rc = solve_device(Y, info);
+    } catch (...) {
+      Y.use_ucsr = false;
+      throw;
+    }
+    Y.use_ucsr = false;
     from_lattice<double>(Y, lv, Y.sol.p, sol);
     if (rc) { g_err = rc == PMX_ERR_NOCONV ? "iteration limit reached" : "diverged"; return rc; }
   })

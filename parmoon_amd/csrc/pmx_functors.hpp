@@ -169,4 +169,28 @@ __host__ __device__ inline double rhs_supg_tau(double hK, double eps, double b1,
   return tau * hK;
 }
 
+// The quadrature-point record of the assembly, {w |detJ|, eps, b0, b1, b2, c,
+// f, delta}, for a point of a brick cell of side h with rule weight w and the
+// coefficients c at the point: |detJ| = h^3 / 8 (THexaAffin), hK = sqrt(3) h
+// (THexahedron::GetDiameter of a cube), delta the streamline parameter of
+// the form (0 for the forms without one).
+__host__ __device__ inline void qp_record(const double *prm, int form, double h, double w, const QpCoeffs &c,
+                                          double *o) {
+  const double hK = sqrt(3.0) * h;
+  double delta = 0;
+  if (form == PMX_FORM_SDFEM || form == PMX_FORM_MASS_SUPG) {
+    // BilinearAssemble_SD and MatrixMRhsAssemble_SUPG (TCD3D.C:103-104):
+    // Compute_SDFEM_delta of the unscaled coefficients
+    delta = sdfem_delta(prm, hK, c.eps, c.b0, c.b1, c.b2);
+  } else if (form == PMX_FORM_RHS_SUPG) {
+    delta = rhs_supg_tau(hK, c.eps, c.b0, c.b1, c.b2);
+  } else if (form == PMX_FORM_TCD_SUPG_AK) {
+    // MatricesAKRhsAssemble_SUPG (TCD3D.C:167-181, 206-208): delta of the
+    // theta1 tau scaled coefficients, times theta1 tau
+    const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
+    delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
+  }
+  o[0] = w * (h * h * h / 8.0);
+  o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = delta;
+}
 }  // namespace pmx

@@ -457,21 +457,7 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const double X = (cx * h + 0.5 * h) + 0.5 * h * xi;
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
-  const QpCoeffs c = example_coeffs(prm, X, Y, Z);
-  const double hK = sqrt(3.0) * h;  // THexahedron::GetDiameter of a cube
-  double delta = 0;
-  if (form == PMX_FORM_SDFEM || form == PMX_FORM_MASS_SUPG) {
-    // BilinearAssemble_SD and MatrixMRhsAssemble_SUPG (TCD3D.C:103-104):
-    // Compute_SDFEM_delta of the unscaled coefficients
-    delta = sdfem_delta(prm, hK, c.eps, c.b0, c.b1, c.b2);
-  } else if (form == PMX_FORM_RHS_SUPG) {
-    delta = rhs_supg_tau(hK, c.eps, c.b0, c.b1, c.b2);
-  } else if (form == PMX_FORM_TCD_SUPG_AK) {
-    const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
-    delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
-  }
-  o[0] = w * (h * h * h / 8.0);
-  o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = delta;
+  qp_record(prm, form, h, w, example_coeffs(prm, X, Y, Z), o);
 }
 // Stage 1 (coefficient functions that are expensive to evaluate): the
 // coefficients of every quadrature point of every cell, once.
@@ -625,6 +611,13 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble(const BlockInfo *__r
   }
 #undef PMX_AROW
 }
+__global__ void k_scatter(double *__restrict__ y, const int *__restrict__ idx, const double *__restrict__ v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[idx[i]] = v[i];
+}
+void scatter(double *y, const int *idx, const double *v, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, y, idx, v, n);
+}
 // Dirichlet rhs: RHS[dof] = g(node) (AssembleMat3D.C:1031-1096)
 __global__ void k_boundary_rhs(const int *__restrict__ bnd, int nbd, int M, const double *__restrict__ prm,
                                double *__restrict__ rhs) {
@@ -671,14 +664,15 @@ __global__ void k_gather(T *__restrict__ out, const U *__restrict__ in, const in
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_csr_spmv(int n, const int *__restrict__ rowptr,
                                                      const int *__restrict__ kcol, const T *__restrict__ val,
-                                                     const T *__restrict__ x, T *__restrict__ y) {
+                                                     const T *__restrict__ x, T *__restrict__ y,
+                                                     const T *__restrict__ b) {
   const int lane = threadIdx.x & 15;
   const int row = (blockIdx.x * kBlock + threadIdx.x) >> 4;
   if (row >= n) return;
   double s = 0;
   for (int j = rowptr[row] + lane; j < rowptr[row + 1]; j += 16) s += (double)val[j] * (double)x[kcol[j]];
   for (int o = 8; o > 0; o >>= 1) s += __shfl_down(s, o, 16);
-  if (lane == 0) y[row] = (T)s;
+  if (lane == 0) y[row] = b ? (T)((double)b[row] - s) : (T)s;
 }
 
 // ============================================================ dense coarse solve
@@ -1193,7 +1187,12 @@ void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st) {
 template <typename T>
 void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st) {
   const int g = (int)(((int64_t)n * 16 + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(k_csr_spmv<T>, dim3(g), dim3(kBlock), 0, st, n, rp, kc, val, x, y);
+  hipLaunchKernelGGL(k_csr_spmv<T>, dim3(g), dim3(kBlock), 0, st, n, rp, kc, val, x, y, (const T *)nullptr);
+}
+template <typename T>
+void csr_defect(int n, const int *rp, const int *kc, const T *val, const T *x, const T *b, T *r, hipStream_t st) {
+  const int g = (int)(((int64_t)n * 16 + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(k_csr_spmv<T>, dim3(g), dim3(kBlock), 0, st, n, rp, kc, val, x, r, b);
 }
 template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
@@ -1370,11 +1369,12 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble_kc(const BlockInfo *
 }
 
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
-              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st) {
+              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
+              bool qp_given) {
   const int64_t ncells = (int64_t)D.n * D.n * D.n;
   const int64_t ncq = ncells * nq;
-  const bool inl = coeffs_inline(example);
-  if (!inl)
+  const bool inl = !qp_given && coeffs_inline(example);
+  if (!inl && !qp_given)
     hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
                        nq, D.n, qpbuf);
   if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
@@ -1528,13 +1528,6 @@ __global__ void k_interpolate(const double *__restrict__ prm, int M, double *__r
     x[i] = example_exact(prm, a * h, b * h, c * h);
   }
 }
-// val = mass (+ k) + c * val (MatAdd of Blas2.C:128-150 on the stored stencil
-// values; k: the time-derivative part of the SUPG form, or nullptr)
-__global__ void k_mass_plus(double *__restrict__ val, const double *__restrict__ mass,
-                            const double *__restrict__ k, double c, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    val[i] = (k ? mass[i] + k[i] : mass[i]) + c * val[i];
-}
 // right-hand side of TSystemTCD3D::AssembleSystMat (SystemTCD3D.C:255-283):
 // active rows B = t3 oldrhs + t4 rhs + (M (+ K) - t2 A) oldsol, Dirichlet rows
 // B = rhs (k_old = K oldsol of the SUPG form, SystemTCD2D.C:294-297, or nullptr)
@@ -1553,8 +1546,15 @@ __global__ void k_tcd_rhs(int M, const double *__restrict__ oldrhs, const double
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st) {
   hipLaunchKernelGGL(k_interpolate, dim3(grid_for(D.N)), dim3(kBlock), 0, st, prm_dev, D.M, x);
 }
-void mass_plus(double *val, const double *mass, const double *k, double c, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_mass_plus, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, k, c, n);
+// val = mass (+ k) + c a (MatAdd of Blas2.C:128-150 on the stored stencil values)
+__global__ void k_system_matrix(double *__restrict__ val, const double *__restrict__ mass,
+                                const double *__restrict__ k, const double *__restrict__ a, double c, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    val[i] = (k ? mass[i] + k[i] : mass[i]) + c * a[i];
+}
+void system_matrix(double *val, const double *mass, const double *k, const double *a, double c, int64_t n,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_system_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, k, a, c, n);
 }
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
              const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st) {
@@ -1597,6 +1597,7 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
   template void restrict_<T>(const DevLayout &, const DevLayout &, T *, const T *, hipStream_t, bool);      \
   template void gemv_coarse<T>(int, int, const double *, const int *, const T *, T *, hipStream_t);         \
   template void csr_spmv<T>(int, const int *, const int *, const T *, const T *, T *, hipStream_t);         \
+  template void csr_defect<T>(int, const int *, const int *, const T *, const T *, const T *, T *, hipStream_t); \
   template void css_to_csr<T>(const DevLayout &, const int *, const int *, const int *, int64_t, const T *, \
                               double *, hipStream_t);
 PMX_INST(double)

@@ -128,22 +128,30 @@ void coarse_inverse_dense(const DevLayout &D, const double *val, const int *cidx
 template <typename T, typename U> void gather(T *out, const U *in, const int *map, int64_t n, hipStream_t st);
 template <typename T>
 void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *y, hipStream_t st);
+// r = b - A x on a CSR matrix
+template <typename T>
+void csr_defect(int n, const int *rp, const int *kc, const T *val, const T *x, const T *b, T *r, hipStream_t st);
 template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
                 const T *val, double *out, hipStream_t st);
 // example: prm[PMX_PRM_EXAMPLE] (selects evaluating the coefficients inline or
 // through the quadrature-point buffer); form: prm[PMX_PRM_FORM]; kc: cell
 // buffer of n^3 x 27 x 28 doubles for the Q2 one-matrix forms on MFMA
-// (nullptr: the per-row evaluation)
+// (nullptr: the per-row evaluation); qp_given: qpbuf already holds the
+// quadrature-point records (coefficients from host callbacks)
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
-              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st);
+              const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
+              bool qp_given = false);
+// y[idx[i]] = v[i]
+void scatter(double *y, const int *idx, const double *v, int n, hipStream_t st);
 constexpr int64_t kCellMatrixDoubles = 27 * 28;  // cell buffer entries per Q2 cell
 // time stepping (TSystemTCD3D): nodal interpolation of the example's initial
-// function, val = mass (+ k) + c val, and the Crank-Nicolson / theta
-// right-hand side (k, k_old: the SUPG time-derivative matrix K and K oldsol,
-// nullptr for the Galerkin forms)
+// function, the system matrix val = mass (+ k) + c a, and the Crank-Nicolson
+// / theta right-hand side (k, k_old: the SUPG time-derivative matrix K and
+// K oldsol, nullptr for the Galerkin forms)
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st);
-void mass_plus(double *val, const double *mass, const double *k, double c, int64_t n, hipStream_t st);
+void system_matrix(double *val, const double *mass, const double *k, const double *a, double c, int64_t n,
+                   hipStream_t st);
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
              const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st);
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st);

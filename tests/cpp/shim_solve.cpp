@@ -3,12 +3,37 @@
 //   shim_solve <dir>
 // reads <dir>/prm.bin, <dir>/opt.bin (CD3D) and <dir>/prm_tcd.bin,
 // <dir>/opt_tcd.bin, <dir>/td.bin, <dir>/u0.bin (TCD3D), and writes
-// <dir>/sol.bin, <dir>/rhs.bin, <dir>/entries.bin and <dir>/sol_tcd.bin.
+// <dir>/sol.bin, <dir>/rhs.bin, <dir>/entries.bin, <dir>/sol_tcd.bin and the
+// entries / rhs of Laplace.h's callbacks through TAssembleMat3D and of the
+// built-in functor (<dir>/laplace_{cb,fn}_{entries,rhs}.bin).
 #include <cstdio>
 #include <string>
 #include <vector>
 
+#include <cmath>
+
 #include "pmx_parmoon.hpp"
+
+// Examples/CD_3D/Laplace.h's callbacks with the reference's signatures (the
+// BoundCond& / double& of BoundCondFunct3D / BoundValueFunct3D as pointers);
+// eps = 1 / RE_NR with RE_NR = 50 here
+static const double kPi = 3.14159265358979323846;
+static void BoundCondition(int, double, double, double, int *cond) { *cond = 0; /* DIRICHLET */ }
+static void BoundValue(int, double x, double y, double z, double *value) {
+  *value = std::sin(kPi * x) * std::sin(kPi * y) * std::sin(kPi * z);
+}
+static void BilinearCoeffs(int n_points, double *x, double *y, double *z, double **, double **coeffs) {
+  const double eps = 1.0 / 50.0;
+  for (int i = 0; i < n_points; i++) {
+    double *coeff = coeffs[i];
+    coeff[0] = eps;
+    coeff[1] = 0;
+    coeff[2] = 0;
+    coeff[3] = 0;
+    coeff[4] = 0;
+    coeff[5] = (3 * kPi * kPi * eps) * std::sin(kPi * x[i]) * std::sin(kPi * y[i]) * std::sin(kPi * z[i]);
+  }
+}
 
 static std::vector<double> load(const std::string &f, size_t n) {
   std::vector<double> v(n);
@@ -47,6 +72,34 @@ int main(int argc, char **argv) {
       save(dir + "/rhs.bin", rhs.data(), N);
       save(dir + "/entries.bin", entries.data(), entries.size());
       std::printf("cd3d iterations %d residual %.3e\n", its, info[1]);
+    }
+    // TAssembleMat3D with Laplace.h's callbacks against the built-in Laplace
+    // functor (3 levels 2^2..2^4 Q2, Galerkin): both sets of entries and rhs
+    {
+      pmx::TFESpace3D fine(4, 2);
+      pmx::TSquareStructure3D structure(fine);
+      const int N = fine.GetN_DegreesOfFreedom();
+      std::vector<double> sol(N, 0.0), rhs_cb(N), rhs_fn(N), e_cb(structure.GetN_Entries()), e_fn(e_cb.size());
+      pmx::TSystemCD3D system(ctx, 3, 2, 2, sol.data(), rhs_fn.data());
+      std::vector<double> prm(PMX_PRM_LEN, 0.0);
+      prm[PMX_PRM_EXAMPLE] = PMX_EX_LAPLACE;
+      prm[PMX_PRM_EPS] = 1.0 / 50.0;
+      prm[PMX_PRM_FORM] = PMX_FORM_GALERKIN;
+      prm[PMX_PRM_POLYDEG] = 1;
+      system.Init(prm.data());
+      system.Assemble();
+      system.GetEntries(-1, e_fn.data());
+      pmx::TAssembleMat3D am(system.handle(), -1, prm.data(), BilinearCoeffs, BoundCondition, BoundValue,
+                             e_cb.data(), rhs_cb.data());
+      am.Init();
+      am.Reset();
+      am.Assemble3D();
+      am.DeAllocate();
+      save(dir + "/laplace_cb_entries.bin", e_cb.data(), e_cb.size());
+      save(dir + "/laplace_fn_entries.bin", e_fn.data(), e_fn.size());
+      save(dir + "/laplace_cb_rhs.bin", rhs_cb.data(), N);
+      save(dir + "/laplace_fn_rhs.bin", rhs_fn.data(), N);
+      std::printf("laplace callbacks assembled\n");
     }
     // TCD3D: 4 levels 2^1..2^4 Q1, Crank-Nicolson
     {

@@ -83,6 +83,16 @@ def test_shim_solve_and_time_steps():
                                   smoother=3, omega=1.0), C.rhs(), sol0)
     sol = np.fromfile(os.path.join(d, "sol.bin"))
     assert np.max(np.abs(sol - ref)) < 1e-10 * np.max(np.abs(ref))
+    # Laplace.h's callbacks through TAssembleMat3D == the built-in functor
+    e_cb = np.fromfile(os.path.join(d, "laplace_cb_entries.bin"))
+    e_fn = np.fromfile(os.path.join(d, "laplace_fn_entries.bin"))
+    r_cb = np.fromfile(os.path.join(d, "laplace_cb_rhs.bin"))
+    r_fn = np.fromfile(os.path.join(d, "laplace_fn_rhs.bin"))
+    _, _, rp2, _ = orc.Space(4, 2).arrays()
+    rows2 = np.repeat(np.arange(len(rp2) - 1), np.diff(rp2))
+    scale2 = np.maximum.reduceat(np.abs(e_fn), rp2[:-1])
+    assert np.max(np.abs(e_cb - e_fn) / scale2[rows2]) <= 1e-14
+    assert np.max(np.abs(r_cb - r_fn)) <= 1e-14 * np.max(np.abs(r_fn))
     # TCD3D against the oracle's time loop
     o_sol, _ = O.tcd3d(prm_t, [tau, 1e9, .5, .5, .5, .5], opt_t, u0, max_steps=steps)
     s_t = np.fromfile(os.path.join(d, "sol_tcd.bin"))
