@@ -1,0 +1,53 @@
+"""The DOF numbering and CSR pattern against ParMooN's own code.
+
+tests/golden/reference_numbering.npz holds BeginIndex, GlobalNumbers, RowPtr
+and KCol that the reference's mesh / FE-space / structure sources computed
+(built with g++ by oracle/ref.mk, written by tests/golden/
+make_numbering_fixtures.py) for the unit cube refined 1-2 times (Q1, Q2) and
+3 times (Q1).  The oracle's restatement and libpmx's host set-up must equal
+them bit for bit.  Where the reference build and tree are present (this
+container), larger spaces are compared live: 16^3 and 32^3 Q1, 8^3 Q2.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import orc
+import parmoon_amd as pa
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_numbering_fixtures as MF  # noqa: E402
+
+FIX = np.load(os.path.join(HERE, "golden", "reference_numbering.npz"))
+REF = "/root/reference"
+
+
+def check_equal(nref, order, N, act, bi, gn, rp, kc):
+    sp = orc.Space(nref, order)
+    assert (sp.N, sp.active) == (N, act)
+    o = sp.arrays()
+    d, *h = pa.host_space(nref, order)
+    for name, ref, got_o, got_h in zip(("BeginIndex", "GlobalNumbers", "RowPtr", "KCol"), (bi, gn, rp, kc), o, h):
+        assert np.array_equal(got_o, ref), ("oracle", name, nref, order)
+        assert np.array_equal(got_h, ref), ("libpmx", name, nref, order)
+
+
+@pytest.mark.parametrize("nref,order", MF.CASES)
+def test_numbering_equals_reference_fixture(nref, order):
+    k = f"n{nref}_q{order}"
+    N, act = (int(v) for v in FIX[k + "_dims"])
+    check_equal(nref, order, N, act, FIX[k + "_begin_index"], FIX[k + "_global_numbers"], FIX[k + "_rowptr"],
+                FIX[k + "_kcol"])
+
+
+@pytest.mark.parametrize("nref,order", [(4, 1), (3, 2), (5, 1)])
+def test_numbering_equals_reference_live(nref, order):
+    if not (os.path.exists(MF.EXE) and os.path.isdir(os.path.join(REF, "data"))):
+        pytest.skip("the reference build (oracle/_ref) or the reference tree is not here")
+    N, act, bi, gn, rp, kc = MF.run_reference(nref, order, REF)
+    if (nref, order) == (4, 1):
+        assert (N, len(kc)) == (4913, 92663)  # the golden log's sizes (ReadIn/tcd3d_GMG.out:386,392-394)
+    check_equal(nref, order, N, act, bi, gn, rp, kc)
