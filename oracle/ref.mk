@@ -2,9 +2,10 @@
 # mesh, refinement, FE-space and matrix-structure code compiled with g++ from
 # the sources where they lie under the reference tree (REF), nothing copied,
 # nothing substituted.  The sources are archived and the driver links against
-# the archive, so only the objects its calls reach are linked.  Files the
-# sequential 3D build cannot compile here without libraries this image lacks
-# (UMFPACK, TECIO) are left out; nothing the driver calls reaches them.
+# the archive, so only the objects its calls reach are linked.  The file list
+# is the reference's sequential 3D build (oracle/ref_sources.py reads its
+# CMakeLists); files that need libraries this image lacks (UMFPACK, TECIO)
+# are left out, and nothing the drivers call reaches them.
 # LAPACK (reached through LinAlg.o from THexaAffin, not called on the
 # numbering path) is the OpenBLAS that scipy ships in this image, whose
 # Fortran symbols carry a scipy_ prefix; the compiler maps the names
@@ -18,13 +19,11 @@ INC := $(addprefix -I$(REF)/include/,General FE Geometry QuadFormulas FE1D FE2D 
        -I$(REF)/Examples -I$(REF)/EXT_LIB/tetgen
 LAPACK_NAMES := $(foreach f,dgetrf dgetrs dgetri dgemm dsptrd dopgtr dsteqr,-D$(f)_=scipy_$(f)_)
 FLAGS := -O2 -w -fpermissive -D__3D__ -D_SEQ $(LAPACK_NAMES)
-SRCS := $(wildcard $(REF)/src/General/*.C $(REF)/src/Geometry/*.C $(REF)/src/Refinement/*.C \
-          $(REF)/src/QuadFormulas/*.C $(REF)/src/FE/*.C)
-SKIP := DirectSolver Output3D Output2D 2d3d_FDM Assemble3D_bkup RKV_FDM TCD2D TimeConvDiff2D Windtunnel_3d4d
-SRCS := $(filter-out $(foreach s,$(SKIP),%/$(s).C),$(SRCS))
+# the sources of the reference's own sequential 3D build (its CMakeLists)
+SRCS := $(shell python3 oracle/ref_sources.py $(REF))
 OBJS := $(patsubst $(REF)/src/%.C,$(OUT)/obj/%.o,$(SRCS)) $(OUT)/obj/tetgen.o
 
-all: $(OUT)/ref_numbering
+all: $(OUT)/ref_numbering $(OUT)/ref_assemble
 
 $(OUT)/obj/%.o: $(REF)/src/%.C
 	@mkdir -p $(dir $@)
@@ -43,4 +42,8 @@ OPENBLAS := $(notdir $(firstword $(wildcard $(SCIPY_LIBS)/libscipy_openblas-*.so
 
 $(OUT)/ref_numbering: oracle/ref_numbering.cpp $(OUT)/libparmoon_ref.a
 	$(CXX) $(FLAGS) $(INC) oracle/ref_numbering.cpp $(OUT)/libparmoon_ref.a -o $@ \
+	  -L$(SCIPY_LIBS) -l:$(OPENBLAS) -Wl,-rpath,$(SCIPY_LIBS)
+
+$(OUT)/ref_assemble: oracle/ref_assemble.cpp include/pmx_prm.h $(OUT)/libparmoon_ref.a
+	$(CXX) $(FLAGS) $(INC) oracle/ref_assemble.cpp $(OUT)/libparmoon_ref.a -o $@ \
 	  -L$(SCIPY_LIBS) -l:$(OPENBLAS) -Wl,-rpath,$(SCIPY_LIBS)

@@ -51,3 +51,54 @@ def test_numbering_equals_reference_live(nref, order):
     if (nref, order) == (4, 1):
         assert (N, len(kc)) == (4913, 92663)  # the golden log's sizes (ReadIn/tcd3d_GMG.out:386,392-394)
     check_equal(nref, order, N, act, bi, gn, rp, kc)
+
+
+# ------------------------------------------------- assembly against ParMooN's own
+import make_assembly_fixtures as AF  # noqa: E402
+
+AFIX = np.load(os.path.join(HERE, "golden", "reference_assembly.npz"))
+ACASES = [(name, nref, order) for name, nref, order, _ in AF.cases()]
+RHS_ONLY = (8, 7)  # RhsAssemble, RhsAssemble_SUPG: no matrix
+
+
+def row_rel_err(E_ref, E, rowptr):
+    rows = np.repeat(np.arange(len(rowptr) - 1), np.diff(rowptr))
+    scale = np.maximum.reduceat(np.abs(E_ref), rowptr[:-1])
+    return float(np.max(np.abs(E - E_ref) / np.maximum(scale[rows], 1e-300)))
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name,nref,order", ACASES, ids=[c[0] for c in ACASES])
+def test_oracle_assembly_equals_reference_fixture(name, nref, order):
+    """The oracle's assembly (every form, both rules, SDFEM types 0 and 2,
+    variable convection) against ParMooN's Assemble3D: 1e-13 of the row's
+    largest entry, rhs 1e-13 of its largest entry."""
+    prm = AFIX[name + "_prm"]
+    sp = orc.Space(nref, order)
+    _, _, rp, _ = sp.arrays()
+    two = name + "_entries2" in AFIX
+    out = sp.assemble(prm, two=two)
+    if int(prm[7]) not in RHS_ONLY:
+        assert row_rel_err(AFIX[name + "_entries"], out[0], rp) < 1e-13
+    assert rel(out[1], AFIX[name + "_rhs"]) < 1e-13
+    if two:
+        assert row_rel_err(AFIX[name + "_entries2"], out[2], rp) < 1e-13
+
+
+@pytest.mark.parametrize("nref,order,form", [(3, 1, 2), (2, 2, 2), (2, 2, 5), (3, 2, 4)])
+def test_oracle_assembly_equals_reference_live(nref, order, form):
+    if not (os.path.exists(AF.EXE) and os.path.isdir(os.path.join(REF, "data"))):
+        pytest.skip("the reference build (oracle/_ref) or the reference tree is not here")
+    from parmoon_amd import prm as P
+    prm = P.config2(nref=nref, form=form, linear=False)
+    prm[P.TAU], prm[P.THETA1] = 0.02, 0.5
+    E, rhs, E2 = AF.run_reference(nref, order, prm, REF)
+    sp = orc.Space(nref, order)
+    _, _, rp, _ = sp.arrays()
+    out = sp.assemble(prm, two=E2 is not None)
+    assert row_rel_err(E, out[0], rp) < 1e-13 and rel(out[1], rhs) < 1e-13
+    if E2 is not None:
+        assert row_rel_err(E2, out[2], rp) < 1e-13
