@@ -47,6 +47,17 @@ const char *pmx_last_error(void);
  * for the unit cube refined nref times, computed on the host only. */
 int pmx_host_space_dims(int nref, int order, long long *dims);
 int pmx_host_space_arrays(int nref, int order, int *begin_index, int *global_numbers, int *rowptr, int *kcol);
+/* Legacy-VTK output of nvar scalar functions of that space (values[k]: N
+ * doubles in the reference numbering, names[k]: the SCALARS name, no blanks),
+ * replacing TOutput3D::WriteVtk (src/FE/Output3D.C:1773-2305) for scalar
+ * Q1/Q2 functions: POINTS are the mesh vertices, CELLS the hexahedra
+ * (type 12) in the reference's cell order with each cell's vertices in its
+ * local order, POINT_DATA the function values at the vertices, CELL_DATA the
+ * region id (0), numbers as %.6e.  The reference lists the points in the
+ * order of their addresses in memory; here they are in lattice order (x
+ * fastest), so the file equals the reference's up to that renumbering. */
+int pmx_host_write_vtk(int nref, int order, int nvar, const char *const *names, const double *const *values,
+                       const char *path);
 
 /* ---- multilevel system --------------------------------------------------
  * Replaces the per-level set-up of 3DPrograms/CD3D_ParMooN.C:264-351
@@ -285,13 +296,14 @@ int pmx_learned_gradient_level(pmx_system *sys, int level, const float *in, floa
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient */
 int pmx_learned_adam(pmx_system *sys, double lr);
 /* train on pairs from plain cycles (opt: the cycle options of pmx_system_solve).
- * topt (8 doubles, unused entries 0): [0] samples per level, [1] Adam steps per level (levels interleaved),
+ * topt (16 doubles, unused entries 0): [0] samples per level, [1] Adam steps per level (levels interleaved),
  * [2] learning rate, [3] seed, [4] finest levels that use the network (0: all
  * but the coarsest), [5] loss: 0 squared error against the cycle's error,
  * 1 the scaled defect left after the correction (as pmx_learned_gradient_level),
  * [6] samples: 0 a random start of the homogeneous problem, 1 a zero start
  * with a random right-hand side, 2 half each, [7] the finest levels among [4]
- * left untrained (for solves with opt[22] of that value).
+ * left untrained (for solves with opt[22] of that value), [8] samples per Adam
+ * step (batch, default 1; the step uses the mean of their gradients).
  * tinfo (16 doubles): [0] first / [1] last mean loss of the finest trained network,
  * [2] s, [3] total steps, [4 + 2j] / [5 + 2j] first / last mean loss of the
  * network of level L - j. */

@@ -11,6 +11,8 @@
 //   TSystemTCD3D       AssembleMRhs, AssembleARhs, AssembleSystMat, Solve,
 //                      RestoreMassMat                     (include/System/SystemTCD3D.h)
 //   TAssembleMat3D     Init, Reset, Assemble3D, DeAllocate (include/System/AssembleMat3D.h)
+//   TFEFunction3D      GetName / GetValues / GetLength    (include/FE/FEFunction3D.h)
+//   TOutput3D          AddFEFunction, WriteVtk            (include/FE/Output3D.h)
 //
 // Differences: parameters come from an explicit block (pmx_prm.h) and an
 // option vector (pmx.h) instead of TDatabase; errors throw pmx::Error
@@ -98,6 +100,66 @@ class TSquareStructure3D {
 
  private:
   TFESpace3D &space_;
+};
+
+// A scalar finite element function: the space and the caller's coefficient
+// array (reference numbering), as TFEFunction3D(fespace, name, description,
+// values, length) (include/FE/FEFunction3D.h:71).
+class TFEFunction3D {
+ public:
+  TFEFunction3D(TFESpace3D *space, const char *name, const char *description, double *values, int length)
+      : space_(space), name_(name ? name : ""), description_(description ? description : ""), values_(values),
+        length_(length) {}
+  TFESpace3D *GetFESpace3D() const { return space_; }
+  const char *GetName() const { return name_.c_str(); }
+  const char *GetDescription() const { return description_.c_str(); }
+  double *GetValues() const { return values_; }
+  int GetLength() const { return length_; }
+
+ private:
+  TFESpace3D *space_;
+  std::string name_, description_;
+  double *values_;
+  int length_;
+};
+
+// Output of scalar functions of one space: AddFEFunction, WriteVtk
+// (include/FE/Output3D.h:128-136, src/FE/Output3D.C:141-162,1773-2305).  The
+// capacities of the constructor grow as needed, as in the reference; the
+// domain argument is not used (the space carries the mesh).
+class TOutput3D {
+ public:
+  TOutput3D(int maxn_fespaces, int maxn_scalar, int maxn_vect, int maxn_parameters, const void *domain = nullptr) {
+    (void)maxn_fespaces; (void)maxn_vect; (void)maxn_parameters; (void)domain;
+    funcs_.reserve(maxn_scalar > 0 ? maxn_scalar : 1);
+  }
+  // returns the number of scalar functions stored (as the reference)
+  int AddFEFunction(TFEFunction3D *f) {
+    if (!funcs_.empty() && (f->GetFESpace3D()->GetRefinements() != funcs_[0]->GetFESpace3D()->GetRefinements() ||
+                            f->GetFESpace3D()->GetOrder() != funcs_[0]->GetFESpace3D()->GetOrder()))
+      throw Error(PMX_ERR_ARG, "TOutput3D: functions of different spaces");
+    if (f->GetLength() != f->GetFESpace3D()->GetN_DegreesOfFreedom())
+      throw Error(PMX_ERR_ARG, "TOutput3D: function length differs from the space's DOFs");
+    funcs_.push_back(f);
+    return (int)funcs_.size();
+  }
+  int WriteVtk(const char *name) {
+    if (funcs_.empty()) throw Error(PMX_ERR_ARG, "TOutput3D: no function added");
+    std::vector<const char *> names;
+    std::vector<const double *> vals;
+    for (auto *f : funcs_) {
+      names.push_back(f->GetName());
+      vals.push_back(f->GetValues());
+    }
+    const TFESpace3D *s = funcs_[0]->GetFESpace3D();
+    check(pmx_host_write_vtk(s->GetRefinements(), s->GetOrder(), (int)funcs_.size(), names.data(), vals.data(),
+                             name),
+          "pmx_host_write_vtk");
+    return 0;
+  }
+
+ private:
+  std::vector<TFEFunction3D *> funcs_;
 };
 
 // TSystemCD3D: levels 0 .. N_levels-1 (coarsest first), solution and rhs

@@ -47,6 +47,7 @@ def lib():
         L.pmx_ctx_sync.argtypes = [vp]
         L.pmx_host_space_dims.argtypes = [ip, ip, vp]
         L.pmx_host_space_arrays.argtypes = [ip, ip, vp, vp, vp, vp]
+        L.pmx_host_write_vtk.argtypes = [ip, ip, ip, vp, vp, ctypes.c_char_p]
         L.pmx_system_create.argtypes = [vp, ip, ip, ip, ctypes.POINTER(vp)]
         L.pmx_system_destroy.argtypes = [vp]
         L.pmx_system_dims.argtypes = [vp, ip, vp]
@@ -112,6 +113,27 @@ def host_space(nref, order):
     kc = np.zeros(nnz, np.int32)
     _ok(L.pmx_host_space_arrays(nref, order, _p(bi), _p(gn), _p(rp), _p(kc)), "pmx_host_space_arrays")
     return d, bi, gn, rp, kc
+
+
+def write_vtk(nref, order, functions, path):
+    """Legacy-VTK file of scalar functions of the host space (nref, order):
+    `functions` maps SCALARS names to N values in the reference numbering
+    (pmx_host_write_vtk)."""
+    names = list(functions)
+    vals = [np.ascontiguousarray(functions[k], np.float64) for k in names]
+    N = int(host_space_dims(nref, order)[0])
+    for k, v in zip(names, vals):
+        if v.shape != (N,):
+            raise ValueError(f"{k}: {v.shape} values for {N} DOFs")
+    cn = (ctypes.c_char_p * len(names))(*[k.encode() for k in names])
+    cv = (ctypes.c_void_p * len(vals))(*[v.ctypes.data for v in vals])
+    _ok(lib().pmx_host_write_vtk(nref, order, len(names), cn, cv, str(path).encode()), "pmx_host_write_vtk")
+
+
+def host_space_dims(nref, order):
+    d = np.zeros(8, np.int64)
+    _ok(lib().pmx_host_space_dims(nref, order, _p(d)), "pmx_host_space_dims")
+    return d
 
 
 class Context:
@@ -418,10 +440,12 @@ class System:
         _ok(lib().pmx_learned_gradient_level(self.h, level, _p(u), _p(g), _p(loss)), "pmx_learned_gradient_level")
         return g, float(loss[0])
 
-    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=0, loss=1, data=0, skip=0):
+    def learned_train(self, opt, samples=16, steps=400, lr=2e-3, seed=0, levels=0, loss=1, data=0, skip=0,
+                      batch=1):
         o = np.zeros(24)
         o[:len(opt)] = opt
-        t = np.array([samples, steps, lr, seed, levels, loss, data, skip], float)
+        t = np.zeros(16)
+        t[:9] = [samples, steps, lr, seed, levels, loss, data, skip, batch]
         info = np.zeros(16)
         _ok(lib().pmx_learned_train(self.h, _p(o), _p(t), _p(info)), "pmx_learned_train")
         return info

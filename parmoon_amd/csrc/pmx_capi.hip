@@ -892,7 +892,10 @@ static void learned_res_step(System &Y, Level &lv, LearnedNet &net, const float 
 // all but the coarsest), [5] loss: 0 squared error against -x, 1 the scaled
 // defect left after the correction (learned_res_step), [6] samples: 0 random
 // start and f = 0, 1 zero start and a random f, 2 half each, [7] the finest
-// levels among [4] that are not trained (they keep the plain smoother).
+// levels among [4] that are not trained (they keep the plain smoother), [8]
+// samples per Adam step (the batch, default 1: the step takes the mean of
+// their gradients; sample k of level j's list of [0] is used at step k / B,
+// cyclically).
 // tinfo: [0] mean loss of the first 10 steps of the finest trained level, [1]
 // of its last ones, [2] seconds, [3] steps, [4 + 2j] / [5 + 2j] the same for
 // level L - j.
@@ -908,6 +911,8 @@ static void learned_train(System &Y, const double *to, double *ti) {
   const int data_mode = (int)to[6];  // 0 random error, f = 0; 1 x = 0, random f; 2 both
   const int nl = Y.learned_levels;
   const int j0 = std::max(0, std::min((int)to[7], nl - 1));  // finest levels left untrained
+  const int batch = to[8] > 0 ? (int)to[8] : 1;
+  if (batch > 4096) throw std::invalid_argument("learned_train: batch above 4096");
   Y.trained_levels = nl;
   Y.learn_active = false;
   ensure_learned(Y);
@@ -925,7 +930,7 @@ static void learned_train(System &Y, const double *to, double *ti) {
   const int nt = nl - j0;  // trained levels
   const int total = steps * nt;
   DevBuf<float> losses;
-  losses.alloc(total);
+  losses.alloc((size_t)total * batch);
   hipEvent_t e0, e1;
   check(hipEventCreate(&e0), "event");
   check(hipEventCreate(&e1), "event");
@@ -961,20 +966,25 @@ static void learned_train(System &Y, const double *to, double *ti) {
   if (residual_loss)
     for (int j = j0; j < nl; j++) learned_transpose(Y, *Y.lev[L - j]);
   for (int k = 0; k < total; k++) {
-    const int j = j0 + k % nt, kk = k / nt, smp = kk % ns;
+    const int j = j0 + k % nt, kk = k / nt;
     Level &lv = *Y.lev[L - j];
     LearnedNet &net = Y.nets[L - j];
-    const size_t o = off[j] + (size_t)smp * lv.L.N;
     const double lrk = lr * (0.1 + 0.45 * (1.0 + std::cos(M_PI * kk / std::max(1, steps - 1))));
     net_zero_grads(net, st);
-    if (residual_loss) {
-      learned_res_step(Y, lv, net, du.p + o, losses.p + k);
-    } else {
-      learned_build_input(net, du.p + o, nullptr, 1.0, lv.feat.p, lv.L.N, st);
-      net_forward_act(net, lv.L.M, true, st);
-      net_backward(net, lv.L.M, dt.p + o, losses.p + k, st);
+    for (int b = 0; b < batch; b++) {
+      // the gradients of the batch accumulate in net.grads
+      const int smp = (int)(((int64_t)kk * batch + b) % ns);
+      const size_t o = off[j] + (size_t)smp * lv.L.N;
+      float *lk = losses.p + (size_t)k * batch + b;
+      if (residual_loss) {
+        learned_res_step(Y, lv, net, du.p + o, lk);
+      } else {
+        learned_build_input(net, du.p + o, nullptr, 1.0, lv.feat.p, lv.L.N, st);
+        net_forward_act(net, lv.L.M, true, st);
+        net_backward(net, lv.L.M, dt.p + o, lk, st);
+      }
     }
-    net_adam(net, (float)lrk, st);
+    net_adam(net, (float)lrk, st, 1.0f / (float)batch);
   }
   check(hipEventRecord(e1, st), "event");
   check(hipEventSynchronize(e1), "event");
@@ -982,8 +992,13 @@ static void learned_train(System &Y, const double *to, double *ti) {
   check(hipEventElapsedTime(&ms, e0, e1), "event");
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  std::vector<float> h(total);
-  check(hipMemcpy(h.data(), losses.p, sizeof(float) * total, hipMemcpyDeviceToHost), "losses");
+  std::vector<float> hb((size_t)total * batch), h(total, 0.0f);
+  check(hipMemcpy(hb.data(), losses.p, sizeof(float) * hb.size(), hipMemcpyDeviceToHost), "losses");
+  for (int k = 0; k < total; k++) {
+    double a = 0;
+    for (int b = 0; b < batch; b++) a += hb[(size_t)k * batch + b];
+    h[k] = (float)(a / batch);
+  }
   // mean loss of the first / last 10 steps of each level's network
   if (ti) {
     const int w = std::min(10, steps);
@@ -1286,6 +1301,25 @@ int pmx_host_space_arrays(int nref, int order, int *bi, int *gn, int *rp, int *k
       if (rp) std::memcpy(rp, Nb.rowptr.data(), sizeof(int) * Nb.rowptr.size());
       if (kc) std::memcpy(kc, Nb.kcol.data(), sizeof(int) * Nb.kcol.size());
     }
+  })
+}
+
+int pmx_host_write_vtk(int nref, int order, int nvar, const char *const *names, const double *const *values,
+                       const char *path) {
+  PMX_TRY({
+    if (!path || nvar < 0 || (nvar > 0 && (!names || !values)))
+      throw std::invalid_argument("NULL argument or negative nvar");
+    for (int k = 0; k < nvar; k++) {
+      if (!names[k] || !values[k]) throw std::invalid_argument("NULL name or values");
+      for (const char *c = names[k]; *c; c++)
+        if (*c == ' ' || *c == '\n' || *c == '\t') throw std::invalid_argument("VTK names carry no blanks");
+      if (!*names[k]) throw std::invalid_argument("empty name");
+    }
+    Layout L;
+    Numbering Nb;
+    build_layout(L, nref, order, kBlock);
+    build_numbering(Nb, L);
+    write_vtk(L, Nb, nvar, names, values, path);
   })
 }
 

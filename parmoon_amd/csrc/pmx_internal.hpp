@@ -94,6 +94,11 @@ void build_layout(Layout &L, int nref, int order, int block);
 void build_numbering(Numbering &Nb, const Layout &L);
 void build_csr(Numbering &Nb, const Layout &L);
 
+// legacy-VTK file of nvar scalar functions (reference numbering) on the
+// level's hexahedra, in the layout of TOutput3D::WriteVtk
+void write_vtk(const Layout &L, const Numbering &Nb, int nvar, const char *const *names,
+               const double *const *values, const char *path);
+
 // reference-numbering colour of a DOF (the multicolour smoother class)
 int node_colour(const Layout &L, int x, int y, int z);
 

@@ -426,10 +426,11 @@ __global__ void k_reduce_chunks(const float *__restrict__ tmp, int nc, int ne, f
   grads[e] += s;
 }
 __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       float gscale) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
+  const float gi = g[i] * gscale;
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
@@ -643,13 +644,13 @@ static void backward_layers(LearnedNet &net, int M, hipStream_t st) {
   }
 }
 
-void net_adam(LearnedNet &net, float lr, hipStream_t st) {
+void net_adam(LearnedNet &net, float lr, hipStream_t st, float gscale) {
   net.adam_t++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, (float)net.adam_t), c2 = 1.f - std::pow(b2, (float)net.adam_t);
   const int P = net_total_params();
   hipLaunchKernelGGL(k_adam, dim3((P + 255) / 256), dim3(256), 0, st, net.params, net.grads, net.adam_m, net.adam_v, P,
-                     lr, b1, b2, 1e-8f, c1, c2);
+                     lr, b1, b2, 1e-8f, c1, c2, gscale);
   net_sync_bf16(net, st);
 }
 

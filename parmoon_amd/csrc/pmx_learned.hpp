@@ -78,7 +78,9 @@ void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipSt
 // backward from a given FP32 gradient of the loss w.r.t. net.out (accumulates into net.grads)
 void net_backward_from(LearnedNet &net, int M, const float *gy, hipStream_t st);
 void net_zero_grads(LearnedNet &net, hipStream_t st);
-void net_adam(LearnedNet &net, float lr, hipStream_t st);
+// one Adam step with the accumulated gradient times gscale (1/B for the
+// mean over a batch of B samples)
+void net_adam(LearnedNet &net, float lr, hipStream_t st, float gscale = 1.0f);
 
 struct DevLayout;
 // dinv[p] = 1 / a_pp on interior nodes, 0 on Dirichlet nodes (FP32, lattice order)

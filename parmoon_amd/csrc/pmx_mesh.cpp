@@ -15,8 +15,10 @@
 //    last (FESpace3D.C:1739-1830) -- i.e. first-encounter order per class.
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 #include "pmx_internal.hpp"
 
@@ -100,48 +102,21 @@ void build_layout(Layout &L, int nref, int order, int block) {
           L.boundary.push_back(x + M * (y + M * z));
 }
 
-void build_numbering(Numbering &Nb, const Layout &L) {
-  const int p = L.order, M = L.M, k = p + 1;
-  Nb.nb = k * k * k;
-  Nb.ncells = (int64_t)L.n * L.n * L.n;
-  Nb.N = L.N;
-  Nb.active = L.nactive;
-  Nb.begin_index.resize(Nb.ncells + 1);
-  for (int64_t i = 0; i <= Nb.ncells; i++) Nb.begin_index[i] = (int)(i * Nb.nb);
-  Nb.global_numbers.assign(Nb.ncells * Nb.nb, -1);
-  Nb.lat2dof.assign(L.N, -1);
-  int64_t inner = 0, dir = 0;
-  std::vector<uint8_t> isdir(L.N, 0);
-  // depth-first traversal of the refinement tree (children 0..7)
+// Calls leaf(frame) for the cells of the unit cube refined until the cells
+// have size 1, in the reference's cell order: depth first through the
+// refinement tree, children 0..7 (It_Finest over the regular refinement).
+template <class F>
+static void walk_leaves(int n, F &&leaf) {
   std::vector<Frame> stack;
   Frame root{};
   root.e[0][0] = root.e[1][1] = root.e[2][2] = 1;
-  root.size = L.n;
+  root.size = n;
   stack.push_back(root);
-  int64_t cell = 0;
-  int loc[27];
   while (!stack.empty()) {
     Frame f = stack.back();
     stack.pop_back();
     if (f.size == 1) {
-      // leaf: local DOF (a,b,c), x fastest, sits at p*origin + a e1 + b e2 + c e3
-      int j = 0;
-      for (int c = 0; c < k; c++)
-        for (int b = 0; b < k; b++)
-          for (int a = 0; a < k; a++, j++) {
-            int X[3];
-            for (int d = 0; d < 3; d++) X[d] = p * f.o[d] + a * f.e[0][d] + b * f.e[1][d] + c * f.e[2][d];
-            loc[j] = X[0] + M * (X[1] + M * X[2]);
-            const int64_t l = loc[j];
-            if (Nb.lat2dof[l] < 0) {
-              const bool bd = X[0] == 0 || X[1] == 0 || X[2] == 0 || X[0] == M - 1 || X[1] == M - 1 ||
-                              X[2] == M - 1;
-              if (bd) { Nb.lat2dof[l] = (int)dir++; isdir[l] = 1; }
-              else Nb.lat2dof[l] = (int)inner++;
-            }
-          }
-      for (int q = 0; q < Nb.nb; q++) Nb.global_numbers[cell * Nb.nb + q] = loc[q];  // lattice for now
-      cell++;
+      leaf(f);
       continue;
     }
     // push children in reverse so that child 0 is visited first
@@ -160,6 +135,42 @@ void build_numbering(Numbering &Nb, const Layout &L) {
       stack.push_back(c);
     }
   }
+}
+
+void build_numbering(Numbering &Nb, const Layout &L) {
+  const int p = L.order, M = L.M, k = p + 1;
+  Nb.nb = k * k * k;
+  Nb.ncells = (int64_t)L.n * L.n * L.n;
+  Nb.N = L.N;
+  Nb.active = L.nactive;
+  Nb.begin_index.resize(Nb.ncells + 1);
+  for (int64_t i = 0; i <= Nb.ncells; i++) Nb.begin_index[i] = (int)(i * Nb.nb);
+  Nb.global_numbers.assign(Nb.ncells * Nb.nb, -1);
+  Nb.lat2dof.assign(L.N, -1);
+  int64_t inner = 0, dir = 0;
+  std::vector<uint8_t> isdir(L.N, 0);
+  int64_t cell = 0;
+  int loc[27];
+  walk_leaves(L.n, [&](const Frame &f) {
+    // local DOF (a,b,c), x fastest, sits at p*origin + a e1 + b e2 + c e3
+    int j = 0;
+    for (int c = 0; c < k; c++)
+      for (int b = 0; b < k; b++)
+        for (int a = 0; a < k; a++, j++) {
+          int X[3];
+          for (int d = 0; d < 3; d++) X[d] = p * f.o[d] + a * f.e[0][d] + b * f.e[1][d] + c * f.e[2][d];
+          loc[j] = X[0] + M * (X[1] + M * X[2]);
+          const int64_t l = loc[j];
+          if (Nb.lat2dof[l] < 0) {
+            const bool bd = X[0] == 0 || X[1] == 0 || X[2] == 0 || X[0] == M - 1 || X[1] == M - 1 ||
+                            X[2] == M - 1;
+            if (bd) { Nb.lat2dof[l] = (int)dir++; isdir[l] = 1; }
+            else Nb.lat2dof[l] = (int)inner++;
+          }
+        }
+    for (int q = 0; q < Nb.nb; q++) Nb.global_numbers[cell * Nb.nb + q] = loc[q];  // lattice for now
+    cell++;
+  });
   if (inner != L.nactive || inner + dir != L.N) throw std::runtime_error("numbering count mismatch");
   for (int64_t l = 0; l < L.N; l++)
     if (isdir[l]) Nb.lat2dof[l] += (int)inner;
@@ -201,6 +212,48 @@ void build_csr(Numbering &Nb, const Layout &L) {
     std::sort(c, c + n);
   }
   Nb.have_csr = true;
+}
+
+void write_vtk(const Layout &L, const Numbering &Nb, int nvar, const char *const *names,
+               const double *const *values, const char *path) {
+  const int n = L.n, V = n + 1, p = L.order, M = L.M;
+  const int64_t nv = (int64_t)V * V * V, nc = (int64_t)n * n * n;
+  FILE *fp = std::fopen(path, "w");
+  if (!fp) throw std::runtime_error(std::string("cannot open ") + path + " for writing");
+  std::fprintf(fp, "# vtk DataFile Version 4.2\nfile created by ParMooN\nASCII\nDATASET UNSTRUCTURED_GRID\n\n");
+  std::fprintf(fp, "POINTS %lld double\n", (long long)nv);
+  const double h = 1.0 / n;
+  for (int z = 0; z < V; z++)
+    for (int y = 0; y < V; y++)
+      for (int x = 0; x < V; x++) std::fprintf(fp, "%.6e %.6e %.6e\n", x * h, y * h, z * h);
+  std::fprintf(fp, "\nCELLS %lld %lld\n", (long long)nc, (long long)(nc * 9));
+  walk_leaves(n, [&](const Frame &f) {
+    // vertex j of the cell at the reference position kVertexPos[j] of its frame
+    std::fprintf(fp, "8 ");
+    for (int j = 0; j < 8; j++) {
+      int X[3];
+      for (int d = 0; d < 3; d++)
+        X[d] = f.o[d] + kVertexPos[j][0] * f.e[0][d] + kVertexPos[j][1] * f.e[1][d] + kVertexPos[j][2] * f.e[2][d];
+      std::fprintf(fp, "%d ", X[0] + V * (X[1] + V * X[2]));
+    }
+    std::fprintf(fp, "\n");
+  });
+  std::fprintf(fp, "\nCELL_TYPES %lld\n", (long long)nc);
+  for (int64_t c = 0; c < nc; c++) std::fputs("12 ", fp);
+  std::fprintf(fp, "\n\nPOINT_DATA %lld\n", (long long)nv);
+  for (int k = 0; k < nvar; k++) {
+    std::fprintf(fp, "SCALARS %s double\nLOOKUP_TABLE default\n", names[k]);
+    const double *u = values[k];
+    // a nodal Q1/Q2 function's value at a vertex is its coefficient there
+    for (int z = 0; z < V; z++)
+      for (int y = 0; y < V; y++)
+        for (int x = 0; x < V; x++) std::fprintf(fp, "%.6e\n", u[Nb.lat2dof[p * x + M * (p * y + M * p * z)]]);
+    std::fprintf(fp, "\n\n");
+  }
+  std::fprintf(fp, "\nCELL_DATA %lld\nSCALARS RegionID int  1\nLOOKUP_TABLE default  \n", (long long)nc);
+  for (int64_t c = 0; c < nc; c++) std::fputs("0\n", fp);
+  const bool bad = std::ferror(fp) != 0;
+  if (std::fclose(fp) != 0 || bad) throw std::runtime_error(std::string("error writing ") + path);
 }
 
 }  // namespace pmx
