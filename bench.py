@@ -88,6 +88,30 @@ def cpu_baseline(nref_cpu=5, seed=2024):
             "iters": int(info[0]), "final_rel_res": float(info[1] / r0)}
 
 
+REF_SOLVE = os.path.join(ROOT, "oracle", "_ref", "ref_solve")
+
+
+def cpu_baseline_reference(nref_cpu=5, seed=2024):
+    """ParMooN's own CPU path (oracle/_ref/ref_solve: the reference's
+    Assemble3D, TMultiGrid3D / TMGLevel3D with lexicographic SSOR,
+    TMultiGridScaIte and TFgmresIte, compiled from its sources with g++ -O2)
+    on the same workload at 32^3 Q2 (274,625 DOFs, 5 levels 2^3..32^3):
+    set-up, assembly of every level and the FGMRES(30) V(2,2) SSOR solve to
+    FP64 relative residual 1e-10, on one core of this host (the reference's
+    sequential build).  The mesh is the reference's unit cube
+    (tests/golden/Wuerfel.PRM / .GEO)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_solve_fixtures as SF
+    from parmoon_amd import prm as P
+    prm = P.config2(nref=nref_cpu, seed=seed)
+    opt = P.solver_opt(solver=16, restart=30, maxit=400, res_norm_min=0.0, red_factor=0.0, smoother=3, omega=1.0,
+                       pre=2, post=2, rel_target=1e-10)
+    mesh = (os.path.join(ROOT, "tests", "golden", "Wuerfel.PRM"), os.path.join(ROOT, "tests", "golden", "Wuerfel.GEO"))
+    x, _, st = SF.run_reference(nref_cpu - 4, 5, 2, prm, opt, stats=True, exe=REF_SOLVE, mesh=mesh)
+    return {"N": len(x), "t_setup": st["setup_s"], "t_assembly_all": st["assembly_s"], "t_solve": st["solve_s"],
+            "iters": st["iterations"]}
+
+
 def transient(args):
     """Transient workload: TCD3D on 128^3 Q1 (2,146,689 DOFs), eps = 1e-4,
     b = (-(y - 1/2), x - 1/2, 0.2), a seeded Gaussian blob as initial value,
@@ -175,10 +199,12 @@ def train_set(args):
     """Training-set workload: learned-smoother networks trained across seeded
     coefficient fields on 64^3 Q1 (274,625 DOFs): eps = exp(G) with G a
     Gaussian random field (sigma 1, correlation length 0.1), b = (1,2,3)/sqrt(14)
-    scaled to a cell Peclet number log-uniform in [10, 1000], SUPG; pairs from
-    the four finest levels of the plain cycle; Adam.  Reports training
-    samples/s and convolution GFLOP/s, then the FGMRES solve of held-out
-    fields with and without the networks.  Prints one JSON line."""
+    scaled to a cell Peclet number log-uniform in [10, 1000], SUPG;
+    (residual, exact error) pairs from the plain cycle of the homogeneous
+    problem on the three finest levels; Adam, learning rate 1e-3, batch 8.
+    Reports training samples/s and convolution GFLOP/s, then the FGMRES
+    solve of held-out fields with and without the networks.  Prints one JSON
+    line."""
     import parmoon_amd as pa
     from parmoon_amd import prm as P
     ctx = pa.Context(0)
@@ -196,10 +222,11 @@ def train_set(args):
     for k in range(nseeds):
         S.assemble(P.lognormal(seed=k, nref=6, peclet=pe[k]))
         info = S.learned_train(P.solver_opt(**dict(base, precision=1)), samples=8, steps=per_seed, lr=1e-3,
-                               seed=k, levels=4, data=1)
+                               seed=k, levels=3, loss=0, data=0, batch=8)
         t_train += float(info[2])
-        steps += int(info[3])
-        flops += per_seed * 3.0 * (fwd[4] + fwd[3] + fwd[2] + fwd[1])  # forward + backward (about twice the forward)
+        steps += int(info[3]) * 8  # samples: Adam steps x batch
+        # forward + backward (about twice the forward) per sample
+        flops += per_seed * 8 * 3.0 * (fwd[4] + fwd[3] + fwd[2])
         if (k + 1) % 16 == 0:
             print(f"[bench] train-set: {k + 1}/{nseeds} fields, {t_train:.1f}s", file=sys.stderr, flush=True)
     evals = []
@@ -209,7 +236,7 @@ def train_set(args):
         for name, kw in (("fp64", dict(precision=0)), ("fp32", dict(precision=1)),
                          ("learned1", dict(precision=2, learned_levels=1)),
                          ("learned3", dict(precision=2, learned_levels=3)),
-                         ("learned_coarse", dict(precision=2, learned_levels=4, learned_skip=1, learned_steps=2,
+                         ("learned_coarse", dict(precision=2, learned_levels=3, learned_skip=1, learned_steps=2,
                                                  learned_pre=1))):
             info, rc = S.solve_resident(P.solver_opt(**dict(base, **kw)), check=False)
             row[name] = {"rc": int(rc), "iters": int(info[0]), "s": round(float(info[4]), 4)}
@@ -221,7 +248,8 @@ def train_set(args):
            "ms_per_step": round(t_train / steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic (seeded Gaussian random fields)",
            "config": {"workload": f"{nseeds} lognormal fields 64^3 Q1 SUPG, Pe log-uniform 10..1000, "
-                                  f"{per_seed} Adam steps per field and level, levels 64^3/32^3/16^3/8^3, batch 1",
+                                  f"{per_seed} Adam steps of batch 8 per field and level, levels 64^3/32^3/16^3, "
+                                  "(residual, exact error) pairs, lr 1e-3",
                       "parallelism": "1 GPU"},
            "conv_GFLOPs": round(flops / t_train / 1e9, 1),
            "train_seconds": round(t_train, 3),
@@ -476,7 +504,10 @@ def main():
     }
     if not args.no_cpu:
         try:
-            c = cpu_baseline()
+            kind = "reference" if os.path.exists(REF_SOLVE) else "port"
+            c = cpu_baseline_reference() if kind == "reference" else cpu_baseline()
+            who = ("ParMooN's own code (oracle/_ref/ref_solve, compiled from the reference sources)"
+                   if kind == "reference" else "oracle (C++ restatement of the reference algorithms)")
             model, ncpu = cpu_info()
             its64 = res.get("fp64", {}).get("iters")
             # time per iteration and DOF of the sample, times the DOFs and the
@@ -485,8 +516,8 @@ def main():
             scale_n = N / c["N"]
             scale_it = (its64 / c["iters"]) if its64 else 1.0
             out["cpu_baseline"] = {
-                "value": round(c["t_solve"] * scale_n * scale_it, 2), "unit": "s", "cores": 1, "kind": "port",
-                "sample": f"oracle (one core) on the same workload at 32^3 Q2 ({c['N']} DOFs): solve to 1e-10 in "
+                "value": round(c["t_solve"] * scale_n * scale_it, 2), "unit": "s", "cores": 1, "kind": kind,
+                "sample": f"{who}, one core, on the same workload at 32^3 Q2 ({c['N']} DOFs): solve to 1e-10 in "
                           f"{c['iters']} FGMRES iterations, {c['t_solve']:.2f} s; value = that time x DOF ratio "
                           f"{scale_n:.2f} x iteration ratio {scale_it:.3f} (GPU FP64 iterations at {2 ** nref}^3)",
                 "sample_solve_s": round(c["t_solve"], 3), "sample_iterations": c["iters"],

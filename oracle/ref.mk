@@ -23,7 +23,7 @@ FLAGS := -O2 -w -fpermissive -D__3D__ -D_SEQ $(LAPACK_NAMES)
 SRCS := $(shell python3 oracle/ref_sources.py $(REF))
 OBJS := $(patsubst $(REF)/src/%.C,$(OUT)/obj/%.o,$(SRCS)) $(OUT)/obj/tetgen.o
 
-all: $(OUT)/ref_numbering $(OUT)/ref_assemble
+all: $(OUT)/ref_numbering $(OUT)/ref_assemble $(OUT)/ref_solve
 
 $(OUT)/obj/%.o: $(REF)/src/%.C
 	@mkdir -p $(dir $@)
@@ -44,6 +44,10 @@ $(OUT)/ref_numbering: oracle/ref_numbering.cpp $(OUT)/libparmoon_ref.a
 	$(CXX) $(FLAGS) $(INC) oracle/ref_numbering.cpp $(OUT)/libparmoon_ref.a -o $@ \
 	  -L$(SCIPY_LIBS) -l:$(OPENBLAS) -Wl,-rpath,$(SCIPY_LIBS)
 
-$(OUT)/ref_assemble: oracle/ref_assemble.cpp include/pmx_prm.h $(OUT)/libparmoon_ref.a
+$(OUT)/ref_assemble: oracle/ref_assemble.cpp oracle/ref_examples.hpp include/pmx_prm.h $(OUT)/libparmoon_ref.a
 	$(CXX) $(FLAGS) $(INC) oracle/ref_assemble.cpp $(OUT)/libparmoon_ref.a -o $@ \
+	  -L$(SCIPY_LIBS) -l:$(OPENBLAS) -Wl,-rpath,$(SCIPY_LIBS)
+
+$(OUT)/ref_solve: oracle/ref_solve.cpp oracle/ref_examples.hpp include/pmx_prm.h $(OUT)/libparmoon_ref.a
+	$(CXX) $(FLAGS) $(INC) oracle/ref_solve.cpp $(OUT)/libparmoon_ref.a -o $@ \
 	  -L$(SCIPY_LIBS) -l:$(OPENBLAS) -Wl,-rpath,$(SCIPY_LIBS)

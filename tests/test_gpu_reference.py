@@ -1,9 +1,12 @@
-"""The device assembly against ParMooN's own Assemble3D (no oracle between):
+"""The device against ParMooN's own code (no oracle between).
+
 tests/golden/reference_assembly.npz holds the reference's entries and
 right-hand sides (every local form, both quadrature rules, SDFEM types 0 and
 2, the seeded curl-Fourier convection); libpmx assembles the same parameter
 blocks and must agree within 1e-12 of the row's largest entry (rhs: of its
-largest entry)."""
+largest entry).  tests/golden/reference_solve.npz holds the reference's
+GMG-preconditioned FGMRES iterates (TFgmresIte + TMultiGridScaIte +
+TMultiGrid3D); the device's are compared with them."""
 import os
 import sys
 
@@ -46,3 +49,58 @@ def test_device_assembly_equals_reference(ctx, name, nref, order):
     if name + "_entries2" in AFIX:
         assert row_rel_err(AFIX[name + "_entries2"], S.entries(which=1), rp) < 1e-12
     S.close()
+
+
+# ---------------------------------------------- solves against ParMooN's own
+import make_solve_fixtures as SF  # noqa: E402
+
+SFIX = np.load(os.path.join(HERE, "golden", "reference_solve.npz"))
+# damped Jacobi is independent of the row order, so the device iterate after
+# a fixed number of iterations is the reference's; the SOR/SSOR smoothers run
+# in multicolour order here (lexicographic in the reference), so those cases
+# are compared converged
+ORDER_FREE = ["q1_jacobi", "q1_jacobi_w", "q1_jacobi_f", "q1_jacobi_damp", "q1_restart", "q1_config1_as_stated",
+              "q2_curl_jacobi"]
+CONVERGED = ["q1_converged", "q2_curl_converged"]
+
+
+def device_solve(ctx, name, **over):
+    nref0, nlev, order = (int(v) for v in SFIX[name + "_levels"])
+    S = pa.System(ctx, nref0, nlev, order)
+    S.assemble(SFIX[name + "_prm"])
+    rhs = S.rhs()
+    d = S.dims()
+    s0 = np.zeros_like(rhs)
+    s0[int(d[1]):] = rhs[int(d[1]):]
+    opt = SFIX[name + "_opt"].copy()
+    for k, v in over.items():
+        opt[k] = v
+    x, info, rc = S.solve(opt, rhs, s0, check=False)
+    S.close()
+    return x, rhs, info, rc
+
+
+@pytest.mark.parametrize("name", ORDER_FREE)
+def test_device_iterate_equals_reference(ctx, name):
+    """FGMRES / GMG with damped Jacobi (V, W, F cycles, GMG damping, restarts,
+    config 1 as stated) on the device in FP64 against ParMooN's iterate after
+    the same number of iterations: within 1e-10 (max norm, relative)."""
+    print(f"[case] device iterate vs reference {name}", flush=True)
+    x, rhs, info, rc = device_solve(ctx, name)
+    ref = SFIX[name + "_sol"]
+    assert np.max(np.abs(rhs - SFIX[name + "_rhs"])) <= 1e-12 * np.max(np.abs(rhs))
+    err = float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+    assert rc in (0, 4) and int(info[0]) == int(SFIX[name + "_opt"][2]) and err < 1e-10, (rc, info, err)
+
+
+@pytest.mark.parametrize("name", CONVERGED)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_solution_equals_reference(ctx, name, precision):
+    """Converged solves (SSOR, multicolour here): the device solution in FP64
+    and in FP32 inside FP64 defect correction, run to FP64 relative residual
+    1e-12, against ParMooN's converged solution within 1e-9."""
+    print(f"[case] device solution vs reference {name} precision {precision}", flush=True)
+    x, rhs, info, rc = device_solve(ctx, name, **{2: 400, 4: 0.0, 15: precision, 16: 1e-12})
+    ref = SFIX[name + "_sol"]
+    err = float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
+    assert rc == 0 and err < 1e-9, (rc, info, err)
