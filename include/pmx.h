@@ -111,7 +111,9 @@ int pmx_system_assemble_cb(pmx_system *sys, const double *prm, pmx_coeff_fct *co
 /* TSquareMatrix3D::GetEntries() of a level in CSR order; which = 0 the main
  * matrix (the operator of the solves), 1 the second matrix of two-matrix
  * forms (SUPG K of TCD3D.C:128-266), 2 the mass matrix and 3 the stiffness
- * matrix A of the time stepping. */
+ * matrix A of the time stepping; 4 the FP32 copy of the operator and 5 the
+ * row-scaled FP16 smoother values of opt[15] = 3 / 4 (a_ij / a_ii, Dirichlet
+ * rows 1), both as doubles. */
 int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries);
 /* right-hand side of a level (Rhs_array[level]) */
 int pmx_system_rhs(pmx_system *sys, int level, double *rhs);
@@ -159,7 +161,12 @@ int pmx_csr_matvec(pmx_ctx *ctx, int n, const int *rowptr, const int *kcol, cons
  *  [12] cycle 1 V 2 W 0 F  [13] GMG damp
  *  [14] BiCGStab flexible branch (1 = v = A y)
  *  [15] precision: 0 FP64 throughout, 1 FP32 operator/MG/Krylov vectors
- *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother
+ *       inside an FP64 defect correction, 2 = 1 plus the learned post-smoother;
+ *       3 / 4 = 1 / 2 with the SOR/SSOR sweeps that run launch per colour
+ *       reading row-scaled FP16 values (a_ij / a_ii, saturated to the finite
+ *       FP16 range) and 1 / a_ii in FP32: half the value bytes of the sweeps,
+ *       which only precondition (defects, transfers and Krylov vectors keep
+ *       the FP32 values, the outer residual FP64)
  *  [16] stop when ||r|| <= [16] * ||r0|| (r0 = initial FP64 defect; 0 = off)
  *  [17] FP32 inner solve: residual reduction per outer correction (default 1e-4)
  *  [18] 1 = launch the MG cycle kernel by kernel instead of replaying its HIP graph
@@ -262,7 +269,8 @@ int pmx_tcd_restore_massmat(pmx_system *sys);
  * 10 and 11 give the average duration of a single dispatch, from events
  * recorded by the dispatch itself (hipExtLaunchKernelGGL), without the gaps
  * between dispatches: 10 one colour launch of an SSOR sweep (launch per
- * colour), 11 one SpMV.  precision 0 FP64, 1 FP32.  out: ms. */
+ * colour), 11 one SpMV.  precision 0 FP64, 1 FP32, 3 FP32 with the SOR/SSOR
+ * colour launches on row-scaled FP16 values (opt[15] = 3).  out: ms. */
 int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms);
 
 /* ---- learned post-smoother (no counterpart in the reference; it replaces

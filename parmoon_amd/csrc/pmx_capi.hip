@@ -96,6 +96,10 @@ struct Level {
   DevBuf<float> valT;  // transposed FP32 stencil values (residual-loss training)
   DevBuf<uint16_t> feat;              // network input features [N][8] (BF16)
   bool have_dinv = false;
+  // FP16 smoother values: a_ij / a_ii of the FP32 values and 1 / a_ii
+  DevBuf<_Float16> val16;
+  DevBuf<float> sdinv;
+  bool have_val16 = false;
 };
 
 struct System {
@@ -134,7 +138,7 @@ struct System {
   struct Graph {
     bool valid = false;
     int prec = 0;
-    double key[15] = {0};
+    double key[16] = {0};
     hipGraphExec_t exec = nullptr;
   };
   Graph graphs[2];
@@ -154,6 +158,9 @@ struct System {
   bool learn_active = false;
   bool x_zero_shortcut = true;  // see mg_cycle
   int fallbacks = 0;
+  // FP32 forms: SOR/SSOR sweeps launched per colour read row-scaled FP16
+  // values (opt[15] = 3 / 4)
+  bool half_smoother = false;
   DevBuf<double> lscal;               // norm, (d, Ac), (Ac, Ac)
   ~System() {
     for (auto &n : nets) n.release();
@@ -387,6 +394,23 @@ static void ensure_val32(System &Y) {
   }
 }
 
+// row-scaled FP16 smoother values of every level (from the FP32 values)
+static void ensure_val16(System &Y) {
+  ensure_val32(Y);
+  for (auto &lp : Y.lev) {
+    Level &lv = *lp;
+    if (lv.have_val16) continue;
+    if (!lv.val16.p) {
+      Y.invalidate_graphs();
+      lv.val16.alloc(lv.L.nvals);
+      lv.sdinv.alloc(lv.L.N);
+      fill<float>(lv.sdinv.p, 0.0f, lv.L.N, Y.ctx->st);
+    }
+    scaled_values(lv.D, lv.v32.val.p, lv.val16.p, lv.sdinv.p, Y.ctx->st);
+    lv.have_val16 = true;
+  }
+}
+
 // buffers of the learned post-smoother on the levels that use it
 static void ensure_learned(System &Y) {
   ensure_val32(Y);
@@ -436,11 +460,19 @@ static void level_smooth(System &Y, Level &lv, int kind, double omega, int sweep
       const bool skip = kind == 3 && omega == 1.0;
       // Dirichlet rows x = f (no sweep changes them afterwards)
       if (s == 0) boundary<T>(lv.D, 2, v.x.p, nullptr, v.f.p, st);
-      for (int c = (skip && s > 0) ? 1 : 0; c < lv.L.ncolours; c++)
+      auto colour = [&](int c) {
+        if constexpr (std::is_same<T, float>::value) {
+          if (Y.half_smoother) {
+            if (!lv.have_val16) throw std::logic_error("level_smooth: FP16 smoother values missing");
+            css_sor_scaled(lv.D, lv.val16.p, lv.sdinv.p, v.x.p, v.f.p, omega, c, st);
+            return;
+          }
+        }
         css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
+      };
+      for (int c = (skip && s > 0) ? 1 : 0; c < lv.L.ncolours; c++) colour(c);
       if (kind == 3)
-        for (int c = lv.L.ncolours - (skip ? 2 : 1); c >= 0; c--)
-          css_apply<T>(lv.D, 3, v.val.p, v.x.p, v.f.p, v.x.p, omega, c, st);
+        for (int c = lv.L.ncolours - (skip ? 2 : 1); c >= 0; c--) colour(c);
     }
   }
 }
@@ -608,7 +640,7 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   copy<T>(v.f.p, vin, lv.L.N, st);
   const int pi = std::is_same<T, double>::value ? 0 : 1;
   System::Graph &g = Y.graphs[pi];
-  double key[15];
+  double key[16];
   for (int k = 0; k < 9; k++) key[k] = Y.opt[5 + k];
   const bool learned = pi == 1 && Y.learn_active;
   key[9] = learned ? Y.learned_levels : 0;
@@ -617,6 +649,7 @@ static void mg_prec(System &Y, T *z, const T *vin) {
   key[12] = learned ? Y.learned_skip : 0;
   key[13] = learned ? Y.learned_pre : 0;
   key[14] = Y.slc;
+  key[15] = pi == 1 && Y.half_smoother ? 1 : 0;
   if (Y.use_graphs) {
     if (!g.valid || std::memcmp(key, g.key, sizeof(key)) != 0) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1026,7 +1059,12 @@ static int solve_device(System &Y, double *info) {
   Y.use_graphs = Y.opt[18] == 0 && !(eager && eager[0] == '1');
   Level &lv = *Y.lev.back();
   const int64_t N = lv.L.N;
-  const int prec = (int)Y.opt[15];
+  // opt[15]: 0 FP64, 1 FP32, 2 FP32 + learned; 3 / 4 = 1 / 2 with the
+  // per-colour SOR/SSOR sweeps on row-scaled FP16 values
+  const int prec_opt = (int)Y.opt[15];
+  if (prec_opt < 0 || prec_opt > 4) throw std::invalid_argument("opt[15] (precision) must be 0..4");
+  const int prec = prec_opt >= 3 ? prec_opt - 2 : prec_opt;
+  Y.half_smoother = prec_opt >= 3;
   // ILU (id 4 of MultiGrid3D.C's switches, TMGLevel3D::ILU) is not built;
   // every other id follows the switches' branches (default: SOR / SSOR)
   if ((int)Y.opt[5] == 4 || (int)Y.opt[9] == 4)
@@ -1063,6 +1101,7 @@ static int solve_device(System &Y, double *info) {
     // FP64 defect correction around an FP32 Krylov/MG solve; precision 2
     // replaces the post-smoother of the finest levels by the learned one
     ensure_val32(Y);
+    if (Y.half_smoother) ensure_val16(Y);
     Y.fallbacks = 0;
     if (prec == 2) {
       Y.learned_levels = std::min(Y.opt[19] > 0 ? (int)Y.opt[19] : Y.trained_levels, (int)Y.lev.size() - 1);
@@ -1484,6 +1523,7 @@ static void system_assemble(System &Y, const double *prm, const HostCallbacks *c
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
                Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr);
       lv.have_val32 = false;
+      lv.have_val16 = false;
       lv.have_dinv = false;
     }
     if (cb) {
@@ -1634,6 +1674,7 @@ static void tcd_systmat(System &Y) {
     Level &l = *lp;
     system_matrix(l.v64.val.p, l.mass.p, supg ? l.val2.p : nullptr, l.stiff.p, tau * th1, l.L.nvals, st);
     l.have_val32 = false;
+    l.have_val16 = false;
     l.have_dinv = false;
   }
   check(hipMemcpyAsync(Y.oldrhs.p, lv.rhs.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
@@ -1696,14 +1737,23 @@ int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
     System &Y = sys->s;
     if (!Y.assembled) throw std::logic_error("assemble first");
     Level &lv = level_of(Y, level);
-    const double *src = which == 0 ? lv.v64.val.p : which == 1 ? lv.val2.p : which == 2 ? lv.mass.p
-                      : which == 3 ? lv.stiff.p : nullptr;
-    if (which < 0 || which > 3) throw std::invalid_argument("which: 0 operator, 1 K, 2 mass, 3 stiffness");
-    if (!src) throw std::logic_error("that matrix has not been assembled");
+    if (which < 0 || which > 5)
+      throw std::invalid_argument("which: 0 operator, 1 K, 2 mass, 3 stiffness, 4 FP32 operator, 5 FP16 smoother");
     ensure_csr_dev(Y, lv);
     DevBuf<double> out;
     out.alloc(lv.Nb.kcol.size());
-    css_to_csr<double>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, src, out.p, Y.ctx->st);
+    if (which == 4) {
+      ensure_val32(Y);
+      css_to_csr<float>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, lv.v32.val.p, out.p, Y.ctx->st);
+    } else if (which == 5) {
+      ensure_val16(Y);
+      css_to_csr<_Float16>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, lv.val16.p, out.p,
+                           Y.ctx->st);
+    } else {
+      const double *src = which == 0 ? lv.v64.val.p : which == 1 ? lv.val2.p : which == 2 ? lv.mass.p : lv.stiff.p;
+      if (!src) throw std::logic_error("that matrix has not been assembled");
+      css_to_csr<double>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, src, out.p, Y.ctx->st);
+    }
     check(hipMemcpyAsync(entries, out.p, sizeof(double) * out.n, hipMemcpyDeviceToHost, Y.ctx->st), "d2h");
     check(hipStreamSynchronize(Y.ctx->st), "sync");
   })
@@ -1940,7 +1990,12 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
     if (kernel == 7 && li != (int)Y.lev.size() - 1) throw std::invalid_argument("V-cycle timing is on the finest level");
     std::memcpy(Y.opt, PMX_DEFAULT_VCYCLE, sizeof(Y.opt));
     hipStream_t st = Y.ctx->st;
+    if (precision < 0 || precision > 3 || precision == 2) throw std::invalid_argument("precision must be 0, 1 or 3");
+    // 3: FP32 with the SOR/SSOR colour launches on row-scaled FP16 values
+    Y.half_smoother = precision == 3;
+    if (precision == 3) precision = 1;
     if (precision) ensure_val32(Y);
+    if (Y.half_smoother) ensure_val16(Y);
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "event");
     check(hipEventCreate(&e1), "event");

@@ -31,6 +31,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const double *__restric
 // 1: y = f - A x                      (ScalarDefect, Blas2.C:37-83)
 // 2: y = (1-w) x + w (f - A_off x)/d  (TMGLevel3D::Jacobi, MGLevel3D.C:469-563), x = old iterate
 // 3: x = w ((f - A_off x)/d - x) + x  (TMGLevel3D::SOR, MGLevel3D.C:280-361) in place, one colour
+// 4: mode 3 on row-scaled values: val holds a_ij / a_ii (the diagonal slot
+//    unused), dinv the lattice vector 1 / a_ii: x = w (f dinv - W_off x - x) + x
 template <typename T> struct Acc { using type = T; };
 
 // W consecutive lattice values starting at q (the x-run of one (dy, dz) group
@@ -66,10 +68,10 @@ __device__ __forceinline__ void load_run(const T *q, T (&r)[W]) {
 // element by element).  Lane `part` of the R lanes takes groups part,
 // part+R, ...  The diagonal, f and the old x of the row are loaded up front
 // by all R lanes, so their latency overlaps the stencil loads.
-template <typename T, int EX, int EY, int EZ, int R, int MODE, bool XL = false>
+template <typename T, int EX, int EY, int EZ, int R, int MODE, bool XL = false, typename V = T>
 __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part, int M,
-                                          const T *__restrict__ val, const T *x, const T *__restrict__ f, T *y,
-                                          double omega) {
+                                          const V *__restrict__ val, const T *x, const T *__restrict__ f, T *y,
+                                          double omega, const float *__restrict__ dinv = nullptr) {
   using A = typename Acc<T>::type;
   constexpr int WX = 2 * EX + 1, WY = 2 * EY + 1, WZ = 2 * EZ + 1, NG = WY * WZ;
   constexpr int DS = EX + WX * (EY + WY * EZ), DG = DS / WX;
@@ -77,10 +79,13 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   const int t = idx / s.cx;
   const int j = t % s.cy, k = t / s.cy;
   const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
-  const T *v = val + vbase(s, idx);
+  const V *v = val + vbase(s, idx);
   A fp = 0, diag = 1, told = 0;
   if constexpr (MODE >= 1) fp = (A)f[p];
-  if constexpr (MODE >= 2) {
+  if constexpr (MODE == 4) {
+    fp *= (A)dinv[p];
+    told = (A)x[p];
+  } else if constexpr (MODE >= 2) {
     diag = (A)v[DS * kSlice];
     told = (A)x[p];
   }
@@ -96,7 +101,8 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
     const int g = ok ? g0 : NG - 1;
     const int dy = g % WY - EY, dz = g / WY - EZ;
     const T *q = x + (p - EX + M * (dy + M * dz));
-    T xr[WX], vr[WX];
+    T xr[WX];
+    V vr[WX];
     if constexpr (XL) {
 #pragma unroll
       for (int e = 0; e < WX; e++) xr[e] = q[e];
@@ -108,7 +114,7 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
 #pragma unroll
     for (int e = 0; e < WX; e++) {
       const bool use = ok && !(MODE >= 2 && e == EX && g == DG);
-      acc += (A)(use ? vr[e] : (T)0) * (A)xr[e];
+      acc += (use ? (A)vr[e] : (A)0) * (A)xr[e];
     }
   }
 #pragma unroll
@@ -119,7 +125,7 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   else if (MODE == 2) y[p] = (T)((1 - (A)omega) * told + (A)omega * (fp - acc) / diag);
   // SOR: the Gauss-Seidel value itself when omega = 1, so that a colour
   // applied twice in a row gives the same bits (the neighbours it reads are
-  // unchanged) and sweeps may skip such a repeat
+  // unchanged) and sweeps may skip such a repeat (diag = 1 in mode 4)
   else if (omega == 1.0) y[p] = (T)((fp - acc) / diag);
   else y[p] = (T)((A)omega * ((fp - acc) / diag - told) + told);
 }
@@ -148,6 +154,58 @@ __global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(cons
     default: PMX_ROW(2, 2, 2); break;
   }
 #undef PMX_ROW
+}
+
+// Mode 4 (SOR of one colour on row-scaled FP16 values, FP32 vectors), laid
+// out as k_rows.
+template <int R>
+__global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_sor_scaled(
+    const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
+    const _Float16 *__restrict__ val, const float *__restrict__ dinv, float *x, const float *__restrict__ f,
+    double omega) {
+  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x / R;
+  const int part = threadIdx.x % R;
+  if (idx >= s.count) return;
+#define PMX_ROW(EX, EY, EZ) row_split<float, EX, EY, EZ, R, 4, false, _Float16>(s, idx, part, M, val, x, f, x, omega, dinv)
+  switch (s.ex * 100 + s.ey * 10 + s.ez) {
+    case 111: PMX_ROW(1, 1, 1); break;
+    case 211: PMX_ROW(2, 1, 1); break;
+    case 121: PMX_ROW(1, 2, 1); break;
+    case 221: PMX_ROW(2, 2, 1); break;
+    case 112: PMX_ROW(1, 1, 2); break;
+    case 212: PMX_ROW(2, 1, 2); break;
+    case 122: PMX_ROW(1, 2, 2); break;
+    default: PMX_ROW(2, 2, 2); break;
+  }
+#undef PMX_ROW
+}
+
+// Row-scaled FP16 copy of the FP32 stencil values for mode 4: slot k of an
+// active row holds a_k / a_diag rounded to FP16 (saturated at +-65504, the
+// largest finite FP16 value), and dinv[p] = 1 / a_diag in FP32.  One thread
+// per row.
+__global__ __launch_bounds__(kRowsPerBlock) void k_scaled_values(const BlockInfo *__restrict__ blocks,
+                                                                 const DevSegment *__restrict__ segs, int M,
+                                                                 const float *__restrict__ val,
+                                                                 _Float16 *__restrict__ val16,
+                                                                 float *__restrict__ dinv) {
+  const BlockInfo bi = blocks[blockIdx.x];
+  const DevSegment s = segs[bi.seg];
+  const int idx = bi.base + threadIdx.x;
+  if (idx >= s.count) return;
+  const int i = idx % s.cx, t = idx / s.cx;
+  const int j = t % s.cy, k = t / s.cy;
+  const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
+  const int64_t vb = vbase(s, idx);
+  const float d = val[vb + (int64_t)s.dslot * kSlice];
+  const float r = 1.0f / d;
+  dinv[p] = r;
+  for (int q = 0; q < s.ns; q++) {
+    const float w = val[vb + (int64_t)q * kSlice] * r;
+    val16[vb + (int64_t)q * kSlice] = (_Float16)fminf(fmaxf(w, -65504.0f), 65504.0f);
+  }
 }
 
 // The rows of colour c of a level staged in LDS (segment table lseg, colour
@@ -1092,6 +1150,32 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
 #undef PMX_ROWS_R
 #undef PMX_ROWS
 }
+void css_sor_scaled(const DevLayout &D, const _Float16 *val16, const float *dinv, float *x, const float *f,
+                    double omega, int colour, hipStream_t st) {
+  const BlockInfo *blk = D.blocks_colour + D.colour_off[colour];
+  const int nblk = D.colour_off[colour + 1] - D.colour_off[colour];
+  if (nblk == 0) return;
+#define PMX_ROWS(R)                                                                                            \
+  if (g_timer) {                                                                                               \
+    hipEvent_t e0, e1;                                                                                         \
+    timer_events(&e0, &e1);                                                                                    \
+    hipExtLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk, D.segs, \
+                          D.M, val16, dinv, x, f, omega);                                                      \
+  } else                                                                                                       \
+    hipLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val16,   \
+                       dinv, x, f, omega)
+  switch (row_lanes(D)) {
+    case 1: PMX_ROWS(1); break;
+    case 2: PMX_ROWS(2); break;
+    default: PMX_ROWS(4); break;
+  }
+#undef PMX_ROWS
+}
+void scaled_values(const DevLayout &D, const float *val, _Float16 *val16, float *dinv, hipStream_t st) {
+  if (D.nblocks_all == 0) return;
+  hipLaunchKernelGGL(k_scaled_values, dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st, D.blocks_all, D.segs, D.M,
+                     val, val16, dinv);
+}
 template <typename T>
 void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st) {
   if (D.nsegs > kMaxSegs || D.ncolours > kMaxColours) throw std::logic_error("sweep_wg: segment table too large");
@@ -1602,6 +1686,8 @@ void set_scalar(double *dst, const double *src, double scale, hipStream_t st) {
                               double *, hipStream_t);
 PMX_INST(double)
 PMX_INST(float)
+template void css_to_csr<_Float16>(const DevLayout &, const int *, const int *, const int *, int64_t,
+                                   const _Float16 *, double *, hipStream_t);
 template void fill<uint16_t>(uint16_t *, uint16_t, int64_t, hipStream_t);
 template void fill<int>(int *, int, int64_t, hipStream_t);
 template void convert<float, double>(float *, const double *, int64_t, hipStream_t);

@@ -96,6 +96,13 @@ inline int row_lanes(const DevLayout &D) {
   if (D.max_colour_rows <= kSweepWgRows) return 4;
   return D.nactive >= (int64_t(1) << 20) ? 1 : 2;
 }
+// one SOR colour (mode 4 of k_rows) on the row-scaled FP16 values and the
+// FP32 1/diag of scaled_values; x in place
+void css_sor_scaled(const DevLayout &D, const _Float16 *val16, const float *dinv, float *x, const float *f,
+                    double omega, int colour, hipStream_t st);
+// the row-scaled FP16 copy of the FP32 stencil values (a_ij / a_ii,
+// saturated to the finite FP16 range) and dinv = 1 / a_ii on the active rows
+void scaled_values(const DevLayout &D, const float *val, _Float16 *val16, float *dinv, hipStream_t st);
 template <typename T>
 void sweep_wg(const DevLayout &D, const T *val, T *x, const T *f, double omega, int ssor, hipStream_t st);
 // the coarse-level smoothing loop of TMultiGrid3D (MultiGrid3D.C:225-337) in

@@ -64,7 +64,7 @@ ORDER_FREE = ["q1_jacobi", "q1_jacobi_w", "q1_jacobi_f", "q1_jacobi_damp", "q1_r
 CONVERGED = ["q1_converged", "q2_curl_converged"]
 
 
-def device_solve(ctx, name, **over):
+def device_solve(ctx, name, over=None):
     nref0, nlev, order = (int(v) for v in SFIX[name + "_levels"])
     S = pa.System(ctx, nref0, nlev, order)
     S.assemble(SFIX[name + "_prm"])
@@ -73,7 +73,7 @@ def device_solve(ctx, name, **over):
     s0 = np.zeros_like(rhs)
     s0[int(d[1]):] = rhs[int(d[1]):]
     opt = SFIX[name + "_opt"].copy()
-    for k, v in over.items():
+    for k, v in (over or {}).items():
         opt[k] = v
     x, info, rc = S.solve(opt, rhs, s0, check=False)
     S.close()
@@ -100,7 +100,7 @@ def test_device_solution_equals_reference(ctx, name, precision):
     and in FP32 inside FP64 defect correction, run to FP64 relative residual
     1e-12, against ParMooN's converged solution within 1e-9."""
     print(f"[case] device solution vs reference {name} precision {precision}", flush=True)
-    x, rhs, info, rc = device_solve(ctx, name, **{2: 400, 4: 0.0, 15: precision, 16: 1e-12})
+    x, rhs, info, rc = device_solve(ctx, name, {2: 400, 4: 0.0, 15: precision, 16: 1e-12})
     ref = SFIX[name + "_sol"]
     err = float(np.max(np.abs(x - ref)) / np.max(np.abs(ref)))
     assert rc == 0 and err < 1e-9, (rc, info, err)
