@@ -12,7 +12,13 @@ residual 1e-10 with V(2,2) multicolour-SSOR GMG preconditioning FGMRES(30):
               (three network steps replace the pre- and the post-smoothing
               there; the finest level keeps SSOR).  The networks are trained
               once before the timed steps; training time is reported
-              separately, not part of the step.
+              separately, not part of the step;
+  * learned_fp16s : learned with the finest level's SSOR colour launches
+              on row-scaled FP16 copies of the values (reported alongside;
+              the metric's value is the learned form with FP32 storage).
+After the timed forms the networks are trained again on other seeds of the
+Fourier potential and the bench operator is solved with them (off-operator
+networks), and the training-inclusive time of the learned form is reported.
 The metric is the learned form's time to 1e-10 (seconds, lower is better),
 with the plain FP64 time, assembly DOF/s and the SpMV roofline alongside.
 
@@ -175,6 +181,30 @@ def transient(args):
         res[form] = {"total_s": round(wall, 4), "solve_s": round(t_solve, 4), "per_step_ms": round(wall / steps * 1e3, 3),
                      "iterations": its, "fallbacks": fb, "rc": int(rc), "final_max": float(np.max(sol))}
         print(f"[bench] transient {form}: {res[form]}", file=sys.stderr, flush=True)
+    offop = None
+    if train is not None and not args.no_offop:
+        # networks trained on other seeds of the Fourier potential (the same
+        # steps in total, spread over three seeds), then the bench operator
+        # solved with them
+        seeds = (7, 11, 13)
+        S.learned_init(2)
+        t_off = 0.0
+        for sd in seeds:
+            S.assemble(P.config2(nref=nref, seed=sd))
+            ti = S.learned_train(opt(1), samples=32, steps=max(1, args.train_steps // len(seeds)), lr=2e-3, seed=sd,
+                                 levels=args.learned_levels, data=1, skip=args.learned_skip)
+            t_off += float(ti[2])
+        S.assemble(prm)
+        rows = []
+        for k in range(max(1, args.steps)):
+            info, rc = S.solve_resident(opt(2), check=False)
+            rows.append((float(info[4]), int(info[0]), int(rc), int(info[5])))
+        t_o = float(np.median([r[0] for r in rows]))
+        offop = {"train_seeds": list(seeds), "train_seconds": round(t_off, 3), "solve_s": round(t_o, 5),
+                 "iterations": rows[-1][1], "rc": rows[-1][2], "fallbacks": rows[-1][3],
+                 "speedup_vs_plain_fp64": round(res["fp64"]["t_solve"] / t_o, 3) if "fp64" in res else None}
+        print(f"[bench] off-operator networks: {offop}", file=sys.stderr, flush=True)
+
     main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
     t64 = res.get("fp64", {}).get("total_s")
     out = {"metric": "TCD3D transient: total time of the time steps (s), learned+FP32 vs plain FP64",
@@ -280,7 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nref", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="learned,fp32,fp64")
+    ap.add_argument("--forms", default="learned,learned_fp16s,fp32,fp64")
+    ap.add_argument("--no-offop", action="store_true", help="skip the off-operator training run")
     # learned-form defaults: the steady Q2 workload smooths the levels below
     # the finest with three network steps in place of pre- and post-smoothing;
     # the transient Q1 workload (8-colour SSOR, cheap on coarse levels) uses
@@ -344,9 +375,9 @@ def main():
                             **learned_opts(args))
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    prec_of = {"fp64": 0, "fp32": 1, "learned": 2}
+    prec_of = {"fp64": 0, "fp32": 1, "learned": 2, "learned_fp16s": 4}
     train = None
-    if "learned" in forms:
+    if "learned" in forms or "learned_fp16s" in forms:
         t0 = time.perf_counter()
         tinfo = S.learned_train(opt(1), samples=32, steps=args.train_steps, lr=2e-3, seed=1,
                                 levels=args.learned_levels, data=1, skip=args.learned_skip)
@@ -467,7 +498,15 @@ def main():
         "plain_fp64_time_s": t64,
         "fp32_time_s": res.get("fp32", {}).get("t_solve"),
         "learned_time_s": res.get("learned", {}).get("t_solve"),
+        "learned_fp16_smoother_time_s": res.get("learned_fp16s", {}).get("t_solve"),
         "speedup_vs_plain_fp64": round(t64 / t_main, 3) if t64 else None,
+        "speedup_learned_fp16_smoother_vs_plain_fp64": (round(t64 / res["learned_fp16s"]["t_solve"], 3)
+                                                        if t64 and "learned_fp16s" in res else None),
+        # training counted once, with one solve: the whole cost of the learned
+        # form for a single right-hand side
+        "learned_with_training_s": (round(train["seconds"] + t_main, 4) if train and main_form == "learned"
+                                    else None),
+        "learned_off_operator": offop,
         "learned_training": train,
         "setup_s": round(t_setup, 3),
         "assembly_dofs_per_s": round(N / t_asm, 1),

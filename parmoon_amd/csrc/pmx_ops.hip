@@ -1164,7 +1164,12 @@ void css_sor_scaled(const DevLayout &D, const _Float16 *val16, const float *dinv
   } else                                                                                                       \
     hipLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val16,   \
                        dinv, x, f, omega)
-  switch (row_lanes(D)) {
+#ifdef PMX_SCALED_LANES
+  const int lanes = PMX_SCALED_LANES;  // a fixed lane count (tuning builds)
+#else
+  const int lanes = row_lanes(D);
+#endif
+  switch (lanes) {
     case 1: PMX_ROWS(1); break;
     case 2: PMX_ROWS(2); break;
     default: PMX_ROWS(4); break;

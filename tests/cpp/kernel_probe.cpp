@@ -3,8 +3,9 @@
 // PMX_PRM_LEN doubles (written by `python -m parmoon_amd.prm config2 FILE`,
 // the curl-Fourier convection of the benchmark).  Five levels 4^3 .. 64^3,
 // then on the finest level: 6 FP32 stencil SpMV launches, 3 FP32 SSOR sweeps
-// (53 colour launches each; the first SpMV and sweep are untimed) and 5
-// device copies of 1 GiB (hipMemcpy device to device, the calibration kernel
+// (53 colour launches each; the first SpMV and sweep are untimed), 3 SSOR
+// sweeps on the row-scaled FP16 values (k_sor_scaled, the first untimed) and
+// 5 device copies of 1 GiB (hipMemcpy device to device, the calibration kernel
 // of known size).  Prints the average dispatch durations.
 //
 //   kernel_probe PRM_FILE
@@ -48,6 +49,11 @@ int main(int argc, char **argv) {
     return 1;
   }
   std::printf("fp32 ssor colour launch %.4f ms per dispatch\n", ms);
+  if (pmx_time_kernel(sys, 10, -1, 3, 2, &ms)) {
+    std::printf("time_kernel failed: %s\n", pmx_last_error());
+    return 1;
+  }
+  std::printf("fp32 ssor colour launch, FP16 values %.4f ms per dispatch\n", ms);
   const size_t bytes = size_t(1) << 30;
   void *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) return 1;
