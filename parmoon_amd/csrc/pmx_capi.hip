@@ -109,6 +109,7 @@ struct System {
   bool assembled = false;
   bool two_matrices = false;
   DevBuf<double> prm, qref, tab, qpbuf;
+  int64_t qp_tab_off = 0;  // offset of the curl-Fourier phase tables in qpbuf
   int nq = 0;
   int tab_key = -1;                   // rule of the uploaded tables
   int example = 0;                    // prm[PMX_PRM_EXAMPLE] of the last assembly
@@ -323,7 +324,10 @@ static void ensure_tables(System &Y, bool linear) {
   Y.qref.upload(qref.data(), qref.size(), Y.ctx->st);
   Y.tab.upload(tab.data(), tab.size(), Y.ctx->st);
   Y.nq = nq;
-  Y.qpbuf.alloc((size_t)Y.lev.back()->Nb.ncells * nq * 8);
+  // the qp records of the finest level, then room for the curl-Fourier
+  // phase tables of any level
+  Y.qp_tab_off = (int64_t)Y.lev.back()->Nb.ncells * nq * 8;
+  Y.qpbuf.alloc((size_t)(Y.qp_tab_off + fourier_tab_len(Y.lev.back()->L.n, nq)));
   Y.tab_key = key;
 }
 
@@ -1517,11 +1521,11 @@ static void system_assemble(System &Y, const double *prm, const HostCallbacks *c
     if (rhs_only) {
       // Assemble3D with no matrices: only the right-hand side
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
-               lv.rhs.p, st, cb != nullptr);
+               lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off);
     } else {
       if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
-               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr);
+               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off);
       lv.have_val32 = false;
       lv.have_val16 = false;
       lv.have_dinv = false;
@@ -1598,7 +1602,8 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
     else lv.stiff.alloc(lv.L.nvals);
     if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
     assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
-             mass ? lv.mass.p : lv.stiff.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st);
+             mass ? lv.mass.p : lv.stiff.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false,
+             Y.qpbuf.p + Y.qp_tab_off);
   }
 }
 
@@ -2007,7 +2012,7 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 1: css_apply<double>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
           case 2: level_smooth<double>(Y, lv, 1, 0.8, 1); break;
           case 3: level_smooth<double>(Y, lv, 3, 1.0, 1); break;
-          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st); break;
+          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false, Y.qpbuf.p + Y.qp_tab_off); break;
           case 5: if (li > 0) prolongate<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.f.p, v.d.p, st); break;
           case 7: mg_prec<double>(Y, v.aux.p, v.d.p); break;

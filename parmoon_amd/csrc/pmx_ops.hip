@@ -529,6 +529,76 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
   const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
   qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], out + id * 8);
 }
+// The curl-Fourier convection (fourier_b) without a cosine per mode and
+// point.  The quadrature rules are tensor products (point q = ix + np (iy +
+// np iz) at (xi_ix, xi_iy, xi_iz)), so the phase of mode m at point q of cell
+// (cx, cy, cz), 2 pi k . X + phi, is a_x(cx, ix) + a_y(cy, iy) + a_z(cz, iz)
+// with a_d = 2 pi k_d X_d along one axis (phi in a_x), and
+// cos(phase) = Re(e^{i a_x} e^{i a_y} e^{i a_z}).  The per-axis factors are
+// one table per level (mode, axis, cell index, 1D point); a point then costs
+// two complex products per mode instead of a cosine, and the three loads of
+// a mode touch one cache line per wavefront.  The values differ from the
+// direct evaluation by rounding only.
+__device__ __forceinline__ int rule_points_1d(int nq) { return nq <= 8 ? 2 : (nq <= 27 ? 3 : 4); }
+__global__ __launch_bounds__(kBlock) void k_fourier_tables(const double *__restrict__ prm,
+                                                           const double *__restrict__ qref, int nq, int n,
+                                                           double *__restrict__ tab) {
+  const int np = rule_points_1d(nq);
+  const int id = blockIdx.x * kBlock + threadIdx.x;
+  if (id >= kFourierModes * 3 * n * np) return;
+  const int ci = id % (n * np), d = (id / (n * np)) % 3, m = id / (3 * n * np);
+  const int c = ci / np, i = ci % np;
+  const double twopi = 6.283185307179586476925286766559;
+  const double h = 1.0 / n;
+  const double *t = prm + PMX_PRM_FOURIER + m * 5;
+  // point i of the 1D rule is the first coordinate of point q = i
+  const double X = (c * h + 0.5 * h) + 0.5 * h * qref[4 * i];
+  const double a = twopi * t[1 + d] * X + (d == 0 ? t[4] : 0.0);
+  tab[2 * id] = cos(a);
+  tab[2 * id + 1] = sin(a);
+}
+// Stage 1 for the curl-Fourier example: the records of k_qp_data with b
+// from the table above (eps, c = 0 and f = 1 as example_coeffs).
+// Grid: x over the points of one line of cells (cx, q), y = cy + n cz; the
+// rule size is a template parameter so that no index needs a runtime division.
+template <int NQ>
+__global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__restrict__ prm,
+                                                            const double *__restrict__ qref, int n,
+                                                            const double *__restrict__ tab, double *__restrict__ out) {
+  constexpr int np = NQ == 8 ? 2 : (NQ == 27 ? 3 : 4);
+  static_assert(np * np * np == NQ, "tensor rule");
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * NQ) return;
+  const int cx = t / NQ, q = t % NQ;
+  const int cy = blockIdx.y % n, cz = blockIdx.y / n;
+  const int64_t id = ((int64_t)blockIdx.y * n + cx) * NQ + q;
+  const int ix = q % np, iy = (q / np) % np, iz = q / (np * np);
+  const double twopi = 6.283185307179586476925286766559;
+  const int64_t L = (int64_t)n * np;  // entries per (mode, axis)
+  double dA[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 4
+  for (int m = 0; m < kFourierModes; m++) {
+    const double *t = prm + PMX_PRM_FOURIER + m * 5;
+    const double *fx = tab + 2 * ((m * 3 + 0) * L + cx * np + ix);
+    const double *fy = tab + 2 * ((m * 3 + 1) * L + cy * np + iy);
+    const double *fz = tab + 2 * ((m * 3 + 2) * L + cz * np + iz);
+    const double r1 = fx[0] * fy[0] - fx[1] * fy[1], i1 = fx[0] * fy[1] + fx[1] * fy[0];
+    const double cs = t[0] * (r1 * fz[0] - i1 * fz[1]) * twopi;
+    const int c = m / 8;
+    dA[c][0] += cs * t[1];
+    dA[c][1] += cs * t[2];
+    dA[c][2] += cs * t[3];
+  }
+  const double sc = prm[PMX_PRM_BSCALE];
+  QpCoeffs k;
+  k.eps = prm[PMX_PRM_EPS];
+  k.b0 = sc * (dA[2][1] - dA[1][2]);
+  k.b1 = sc * (dA[0][2] - dA[2][0]);
+  k.b2 = sc * (dA[1][0] - dA[0][1]);
+  k.c = 0;
+  k.f = 1.0;
+  qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8);
+}
 // examples whose coefficients are cheaper to evaluate again than to read back
 __host__ __device__ inline bool coeffs_inline(int example) {
   return example == PMX_EX_LAPLACE || example == PMX_EX_SINE_CD || example == PMX_EX_HEAT_AMC ||
@@ -1459,13 +1529,26 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble_kc(const BlockInfo *
 
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
               const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
-              bool qp_given) {
+              bool qp_given, double *ftab) {
   const int64_t ncells = (int64_t)D.n * D.n * D.n;
   const int64_t ncq = ncells * nq;
   const bool inl = !qp_given && coeffs_inline(example);
-  if (!inl && !qp_given)
-    hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev, qref,
-                       nq, D.n, qpbuf);
+  if (!inl && !qp_given) {
+    // (grid y = n^2 lines of cells: up to 255 cells per direction)
+    if (example == PMX_EX_CURL_FOURIER && ftab && (int64_t)D.n * D.n <= 65535) {
+      const int64_t nt = fourier_tab_len(D.n, nq) / 2;  // complex entries
+      hipLaunchKernelGGL(k_fourier_tables, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                         prm_dev, qref, nq, D.n, ftab);
+      const dim3 g((unsigned)((D.n * nq + kBlock - 1) / kBlock), (unsigned)(D.n * D.n));
+      if (nq == 8) hipLaunchKernelGGL(k_qp_data_fourier<8>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
+      else if (nq == 27) hipLaunchKernelGGL(k_qp_data_fourier<27>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
+      else if (nq == 64) hipLaunchKernelGGL(k_qp_data_fourier<64>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
+      else throw std::logic_error("assemble: rule without the curl-Fourier kernel");
+    } else {
+      hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev,
+                         qref, nq, D.n, qpbuf);
+    }
+  }
   if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
     // right-hand-side forms (no local matrices): the matrices are left as they are
 #define PMX_RHS(O, IN)                                                                                     \

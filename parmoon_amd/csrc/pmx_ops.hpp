@@ -146,9 +146,19 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
 // buffer of n^3 x 27 x 28 doubles for the Q2 one-matrix forms on MFMA
 // (nullptr: the per-row evaluation); qp_given: qpbuf already holds the
 // quadrature-point records (coefficients from host callbacks)
+// ftab (optional): room for fourier_table_len(n, nq) doubles; with it the
+// curl-Fourier example's coefficients come from per-level phase tables
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
               const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
-              bool qp_given = false);
+              bool qp_given = false, double *ftab = nullptr);
+// doubles of the curl-Fourier phase table of a level with n cells per
+// direction and an nq-point tensor rule (complex factors of the 3 x 8 modes
+// of PMX_PRM_FOURIER per axis, cell index and 1D point)
+constexpr int kFourierModes = 24;
+inline int64_t fourier_tab_len(int n, int nq) {
+  const int np = nq <= 8 ? 2 : (nq <= 27 ? 3 : 4);
+  return 2 * (int64_t)kFourierModes * 3 * n * np;
+}
 // y[idx[i]] = v[i]
 void scatter(double *y, const int *idx, const double *v, int n, hipStream_t st);
 constexpr int64_t kCellMatrixDoubles = 27 * 28;  // cell buffer entries per Q2 cell
