@@ -47,8 +47,8 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 # WRITE_SIZE x 1.000 passes over tests/cpp/kernel_probe (the factors make the
 # counters of a 1 GiB device copy read 1 GiB each), in
 # profiles/k_rows_ssor_spmv_fp32_q2_64_hbm_traffic.md:
-TRAFFIC_SSOR_LAUNCH = None   # k_rows<float,3,1>, one colour launch (average over the sweep)
-TRAFFIC_SPMV = 556_016_000   # k_rows<float,0,1>
+TRAFFIC_SSOR_LAUNCH = 30_778_389  # k_rows<float,3,1>, one colour launch (average over the sweeps)
+TRAFFIC_SPMV = 557_516_521  # k_rows<float,0,1>
 
 
 def cpu_info():
@@ -141,7 +141,7 @@ def transient(args):
     tau, steps = 1e-3, args.transient
     td = [tau, .5, .5, .5, .5]
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    prec_of = {"fp64": 0, "fp32": 1, "learned": 2}
+    prec_of = {"fp64": 0, "fp32": 1, "learned": 2, "learned_fp16s": 4}
     omega = args.omega if args.omega else (0.4 if supg else 1.0)
     base = dict(solver=16, restart=50, maxit=300, res_norm_min=0.0, red_factor=0.0, smoother=3, omega=omega,
                 pre=2, post=2, rel_target=1e-10)
@@ -149,7 +149,7 @@ def transient(args):
     train = None
     for form in forms:
         S.tcd_init(prm, None)
-        if form == "learned" and train is None:
+        if form in ("learned", "learned_fp16s") and train is None:
             # the system matrix of the first step, then the networks
             p1 = prm.copy()
             p1[P.TIME] = tau
@@ -181,30 +181,6 @@ def transient(args):
         res[form] = {"total_s": round(wall, 4), "solve_s": round(t_solve, 4), "per_step_ms": round(wall / steps * 1e3, 3),
                      "iterations": its, "fallbacks": fb, "rc": int(rc), "final_max": float(np.max(sol))}
         print(f"[bench] transient {form}: {res[form]}", file=sys.stderr, flush=True)
-    offop = None
-    if train is not None and not args.no_offop:
-        # networks trained on other seeds of the Fourier potential (the same
-        # steps in total, spread over three seeds), then the bench operator
-        # solved with them
-        seeds = (7, 11, 13)
-        S.learned_init(2)
-        t_off = 0.0
-        for sd in seeds:
-            S.assemble(P.config2(nref=nref, seed=sd))
-            ti = S.learned_train(opt(1), samples=32, steps=max(1, args.train_steps // len(seeds)), lr=2e-3, seed=sd,
-                                 levels=args.learned_levels, data=1, skip=args.learned_skip)
-            t_off += float(ti[2])
-        S.assemble(prm)
-        rows = []
-        for k in range(max(1, args.steps)):
-            info, rc = S.solve_resident(opt(2), check=False)
-            rows.append((float(info[4]), int(info[0]), int(rc), int(info[5])))
-        t_o = float(np.median([r[0] for r in rows]))
-        offop = {"train_seeds": list(seeds), "train_seconds": round(t_off, 3), "solve_s": round(t_o, 5),
-                 "iterations": rows[-1][1], "rc": rows[-1][2], "fallbacks": rows[-1][3],
-                 "speedup_vs_plain_fp64": round(res["fp64"]["t_solve"] / t_o, 3) if "fp64" in res else None}
-        print(f"[bench] off-operator networks: {offop}", file=sys.stderr, flush=True)
-
     main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
     t64 = res.get("fp64", {}).get("total_s")
     out = {"metric": "TCD3D transient: total time of the time steps (s), learned+FP32 vs plain FP64",
@@ -468,6 +444,30 @@ def main():
     asm4_finest_ms = S.time_kernel(4, -1, 0, reps=3)
     t_asm4 = sum(S.time_kernel(4, lvl, 0, reps=3) for lvl in range(5)) * 1e-3
     flops_asm4 = 428544.0 * (2 ** nref) ** 3  # Q2, 64-point rule, B^T D B model per cell
+
+    offop = None
+    if train is not None and not args.no_offop:
+        # networks trained on other seeds of the Fourier potential (the same
+        # steps in total, spread over three seeds), then the bench operator
+        # solved with them
+        seeds = (7, 11, 13)
+        S.learned_init(2)
+        t_off = 0.0
+        for sd in seeds:
+            S.assemble(P.config2(nref=nref, seed=sd))
+            ti = S.learned_train(opt(1), samples=32, steps=max(1, args.train_steps // len(seeds)), lr=2e-3, seed=sd,
+                                 levels=args.learned_levels, data=1, skip=args.learned_skip)
+            t_off += float(ti[2])
+        S.assemble(prm)
+        rows = []
+        for k in range(max(1, args.steps)):
+            info, rc = S.solve_resident(opt(2), check=False)
+            rows.append((float(info[4]), int(info[0]), int(rc), int(info[5])))
+        t_o = float(np.median([r[0] for r in rows]))
+        offop = {"train_seeds": list(seeds), "train_seconds": round(t_off, 3), "solve_s": round(t_o, 5),
+                 "iterations": rows[-1][1], "rc": rows[-1][2], "fallbacks": rows[-1][3],
+                 "speedup_vs_plain_fp64": round(res["fp64"]["t_solve"] / t_o, 3) if "fp64" in res else None}
+        print(f"[bench] off-operator networks: {offop}", file=sys.stderr, flush=True)
 
     main_form = next(f for f in ("learned", "fp32", "fp64") if f in res)
     t_main = res[main_form]["t_solve"]
