@@ -33,6 +33,10 @@ enum pmx_prm_index {
   PMX_PRM_GRF = 136,       /* 64 features x {wx, wy, wz, phase} = 256           */
   PMX_PRM_GRF_SIGMA = 392, /* log-normal field: eps(x) = eps * exp(sigma*G(x))  */
   PMX_PRM_BLOB = 393,      /* Gaussian blob centre (3) and width (1)            */
+  PMX_PRM_CELL_MEASURE = 397, /* ParamDB->CELL_MEASURE (Assemble3D.C:314-336): hK of the
+                               SUPG parameters; 0 diameter, 1 / 3 cube root of the volume,
+                               2 shortest edge, 4 mesh size in the convection direction
+                               (ConvDiff.C:373-485)                              */
   PMX_PRM_LEN = 512
 };
 

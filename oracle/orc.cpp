@@ -409,6 +409,67 @@ static double diameter(const double *v) {
   return std::sqrt(d);
 }
 
+// hK of a cell for CELL_MEASURE (Assemble3D.C:314-336): 0 and every id
+// without its own case the diameter, 2 the shortest edge, 1 and 3 the cube
+// root of the measure (adet: |det J| of the affine map from [-1,1]^3)
+static double cell_hk(const double *prm, const double *v, double adet) {
+  const int m = (int)prm[PMX_PRM_CELL_MEASURE];
+  if (m == 2) {
+    const int e[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                          {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    double d = HUGE_VAL;
+    for (auto &p : e) {
+      double s2 = 0;
+      for (int k = 0; k < 3; k++) s2 += (v[3 * p[0] + k] - v[3 * p[1] + k]) * (v[3 * p[0] + k] - v[3 * p[1] + k]);
+      d = std::min(d, s2);
+    }
+    return std::sqrt(d);
+  }
+  if (m == 1 || m == 3) return std::cbrt(8.0 * adet);
+  return diameter(v);
+}
+// Mesh_size_in_convection_direction_without_storing, hexahedron branch
+// (src/FE/ConvDiff.C:430-485): the coefficients of the 8 trilinear vertex
+// basis functions in (1, x, y, z, xy, xz, yz, xyz) from the vertex values
+// (a dense 8 x 8 solve), their gradients at the barycentre, and
+// 2 |b| / sum_i |b . grad phi_i| (hK when the sum is below 1e-10)
+static double mesh_size_convection(const double *v, double hK, double b1, double b2, double b3) {
+  double sx = 0, sy = 0, sz = 0;
+  for (int i = 0; i < 8; i++) { sx += v[3 * i]; sy += v[3 * i + 1]; sz += v[3 * i + 2]; }
+  sx /= 8; sy /= 8; sz /= 8;
+  // A[k][j] = m_j(vertex k); solve A C = I (C column i: coefficients of phi_i)
+  double A[8][16];
+  for (int k = 0; k < 8; k++) {
+    const double x = v[3 * k], y = v[3 * k + 1], z = v[3 * k + 2];
+    const double m[8] = {1, x, y, z, x * y, x * z, y * z, x * y * z};
+    for (int j = 0; j < 8; j++) { A[k][j] = m[j]; A[k][8 + j] = k == j ? 1.0 : 0.0; }
+  }
+  for (int c = 0; c < 8; c++) {  // Gauss-Jordan with partial pivoting
+    int piv = c;
+    for (int r = c + 1; r < 8; r++)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    for (int j = 0; j < 16; j++) std::swap(A[c][j], A[piv][j]);
+    const double d = A[c][c];
+    for (int j = 0; j < 16; j++) A[c][j] /= d;
+    for (int r = 0; r < 8; r++)
+      if (r != c) {
+        const double f = A[r][c];
+        for (int j = 0; j < 16; j++) A[r][j] -= f * A[c][j];
+      }
+  }
+  double den = 0;
+  for (int i = 0; i < 8; i++) {
+    double C[8];
+    for (int j = 0; j < 8; j++) C[j] = A[j][8 + i];
+    double val = b1 * (C[1] + C[4] * sy + C[5] * sz + C[7] * sy * sz);
+    val += b2 * (C[2] + C[4] * sx + C[6] * sz + C[7] * sx * sz);
+    val += b3 * (C[3] + C[5] * sx + C[6] * sy + C[7] * sx * sy);
+    den += std::fabs(val);
+  }
+  if (den < 1e-10) return hK;
+  return 2 * std::sqrt(b1 * b1 + b2 * b2 + b3 * b3) / den;
+}
+
 // TFE3DMapper::MapDOF (src/FE/FE3DMapper.C:173-296): union of two local
 // positions, the smaller position keeps the mark, larger marks win on merge.
 static void map_dof(std::vector<int> &G, int dof0, int dof1, int &counter) {
@@ -602,11 +663,20 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
   const double tau = prm[PMX_PRM_TAU], th1 = prm[PMX_PRM_THETA1];
   for (int i = 0; i < (int)S.cells.size(); i++) {
     const double *v = S.verts[i].data();
-    const double hK = diameter(v);
     Affine A(v);
     double I[9];
     A.inv(I);
     const double adet = std::fabs(A.det);
+    const double hK = cell_hk(prm, v, adet);
+    // the h_K of Compute_SDFEM_delta: with CELL_MEASURE 4 the mesh size in the
+    // convection direction of the cell's first call, kept for the cell
+    // (INTERNAL_HK_CONVECTION, ConvDiff.C:373-383, Assemble3D.C:389-405)
+    double hk_conv = -1;
+    auto hd = [&](double b1, double b2, double b3) {
+      if ((int)prm[PMX_PRM_CELL_MEASURE] != 4) return hK;
+      if (hk_conv < 0) hk_conv = mesh_size_convection(v, hK, b1, b2, b3);
+      return hk_conv;
+    };
     std::fill(L.begin(), L.end(), 0.0);
     std::fill(L2.begin(), L2.end(), 0.0);
     std::fill(Lr.begin(), Lr.end(), 0.0);
@@ -636,7 +706,7 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
         }
       } else if (form == PMX_FORM_SDFEM) {
         // BilinearAssemble_SD (ConvDiff3D.C:150-214): no second derivatives
-        const double delta = sdfem_delta(prm, hK, c.eps, c.b[0], c.b[1], c.b[2]);
+        const double delta = sdfem_delta(prm, hd(c.b[0], c.b[1], c.b[2]), c.eps, c.b[0], c.b[1], c.b[2]);
         for (int a = 0; a < nb; a++) {
           const double bgv = c.b[0] * gx[a] + c.b[1] * gy[a] + c.b[2] * gz[a];
           Lr[a] += mult * (f[a] + delta * bgv) * c.f;
@@ -658,7 +728,7 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
       } else if (form == PMX_FORM_MASS_SUPG) {
         // MatrixMRhsAssemble_SUPG (TCD3D.C:74-126): the Galerkin mass matrix and
         // the rhs f (phi + tau b.grad phi), tau from the unscaled coefficients
-        const double dl = sdfem_delta(prm, hK, c.eps, c.b[0], c.b[1], c.b[2]);
+        const double dl = sdfem_delta(prm, hd(c.b[0], c.b[1], c.b[2]), c.eps, c.b[0], c.b[1], c.b[2]);
         for (int a = 0; a < nb; a++) {
           const double bgv = c.b[0] * gx[a] + c.b[1] * gy[a] + c.b[2] * gz[a];
           Lr[a] += mult * (f[a] + dl * bgv) * c.f;
@@ -679,7 +749,7 @@ static int assemble(const Space &S, const double *prm, double *E, double *rhs, d
         // coefficients and the max-norm of b
         const double val0 = th1 * tau;
         const double c00 = val0 * c.eps, c11 = val0 * c.b[0], c22 = val0 * c.b[1], c33 = val0 * c.b[2];
-        double dl = sdfem_delta(prm, hK, c00, c11, c22, c33);
+        double dl = sdfem_delta(prm, hd(c11, c22, c33), c00, c11, c22, c33);
         dl *= val0;
         for (int a = 0; a < nb; a++) {
           const double bgv = dl * (c.b[0] * gx[a] + c.b[1] * gy[a] + c.b[2] * gz[a]);

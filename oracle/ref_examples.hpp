@@ -120,6 +120,7 @@ static void set_assembly_params() {
   TDatabase::ParamDB->INTERNAL_POLYNOMIAL_DEGREE = (int)g_prm[PMX_PRM_POLYDEG];
   TDatabase::TimeDB->THETA1 = g_prm[PMX_PRM_THETA1];
   TDatabase::TimeDB->CURRENTTIMESTEPLENGTH = g_prm[PMX_PRM_TAU];
+  TDatabase::ParamDB->CELL_MEASURE = (int)g_prm[PMX_PRM_CELL_MEASURE];
 }
 
 // the discrete form of the block's form id; nsq = its number of matrices

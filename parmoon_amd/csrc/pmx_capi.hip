@@ -1483,11 +1483,13 @@ static void host_qp_records(const Level &lv, const double *prm, int form, bool l
         std::fill(cbuf.begin(), cbuf.end(), 0.0);
         cf(nq, X.data(), Yc.data(), Z.data(), pptr.data(), cptr.data());
         const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        QpCoeffs k0;
         for (int q = 0; q < nq; q++) {
           const double *c = cptr[q];
           QpCoeffs k;
           k.eps = c[0]; k.b0 = c[1]; k.b1 = c[2]; k.b2 = c[3]; k.c = c[4]; k.f = c[5];
-          qp_record(prm, form, h, qref[4 * q + 3], k, &out[((size_t)cell * nq + q) * 8]);
+          if (q == 0) k0 = k;  // the cell's first point (CELL_MEASURE 4)
+          qp_record(prm, form, h, qref[4 * q + 3], k, &out[((size_t)cell * nq + q) * 8], &k0);
         }
       }
 }

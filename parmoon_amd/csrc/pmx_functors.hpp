@@ -169,26 +169,57 @@ __host__ __device__ inline double rhs_supg_tau(double hK, double eps, double b1,
   return tau * hK;
 }
 
+// hK of a brick cell of side h for CELL_MEASURE (Assemble3D.C:314-336):
+// the diameter sqrt(3) h (0 and every id without its own case), the cube root
+// of the volume (1, 3) or the shortest edge (2), both h.
+__host__ __device__ inline double cell_measure_hk(const double *prm, double h) {
+  const int m = (int)prm[PMX_PRM_CELL_MEASURE];
+  return (m == 1 || m == 2 || m == 3) ? h : sqrt(3.0) * h;
+}
+// Mesh_size_in_convection_direction_without_storing (ConvDiff.C:385-485) for
+// an axis-parallel brick of side h: the gradients of the 8 trilinear basis
+// functions at the barycentre are (sx, sy, sz) / (4h), sx, sy, sz = +-1, so
+// den = sum |b . grad phi_i| and the size is 2 |b| / den (hK when den < 1e-10).
+__host__ __device__ inline double hk_convection(double h, double hK, double b1, double b2, double b3) {
+  double den = 0;
+  for (int i = 0; i < 8; i++) {
+    const double sx = (i & 1) ? 1.0 : -1.0, sy = (i & 2) ? 1.0 : -1.0, sz = (i & 4) ? 1.0 : -1.0;
+    den += fabs(sx * b1 + sy * b2 + sz * b3);
+  }
+  den /= 4 * h;
+  if (den < 1e-10) return hK;
+  return 2 * sqrt(b1 * b1 + b2 * b2 + b3 * b3) / den;
+}
+
 // The quadrature-point record of the assembly, {w |detJ|, eps, b0, b1, b2, c,
 // f, delta}, for a point of a brick cell of side h with rule weight w and the
-// coefficients c at the point: |detJ| = h^3 / 8 (THexaAffin), hK = sqrt(3) h
-// (THexahedron::GetDiameter of a cube), delta the streamline parameter of
-// the form (0 for the forms without one).
+// coefficients c at the point: |detJ| = h^3 / 8 (THexaAffin), hK from
+// CELL_MEASURE, delta the streamline parameter of the form (0 for the forms
+// without one).  c0: the coefficients at the cell's first quadrature point;
+// with CELL_MEASURE 4 the reference computes the mesh size in the convection
+// direction at the cell's first call of Compute_SDFEM_delta (its point 0)
+// and keeps it for the cell (INTERNAL_HK_CONVECTION, ConvDiff.C:373-383,
+// reset per cell in Assemble3D.C:389-405).
 __host__ __device__ inline void qp_record(const double *prm, int form, double h, double w, const QpCoeffs &c,
-                                          double *o) {
-  const double hK = sqrt(3.0) * h;
+                                          double *o, const QpCoeffs *c0 = nullptr) {
+  const double hK = cell_measure_hk(prm, h);
+  const bool conv = (int)prm[PMX_PRM_CELL_MEASURE] == 4;
+  const QpCoeffs &f = c0 ? *c0 : c;
   double delta = 0;
   if (form == PMX_FORM_SDFEM || form == PMX_FORM_MASS_SUPG) {
     // BilinearAssemble_SD and MatrixMRhsAssemble_SUPG (TCD3D.C:103-104):
     // Compute_SDFEM_delta of the unscaled coefficients
-    delta = sdfem_delta(prm, hK, c.eps, c.b0, c.b1, c.b2);
+    const double hd = conv ? hk_convection(h, hK, f.b0, f.b1, f.b2) : hK;
+    delta = sdfem_delta(prm, hd, c.eps, c.b0, c.b1, c.b2);
   } else if (form == PMX_FORM_RHS_SUPG) {
+    // its own parameter with the cell measure (no Compute_SDFEM_delta)
     delta = rhs_supg_tau(hK, c.eps, c.b0, c.b1, c.b2);
   } else if (form == PMX_FORM_TCD_SUPG_AK) {
     // MatricesAKRhsAssemble_SUPG (TCD3D.C:167-181, 206-208): delta of the
     // theta1 tau scaled coefficients, times theta1 tau
     const double v = prm[PMX_PRM_THETA1] * prm[PMX_PRM_TAU];
-    delta = sdfem_delta(prm, hK, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
+    const double hd = conv ? hk_convection(h, hK, v * f.b0, v * f.b1, v * f.b2) : hK;
+    delta = sdfem_delta(prm, hd, v * c.eps, v * c.b0, v * c.b1, v * c.b2) * v;
   }
   o[0] = w * (h * h * h / 8.0);
   o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = delta;

@@ -515,7 +515,15 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const double X = (cx * h + 0.5 * h) + 0.5 * h * xi;
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
-  qp_record(prm, form, h, w, example_coeffs(prm, X, Y, Z), o);
+  const QpCoeffs c = example_coeffs(prm, X, Y, Z);
+  if (q != 0 && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
+    // the coefficients at the cell's first point (mesh size in the convection direction)
+    const QpCoeffs c0 = example_coeffs(prm, (cx * h + 0.5 * h) + 0.5 * h * qref[0],
+                                       (cy * h + 0.5 * h) + 0.5 * h * qref[1], (cz * h + 0.5 * h) + 0.5 * h * qref[2]);
+    qp_record(prm, form, h, w, c, o, &c0);
+  } else {
+    qp_record(prm, form, h, w, c, o);
+  }
 }
 // Stage 1 (coefficient functions that are expensive to evaluate): the
 // coefficients of every quadrature point of every cell, once.
@@ -557,22 +565,11 @@ __global__ __launch_bounds__(kBlock) void k_fourier_tables(const double *__restr
   tab[2 * id] = cos(a);
   tab[2 * id + 1] = sin(a);
 }
-// Stage 1 for the curl-Fourier example: the records of k_qp_data with b
-// from the table above (eps, c = 0 and f = 1 as example_coeffs).
-// Grid: x over the points of one line of cells (cx, q), y = cy + n cz; the
-// rule size is a template parameter so that no index needs a runtime division.
-template <int NQ>
-__global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__restrict__ prm,
-                                                            const double *__restrict__ qref, int n,
-                                                            const double *__restrict__ tab, double *__restrict__ out) {
-  constexpr int np = NQ == 8 ? 2 : (NQ == 27 ? 3 : 4);
-  static_assert(np * np * np == NQ, "tensor rule");
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n * NQ) return;
-  const int cx = t / NQ, q = t % NQ;
-  const int cy = blockIdx.y % n, cz = blockIdx.y / n;
-  const int64_t id = ((int64_t)blockIdx.y * n + cx) * NQ + q;
-  const int ix = q % np, iy = (q / np) % np, iz = q / (np * np);
+// The curl-Fourier coefficients (eps, b, c = 0, f = 1 as example_coeffs) at
+// the 1D points (ix, iy, iz) of cell (cx, cy, cz) from the table above.
+__device__ __forceinline__ QpCoeffs fourier_coeffs_tab(const double *__restrict__ prm,
+                                                       const double *__restrict__ tab, int n, int np, int cx, int cy,
+                                                       int cz, int ix, int iy, int iz) {
   const double twopi = 6.283185307179586476925286766559;
   const int64_t L = (int64_t)n * np;  // entries per (mode, axis)
   double dA[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
@@ -597,7 +594,32 @@ __global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__rest
   k.b2 = sc * (dA[1][0] - dA[0][1]);
   k.c = 0;
   k.f = 1.0;
-  qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8);
+  return k;
+}
+// Stage 1 for the curl-Fourier example: the records of k_qp_data with the
+// coefficients from the table above.
+// Grid: x over the points of one line of cells (cx, q), y = cy + n cz; the
+// rule size is a template parameter so that no index needs a runtime division.
+template <int NQ>
+__global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__restrict__ prm,
+                                                            const double *__restrict__ qref, int n,
+                                                            const double *__restrict__ tab, double *__restrict__ out) {
+  constexpr int np = NQ == 8 ? 2 : (NQ == 27 ? 3 : 4);
+  static_assert(np * np * np == NQ, "tensor rule");
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * NQ) return;
+  const int cx = t / NQ, q = t % NQ;
+  const int cy = blockIdx.y % n, cz = blockIdx.y / n;
+  const int64_t id = ((int64_t)blockIdx.y * n + cx) * NQ + q;
+  const int ix = q % np, iy = (q / np) % np, iz = q / (np * np);
+  const QpCoeffs k = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, ix, iy, iz);
+  if (q != 0 && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
+    // the coefficients at the cell's first point (mesh size in the convection direction)
+    const QpCoeffs k0 = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, 0, 0, 0);
+    qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8, &k0);
+  } else {
+    qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8);
+  }
 }
 // examples whose coefficients are cheaper to evaluate again than to read back
 __host__ __device__ inline bool coeffs_inline(int example) {

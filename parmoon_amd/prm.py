@@ -10,7 +10,7 @@ import numpy as np
 PRM_LEN = 512
 EXAMPLE, EPS, B0, B1, B2, C, TIME, FORM = 0, 1, 2, 3, 4, 5, 6, 7
 SDFEM_TYPE, DELTA0, DELTA1, TAU, THETA1, LINEAR, BSCALE, POLYDEG = 8, 9, 10, 11, 12, 13, 14, 15
-FOURIER, GRF, GRF_SIGMA, BLOB = 16, 136, 392, 393
+FOURIER, GRF, GRF_SIGMA, BLOB, CELL_MEASURE = 16, 136, 392, 393, 397
 
 EX_LAPLACE, EX_SINE_CD, EX_CURL_FOURIER, EX_HEAT_AMC, EX_LOGNORMAL, EX_ROTATING = range(6)
 FORM_GALERKIN, FORM_SDFEM, FORM_MASS, FORM_TCD_A, FORM_TCD_SUPG_AK = 1, 2, 3, 4, 5
@@ -18,7 +18,7 @@ FORM_MASS_SUPG, FORM_RHS_SUPG, FORM_RHS = 6, 7, 8
 
 
 def base(example, eps=1.0, form=FORM_GALERKIN, b=(0.0, 0.0, 0.0), c=0.0, time=0.0,
-         sdfem_type=2, delta0=1.0, delta1=1.0, linear=False):
+         sdfem_type=2, delta0=1.0, delta1=1.0, linear=False, cell_measure=0):
     """A parameter block with ParMooN's defaults (Database.C:471-483, 895, 914)."""
     p = np.zeros(PRM_LEN)
     p[EXAMPLE] = example
@@ -35,6 +35,7 @@ def base(example, eps=1.0, form=FORM_GALERKIN, b=(0.0, 0.0, 0.0), c=0.0, time=0.
     p[THETA1] = 1.0
     p[TAU] = 1.0
     p[BSCALE] = 1.0
+    p[CELL_MEASURE] = cell_measure
     return p
 
 

@@ -43,6 +43,19 @@ def cases():
         out.append((f"q{order}_n{nref}_sine_sdfem_type0", nref, order, p))
         p = P.config2(nref=nref, form=P.FORM_SDFEM, linear=True)
         out.append((f"q{order}_n{nref}_curl_sdfem_linear", nref, order, p))
+        # CELL_MEASURE: shortest edge, cube root of the measure, and the mesh
+        # size in the convection direction (per cell, at its first point)
+        for cm in (2, 3, 4):
+            p = P.config2(nref=nref, form=P.FORM_SDFEM, linear=False)
+            p[P.CELL_MEASURE] = cm
+            out.append((f"q{order}_n{nref}_curl_sdfem_cm{cm}", nref, order, p))
+        for form, fname in ((P.FORM_TCD_SUPG_AK, "supgAK"), (P.FORM_MASS_SUPG, "mass_supg"),
+                            (P.FORM_RHS_SUPG, "rhs_supg")):
+            p = P.config2(nref=nref, form=form, linear=False)
+            p[P.TAU], p[P.THETA1], p[P.CELL_MEASURE] = 0.02, 0.5, 4
+            out.append((f"q{order}_n{nref}_curl_{fname}_cm4", nref, order, p))
+        p = P.base(P.EX_SINE_CD, eps=1e-3, form=P.FORM_SDFEM, b=b, c=0.3, delta0=0.7, cell_measure=4)
+        out.append((f"q{order}_n{nref}_sine_sdfem_cm4", nref, order, p))
     return out
 
 

@@ -539,3 +539,36 @@ def test_config1_as_specified_stagnates():
     s, info = O.solve(opt, rhs, sol0)
     r = fine.defect(O.entries(), s, rhs)[1] / r0
     assert 0.99 < r < 1.0, r
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_cell_measure_closed_forms(order):
+    """CELL_MEASURE on the cube's bricks of side h: the shortest edge and the
+    cube root of the volume are h (the diameter sqrt(3) h is the default),
+    and the mesh size in the convection direction is h for b along an axis
+    (the gradients of the eight vertex functions at the barycentre are
+    (+-1, +-1, +-1) / (4h), so 2|b| / sum |b . grad phi| = 8h|b| / (8|b|))
+    and 2h / sqrt(3) for b along a diagonal (sum = 12 |b| / sqrt(3) / (4h)).
+    Checked through the SD matrix: delta(h_K) of type 0 is delta0 h_K / |b|
+    when eps < h_K |b|, linear in h_K, so the streamline part scales with it."""
+    sp = orc.Space(2, order)
+    h = 0.25
+
+    def sd(b, cm, delta0):
+        p = P.base(P.EX_SINE_CD, eps=1e-8, form=P.FORM_SDFEM, b=b, sdfem_type=0, delta0=delta0, cell_measure=cm)
+        return sp.assemble(p)[0]
+
+    for b in ((1.0, 0.0, 0.0), (0.0, 0.0, 2.0)):
+        ref = sd(b, 2, 1.0)                       # h_K = h
+        for cm in (1, 3, 4):
+            assert np.max(np.abs(sd(b, cm, 1.0) - ref)) <= 1e-14 * np.max(np.abs(ref)), (b, cm)
+        # the default (diameter): the streamline part is sqrt(3) times larger
+        gal = sd(b, 2, 0.0)
+        d0 = sd(b, 0, 1.0) - gal
+        d2 = ref - gal
+        assert np.max(np.abs(d0 - np.sqrt(3.0) * d2)) <= 1e-12 * np.max(np.abs(d0))
+    b = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    gal = sd(b, 2, 0.0)
+    d4 = sd(b, 4, 1.0) - gal
+    d2 = sd(b, 2, 1.0) - gal
+    assert np.max(np.abs(d4 - 2.0 / np.sqrt(3.0) * d2)) <= 1e-12 * np.max(np.abs(d4)), h
