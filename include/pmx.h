@@ -115,6 +115,15 @@ int pmx_system_assemble_cb(pmx_system *sys, const double *prm, pmx_coeff_fct *co
  * row-scaled FP16 smoother values of opt[15] = 3 / 4 (a_ij / a_ii, Dirichlet
  * rows 1), both as doubles. */
 int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries);
+/* TMGLevel3D(level, A, ...) with the caller's matrix (include/FE/MGLevel3D.h:
+ * 142, added by TMultiGrid3D::AddLevel, include/FE/MultiGrid3D.h:94): the
+ * operator of a level is replaced by the caller's entries (CSR of the
+ * level's own pattern, reference numbering, n = the level's DOFs; Dirichlet
+ * rows must be identity rows).  The system must be assembled; the MG cycles
+ * of the following solves use these values (the FP32 copies and the coarse
+ * inverse follow).  A pattern other than the level's is rejected. */
+int pmx_level_set_entries(pmx_system *sys, int level, int n, const int *rowptr, const int *kcol,
+                          const double *entries);
 /* right-hand side of a level (Rhs_array[level]) */
 int pmx_system_rhs(pmx_system *sys, int level, double *rhs);
 

@@ -13,6 +13,9 @@
 //   TAssembleMat3D     Init, Reset, Assemble3D, DeAllocate (include/System/AssembleMat3D.h)
 //   TFEFunction3D      GetName / GetValues / GetLength    (include/FE/FEFunction3D.h)
 //   TOutput3D          AddFEFunction, WriteVtk            (include/FE/Output3D.h)
+//   TSquareMatrix3D    GetEntries / Reset                 (include/FE/SquareMatrix3D.h)
+//   TMGLevel3D, TMultiGrid3D::AddLevel, TItMethod::Iterate (include/FE/MGLevel3D.h,
+//                      MultiGrid3D.h, ItMethod.h)
 //
 // Differences: parameters come from an explicit block (pmx_prm.h) and an
 // option vector (pmx.h) instead of TDatabase; errors throw pmx::Error
@@ -21,6 +24,7 @@
 #ifndef PMX_PARMOON_HPP
 #define PMX_PARMOON_HPP
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -212,6 +216,101 @@ class TSystemCD3D {
   double *sol_, *rhs_;
   int nlev_, n_dof_ = 0, n_active_ = 0;
   std::vector<double> prm_;
+};
+
+// A square matrix on a structure with caller-owned entries
+// (TSquareMatrix3D(structure), include/FE/SquareMatrix3D.h; GetEntries).
+class TSquareMatrix3D {
+ public:
+  explicit TSquareMatrix3D(TSquareStructure3D &structure)
+      : st_(structure), entries_((size_t)structure.GetN_Entries(), 0.0) {}
+  TSquareStructure3D &GetStructure() { return st_; }
+  int GetN_Rows() const { return st_.GetN_Rows(); }
+  int GetActiveBound() const { return st_.GetActiveBound(); }
+  int GetN_Entries() const { return st_.GetN_Entries(); }
+  const int *GetRowPtr() { return st_.GetRowPtr(); }
+  const int *GetKCol() { return st_.GetKCol(); }
+  double *GetEntries() { return entries_.data(); }
+  void Reset() { std::fill(entries_.begin(), entries_.end(), 0.0); }
+
+ private:
+  TSquareStructure3D &st_;
+  std::vector<double> entries_;
+};
+
+// A multigrid level with the caller's matrix (TMGLevel3D(level, A, rhs, sol,
+// n_aux, permutation), include/FE/MGLevel3D.h:142).  rhs and sol are the
+// level's work arrays in the reference; the device keeps its own.
+class TMGLevel3D {
+ public:
+  TMGLevel3D(int level, TSquareMatrix3D *A, double *rhs, double *sol, int n_aux, int *permutation)
+      : level_(level), A_(A) {
+    (void)rhs; (void)sol; (void)n_aux;
+    if (permutation) throw Error(PMX_ERR_ARG, "TMGLevel3D: permuted levels are not supported");
+  }
+  int GetLevel() const { return level_; }
+  TSquareMatrix3D *GetMatrix() const { return A_; }
+  int GetN_DOF() const { return A_->GetN_Rows(); }
+
+ private:
+  int level_;
+  TSquareMatrix3D *A_;
+};
+
+// TMultiGrid3D (include/FE/MultiGrid3D.h:87-110): AddLevel hands the level's
+// matrix to the device hierarchy of a system (pmx_level_set_entries; the
+// reference's constructor takes no system, here it is the first argument).
+// The cycle's options (smoother, sweeps, damping, cycle type) are the
+// solver option vector of the solve, as the reference reads them from
+// ParamDB in Cycle; Parameters are the smoother's damping factors.
+class TMultiGrid3D {
+ public:
+  TMultiGrid3D(pmx_system *system, int n_problems, int n_parameters, const double *parameters)
+      : h_(system), params_(parameters, parameters + (parameters ? n_parameters : 0)) {
+    (void)n_problems;
+  }
+  // returns the number of levels added (level numbers are the system's,
+  // coarsest 0)
+  int AddLevel(TMGLevel3D *lv) {
+    TSquareMatrix3D *A = lv->GetMatrix();
+    check(pmx_level_set_entries(h_, lv->GetLevel(), A->GetN_Rows(), A->GetRowPtr(), A->GetKCol(), A->GetEntries()),
+          "pmx_level_set_entries");
+    levels_.push_back(lv);
+    return (int)levels_.size();
+  }
+  int GetN_Levels() const { return (int)levels_.size(); }
+  TMGLevel3D *GetLevel(int i) const { return levels_.at(i); }
+  const std::vector<double> &GetParameters() const { return params_; }
+  pmx_system *handle() const { return h_; }
+
+ private:
+  pmx_system *h_;
+  std::vector<double> params_;
+  std::vector<TMGLevel3D *> levels_;
+};
+
+// TFgmresIte / TFixedPointIte / TBcgs preconditioned by TMultiGridScaIte on
+// a TMultiGrid3D (include/FE/ItMethod.h:105): Iterate(A, B, sol, rhs) on the
+// caller's finest matrix (pmx_iterate); the solver id and its parameters are
+// opt[0..4] (16 FGMRES, 11 Richardson, 13 BiCGStab).  Returns the iterations.
+class TItMethod {
+ public:
+  TItMethod(TMultiGrid3D &mg, const double *opt) : mg_(mg), opt_(opt, opt + 24) {}
+  int Iterate(TSquareMatrix3D **sqmat, void *mat, double *sol, double *rhs, double *info = nullptr) {
+    (void)mat;
+    TSquareMatrix3D *A = sqmat[0];
+    double inf[8];
+    const int rc = pmx_iterate(mg_.handle(), A->GetN_Rows(), A->GetRowPtr(), A->GetKCol(), A->GetEntries(),
+                               opt_.data(), rhs, sol, inf);
+    if (info)
+      for (int i = 0; i < 8; i++) info[i] = inf[i];
+    check(rc, "pmx_iterate");
+    return (int)inf[0];
+  }
+
+ private:
+  TMultiGrid3D &mg_;
+  std::vector<double> opt_;
 };
 
 // TAssembleMat3D (include/System/AssembleMat3D.h:53-91, Init / Reset /

@@ -66,6 +66,7 @@ def lib():
         L.pmx_csr_matvec.argtypes = [vp, ip, vp, vp, vp, vp, vp]
         L.pmx_system_solve.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_iterate.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp, vp]
+        L.pmx_level_set_entries.argtypes = [vp, ip, ip, vp, vp, vp]
         L.pmx_system_assemble_cb.argtypes = [vp, vp, vp, vp, vp]
         L.pmx_system_solve_resident.argtypes = [vp, vp, vp]
         L.pmx_system_set_step_length_control.argtypes = [vp, ctypes.c_int]
@@ -220,6 +221,20 @@ class System:
         p = np.ascontiguousarray(prm_block, float)
         assert p.size == PRM_LEN
         _ok(lib().pmx_system_assemble(self.h, _p(p)), "pmx_system_assemble")
+
+    def set_entries(self, entries, level=-1, rowptr=None, kcol=None):
+        """The caller's matrix (CSR of the level's pattern, reference
+        numbering; by default the level's own rowptr / kcol) as the level's
+        operator (pmx_level_set_entries)."""
+        if rowptr is None:
+            rowptr, kcol = self.structure(level)
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        kc = np.ascontiguousarray(kcol, np.int32)
+        e = np.ascontiguousarray(entries, np.float64)
+        if e.size != kc.size or rp[-1] != kc.size:
+            raise ValueError("entries, rowptr and kcol do not fit together")
+        _ok(lib().pmx_level_set_entries(self.h, level, int(rp.size - 1), _p(rp), _p(kc), _p(e)),
+            "pmx_level_set_entries")
 
     def entries(self, level=-1, which=0):
         e = np.zeros(int(self.dims(level)[3]))

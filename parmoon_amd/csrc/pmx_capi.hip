@@ -100,6 +100,7 @@ struct Level {
   DevBuf<_Float16> val16;
   DevBuf<float> sdinv;
   bool have_val16 = false;
+  DevBuf<uint8_t> corner;  // per-cell frame codes (Numbering::corner), CELL_MEASURE 4
 };
 
 struct System {
@@ -110,6 +111,7 @@ struct System {
   bool two_matrices = false;
   DevBuf<double> prm, qref, tab, qpbuf;
   int64_t qp_tab_off = 0;  // offset of the curl-Fourier phase tables in qpbuf
+  int cell_measure = 0;    // prm[PMX_PRM_CELL_MEASURE] of the last assembly
   int nq = 0;
   int tab_key = -1;                   // rule of the uploaded tables
   int example = 0;                    // prm[PMX_PRM_EXAMPLE] of the last assembly
@@ -1465,6 +1467,7 @@ static void host_qp_records(const Level &lv, const double *prm, int form, bool l
   make_tables(lv.L.order, linear, qref, tab, nq);
   const int n = lv.L.n;
   const double h = 1.0 / n;
+  const int np = nq <= 8 ? 2 : (nq <= 27 ? 3 : 4);  // points of the 1D rule
   out.assign((size_t)n * n * n * nq * 8, 0.0);
   std::vector<double> X(nq), Yc(nq), Z(nq), cbuf((size_t)nq * 20), pbuf((size_t)nq * 20, 0.0);
   std::vector<double *> cptr(nq), pptr(nq);
@@ -1483,15 +1486,30 @@ static void host_qp_records(const Level &lv, const double *prm, int form, bool l
         std::fill(cbuf.begin(), cbuf.end(), 0.0);
         cf(nq, X.data(), Yc.data(), Z.data(), pptr.data(), cptr.data());
         const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        // the reference's first point of the cell, (a, a, a) in its own frame:
+        // 1D point 0 (a) along an axis where the frame points forward, np - 1
+        // (-a) where it points back (CELL_MEASURE 4)
+        const int code = lv.Nb.corner[cell];
+        const int q0 = ((code & 1) ? 0 : np - 1) + np * (((code & 2) ? 0 : np - 1) + np * ((code & 4) ? 0 : np - 1));
         QpCoeffs k0;
+        {
+          const double *c = cptr[q0];
+          k0.eps = c[0]; k0.b0 = c[1]; k0.b1 = c[2]; k0.b2 = c[3]; k0.c = c[4]; k0.f = c[5];
+        }
         for (int q = 0; q < nq; q++) {
           const double *c = cptr[q];
           QpCoeffs k;
           k.eps = c[0]; k.b0 = c[1]; k.b1 = c[2]; k.b2 = c[3]; k.c = c[4]; k.f = c[5];
-          if (q == 0) k0 = k;  // the cell's first point (CELL_MEASURE 4)
           qp_record(prm, form, h, qref[4 * q + 3], k, &out[((size_t)cell * nq + q) * 8], &k0);
         }
       }
+}
+// the device copy of a level's per-cell frame codes when the assembly needs
+// them (CELL_MEASURE 4: the cell's first quadrature point), else nullptr
+static const uint8_t *cell_corners(System &Y, Level &lv) {
+  if (Y.cell_measure != 4) return nullptr;
+  if (!lv.corner.p) lv.corner.upload(lv.Nb.corner.data(), lv.Nb.corner.size(), Y.ctx->st);
+  return lv.corner.p;
 }
 // TSystemCD3D::Assemble of every level with the device form of prm and the
 // built-in coefficient functor of prm, or the caller's callbacks (cb)
@@ -1502,6 +1520,7 @@ static void system_assemble(System &Y, const double *prm, const HostCallbacks *c
   const bool linear = prm[PMX_PRM_LINEAR] != 0;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
   Y.example = (int)prm[PMX_PRM_EXAMPLE];
+  Y.cell_measure = (int)prm[PMX_PRM_CELL_MEASURE];
   ensure_tables(Y, linear);
   const int nq = Y.nq;
   const bool rhs_only = form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG;
@@ -1523,11 +1542,12 @@ static void system_assemble(System &Y, const double *prm, const HostCallbacks *c
     if (rhs_only) {
       // Assemble3D with no matrices: only the right-hand side
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
-               lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off);
+               lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off, cell_corners(Y, lv));
     } else {
       if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
-               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off);
+               Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off,
+               cell_corners(Y, lv));
       lv.have_val32 = false;
       lv.have_val16 = false;
       lv.have_dinv = false;
@@ -1594,6 +1614,7 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
   hipStream_t st = Y.ctx->st;
   Y.prm.upload(prm, PMX_PRM_LEN, st);
   Y.example = (int)prm[PMX_PRM_EXAMPLE];
+  Y.cell_measure = (int)prm[PMX_PRM_CELL_MEASURE];
   Y.form = (int)prm[PMX_PRM_FORM];
   Y.two_matrices = !mass && Y.form == PMX_FORM_TCD_SUPG_AK;
   ensure_tables(Y, prm[PMX_PRM_LINEAR] != 0);
@@ -1605,7 +1626,7 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
     if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
     assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
              mass ? lv.mass.p : lv.stiff.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false,
-             Y.qpbuf.p + Y.qp_tab_off);
+             Y.qpbuf.p + Y.qp_tab_off, cell_corners(Y, lv));
   }
 }
 
@@ -1763,6 +1784,34 @@ int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
     }
     check(hipMemcpyAsync(entries, out.p, sizeof(double) * out.n, hipMemcpyDeviceToHost, Y.ctx->st), "d2h");
     check(hipStreamSynchronize(Y.ctx->st), "sync");
+  })
+}
+int pmx_level_set_entries(pmx_system *sys, int level, int n, const int *rowptr, const int *kcol,
+                          const double *entries) {
+  PMX_TRY({
+    System &Y = sys->s;
+    if (!Y.assembled) throw std::logic_error("assemble first");
+    if (!rowptr || !kcol || !entries) throw std::invalid_argument("NULL argument");
+    Level &lv = level_of(Y, level);
+    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
+    if (n != (int)lv.L.N) throw std::invalid_argument("n differs from the level's DOFs");
+    // the pattern must be the level's own (TSquareStructure3D of its space)
+    if (std::memcmp(rowptr, lv.Nb.rowptr.data(), sizeof(int) * (n + 1)) != 0 ||
+        std::memcmp(kcol, lv.Nb.kcol.data(), sizeof(int) * lv.Nb.kcol.size()) != 0)
+      throw std::invalid_argument("the matrix pattern differs from the level's structure");
+    // Dirichlet rows (the last N - active) are identity rows
+    for (int64_t r = lv.L.nactive; r < n; r++)
+      if (entries[rowptr[r]] != 1.0) throw std::invalid_argument("a Dirichlet row is not the identity row");
+    ensure_csr_dev(Y, lv);
+    hipStream_t st = Y.ctx->st;
+    DevBuf<double> in;
+    in.upload(entries, lv.Nb.kcol.size(), st);
+    csr_to_css(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, in.p, lv.v64.val.p, st);
+    lv.have_val32 = false;
+    lv.have_val16 = false;
+    lv.have_dinv = false;
+    if (&lv == Y.lev.front().get()) coarse_inverse(Y, lv);
+    check(hipStreamSynchronize(st), "set entries");
   })
 }
 int pmx_system_rhs(pmx_system *sys, int level, double *rhs) {
@@ -2014,7 +2063,7 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
           case 1: css_apply<double>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st); break;
           case 2: level_smooth<double>(Y, lv, 1, 0.8, 1); break;
           case 3: level_smooth<double>(Y, lv, 3, 1.0, 1); break;
-          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false, Y.qpbuf.p + Y.qp_tab_off); break;
+          case 4: assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, Y.nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), v.val.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false, Y.qpbuf.p + Y.qp_tab_off, cell_corners(Y, lv)); break;
           case 5: if (li > 0) prolongate<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.x.p, v.aux.p, st); break;
           case 6: if (li > 0) restrict_<double>(Y.lev[li - 1]->D, lv.D, Y.lev[li - 1]->v64.f.p, v.d.p, st); break;
           case 7: mg_prec<double>(Y, v.aux.p, v.d.p); break;

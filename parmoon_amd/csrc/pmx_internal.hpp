@@ -88,6 +88,10 @@ struct Numbering {
   std::vector<int> lat2dof, dof2lat;
   std::vector<int> rowptr, kcol;                 // built on demand
   bool have_csr = false;
+  // per lattice cell (cx + n (cy + n cz)): the signs of e1 + e2 + e3 of the
+  // cell's frame in the refinement tree (bit d set: +1 along axis d), i.e. the
+  // direction from the cell centre to its reference point (a, a, a)
+  std::vector<uint8_t> corner;
 };
 
 void build_layout(Layout &L, int nref, int order, int block);

@@ -151,7 +151,18 @@ void build_numbering(Numbering &Nb, const Layout &L) {
   std::vector<uint8_t> isdir(L.N, 0);
   int64_t cell = 0;
   int loc[27];
+  Nb.corner.assign((size_t)L.n * L.n * L.n, 0);
   walk_leaves(L.n, [&](const Frame &f) {
+    {
+      int lo[3];
+      uint8_t code = 0;
+      for (int d = 0; d < 3; d++) {
+        const int s = f.e[0][d] + f.e[1][d] + f.e[2][d];
+        lo[d] = f.o[d] + std::min(f.e[0][d], 0) + std::min(f.e[1][d], 0) + std::min(f.e[2][d], 0);
+        if (s > 0) code |= (uint8_t)(1 << d);
+      }
+      Nb.corner[lo[0] + (size_t)L.n * (lo[1] + (size_t)L.n * lo[2])] = code;
+    }
     // local DOF (a,b,c), x fastest, sits at p*origin + a e1 + b e2 + c e3
     int j = 0;
     for (int c = 0; c < k; c++)

@@ -508,7 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T
 // Coefficients at quadrature point q of cell (cx, cy, cz), in the canonical
 // (axis-aligned) frame of the lattice: {w|detJ|, eps, b0, b1, b2, c, f, delta}.
 __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const double *__restrict__ qref, int n,
-                                          int cx, int cy, int cz, int q, int form, double *o) {
+                                          int cx, int cy, int cz, int q, int form, double *o,
+                                          const uint8_t *__restrict__ corner = nullptr) {
   const double h = 1.0 / n;
   const double xi = qref[4 * q], eta = qref[4 * q + 1], zeta = qref[4 * q + 2], w = qref[4 * q + 3];
   // THexaAffin::GetOrigFromRef: X = xc0 + xc1 xi + ... with xc0 the cell centre
@@ -516,10 +517,14 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
   const QpCoeffs c = example_coeffs(prm, X, Y, Z);
-  if (q != 0 && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
-    // the coefficients at the cell's first point (mesh size in the convection direction)
-    const QpCoeffs c0 = example_coeffs(prm, (cx * h + 0.5 * h) + 0.5 * h * qref[0],
-                                       (cy * h + 0.5 * h) + 0.5 * h * qref[1], (cz * h + 0.5 * h) + 0.5 * h * qref[2]);
+  if (corner && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
+    // the coefficients at the reference's first point of the cell, (a, a, a)
+    // with a = qref[0] in the cell's own frame: centre + (h/2) a (e1 + e2 + e3)
+    const int code = corner[cx + (int64_t)n * (cy + (int64_t)n * cz)];
+    const double a = 0.5 * h * qref[0];
+    const QpCoeffs c0 = example_coeffs(prm, (cx * h + 0.5 * h) + ((code & 1) ? a : -a),
+                                       (cy * h + 0.5 * h) + ((code & 2) ? a : -a),
+                                       (cz * h + 0.5 * h) + ((code & 4) ? a : -a));
     qp_record(prm, form, h, w, c, o, &c0);
   } else {
     qp_record(prm, form, h, w, c, o);
@@ -528,14 +533,15 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
 // Stage 1 (coefficient functions that are expensive to evaluate): the
 // coefficients of every quadrature point of every cell, once.
 __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ prm, const double *__restrict__ qref,
-                                                    int nq, int n, double *__restrict__ out) {
+                                                    int nq, int n, double *__restrict__ out,
+                                                    const uint8_t *__restrict__ corner) {
   const int64_t id = blockIdx.x * (int64_t)kBlock + threadIdx.x;
   const int64_t ncell = (int64_t)n * n * n;
   if (id >= ncell * nq) return;
   const int q = id % nq;
   const int64_t cell = id / nq;
   const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
-  qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], out + id * 8);
+  qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], out + id * 8, corner);
 }
 // The curl-Fourier convection (fourier_b) without a cosine per mode and
 // point.  The quadrature rules are tensor products (point q = ix + np (iy +
@@ -603,7 +609,8 @@ __device__ __forceinline__ QpCoeffs fourier_coeffs_tab(const double *__restrict_
 template <int NQ>
 __global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__restrict__ prm,
                                                             const double *__restrict__ qref, int n,
-                                                            const double *__restrict__ tab, double *__restrict__ out) {
+                                                            const double *__restrict__ tab, double *__restrict__ out,
+                                                            const uint8_t *__restrict__ corner) {
   constexpr int np = NQ == 8 ? 2 : (NQ == 27 ? 3 : 4);
   static_assert(np * np * np == NQ, "tensor rule");
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -613,9 +620,13 @@ __global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__rest
   const int64_t id = ((int64_t)blockIdx.y * n + cx) * NQ + q;
   const int ix = q % np, iy = (q / np) % np, iz = q / (np * np);
   const QpCoeffs k = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, ix, iy, iz);
-  if (q != 0 && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
-    // the coefficients at the cell's first point (mesh size in the convection direction)
-    const QpCoeffs k0 = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, 0, 0, 0);
+  if (corner && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
+    // the coefficients at the reference's first point of the cell, (a, a, a)
+    // in its own frame: 1D point 0 (a) along an axis where the frame points
+    // forward, np - 1 (-a; the rules are symmetric) where it points back
+    const int code = corner[cx + (int64_t)n * (cy + (int64_t)n * cz)];
+    const QpCoeffs k0 = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, (code & 1) ? 0 : np - 1,
+                                           (code & 2) ? 0 : np - 1, (code & 4) ? 0 : np - 1);
     qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8, &k0);
   } else {
     qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8);
@@ -799,6 +810,27 @@ __global__ void k_css_to_csr(const int *__restrict__ rowptr, const int *__restri
     const int dx = q % M - x, dy = (q / M) % M - y, dz = q / (M * M) - z;
     const int slot = (dx + s.ex) + (2 * s.ex + 1) * ((dy + s.ey) + (2 * s.ey + 1) * (dz + s.ez));
     out[n] = (double)val[vbase(s, idx) + (int64_t)slot * kSlice];
+  }
+}
+
+// The inverse of k_css_to_csr for the active rows: the entries of a CSR
+// matrix with the level's pattern (reference numbering) into the CSS values.
+__global__ void k_csr_to_css(const int *__restrict__ rowptr, const int *__restrict__ kcol,
+                             const int *__restrict__ dof2lat, int64_t active, int M, int S,
+                             const int *__restrict__ segidx, const DevSegment *__restrict__ segs,
+                             const double *__restrict__ in, double *__restrict__ val) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= active) return;
+  const int b = rowptr[r], e = rowptr[r + 1];
+  const int p = dof2lat[r];
+  const int x = p % M, y = (p / M) % M, z = p / (M * M);
+  const DevSegment s = segs[segidx[(x % S) + S * ((y % S) + S * (z % S))]];
+  const int idx = ((x - s.x0) / s.stride) + s.cx * (((y - s.y0) / s.stride) + s.cy * ((z - s.z0) / s.stride));
+  for (int n = b; n < e; n++) {
+    const int q = dof2lat[kcol[n]];
+    const int dx = q % M - x, dy = (q / M) % M - y, dz = q / (M * M) - z;
+    const int slot = (dx + s.ex) + (2 * s.ex + 1) * ((dy + s.ey) + (2 * s.ey + 1) * (dz + s.ez));
+    val[vbase(s, idx) + (int64_t)slot * kSlice] = in[n];
   }
 }
 
@@ -1375,6 +1407,12 @@ void csr_defect(int n, const int *rp, const int *kc, const T *val, const T *x, c
   const int g = (int)(((int64_t)n * 16 + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_csr_spmv<T>, dim3(g), dim3(kBlock), 0, st, n, rp, kc, val, x, r, b);
 }
+void csr_to_css(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
+                const double *in, double *val, hipStream_t st) {
+  if (active == 0) return;
+  hipLaunchKernelGGL(k_csr_to_css, dim3((unsigned)((active + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, rowptr,
+                     kcol, dof2lat, active, D.M, D.S, D.segidx, D.segs, in, val);
+}
 template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
                 const T *val, double *out, hipStream_t st) {
@@ -1551,10 +1589,12 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_assemble_kc(const BlockInfo *
 
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
               const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
-              bool qp_given, double *ftab) {
+              bool qp_given, double *ftab, const uint8_t *corner) {
   const int64_t ncells = (int64_t)D.n * D.n * D.n;
   const int64_t ncq = ncells * nq;
-  const bool inl = !qp_given && coeffs_inline(example);
+  // with the per-cell frame codes (CELL_MEASURE 4) the coefficients go
+  // through the qp records, which carry the cell's first-point size
+  const bool inl = !qp_given && coeffs_inline(example) && !corner;
   if (!inl && !qp_given) {
     // (grid y = n^2 lines of cells: up to 255 cells per direction)
     if (example == PMX_EX_CURL_FOURIER && ftab && (int64_t)D.n * D.n <= 65535) {
@@ -1562,13 +1602,16 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
       hipLaunchKernelGGL(k_fourier_tables, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
                          prm_dev, qref, nq, D.n, ftab);
       const dim3 g((unsigned)((D.n * nq + kBlock - 1) / kBlock), (unsigned)(D.n * D.n));
-      if (nq == 8) hipLaunchKernelGGL(k_qp_data_fourier<8>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
-      else if (nq == 27) hipLaunchKernelGGL(k_qp_data_fourier<27>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
-      else if (nq == 64) hipLaunchKernelGGL(k_qp_data_fourier<64>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf);
+      if (nq == 8)
+        hipLaunchKernelGGL(k_qp_data_fourier<8>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf, corner);
+      else if (nq == 27)
+        hipLaunchKernelGGL(k_qp_data_fourier<27>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf, corner);
+      else if (nq == 64)
+        hipLaunchKernelGGL(k_qp_data_fourier<64>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf, corner);
       else throw std::logic_error("assemble: rule without the curl-Fourier kernel");
     } else {
       hipLaunchKernelGGL(k_qp_data, dim3((unsigned)((ncq + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, prm_dev,
-                         qref, nq, D.n, qpbuf);
+                         qref, nq, D.n, qpbuf, corner);
     }
   }
   if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {

@@ -138,6 +138,10 @@ void csr_spmv(int n, const int *rp, const int *kc, const T *val, const T *x, T *
 // r = b - A x on a CSR matrix
 template <typename T>
 void csr_defect(int n, const int *rp, const int *kc, const T *val, const T *x, const T *b, T *r, hipStream_t st);
+// CSR entries with the level's pattern (reference numbering) into the CSS
+// values of the active rows (the inverse of css_to_csr there)
+void csr_to_css(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
+                const double *in, double *val, hipStream_t st);
 template <typename T>
 void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const int *dof2lat, int64_t active,
                 const T *val, double *out, hipStream_t st);
@@ -146,11 +150,13 @@ void css_to_csr(const DevLayout &D, const int *rowptr, const int *kcol, const in
 // buffer of n^3 x 27 x 28 doubles for the Q2 one-matrix forms on MFMA
 // (nullptr: the per-row evaluation); qp_given: qpbuf already holds the
 // quadrature-point records (coefficients from host callbacks)
-// ftab (optional): room for fourier_table_len(n, nq) doubles; with it the
-// curl-Fourier example's coefficients come from per-level phase tables
+// ftab (optional): room for fourier_tab_len(n, nq) doubles; with it the
+// curl-Fourier example's coefficients come from per-level phase tables.
+// corner (required for CELL_MEASURE 4, else unused): the per-cell frame codes
+// of Numbering::corner on the device.
 void assemble(const DevLayout &D, const double *prm_dev, int example, int form, const double *qref, int nq,
               const double *tab, double *qpbuf, double *kc, double *val, double *val2, double *rhs, hipStream_t st,
-              bool qp_given = false, double *ftab = nullptr);
+              bool qp_given = false, double *ftab = nullptr, const uint8_t *corner = nullptr);
 // doubles of the curl-Fourier phase table of a level with n cells per
 // direction and an nq-point tensor rule (complex factors of the 3 x 8 modes
 // of PMX_PRM_FOURIER per axis, cell index and 1D point)

@@ -3,9 +3,11 @@
 //   shim_solve <dir>
 // reads <dir>/prm.bin, <dir>/opt.bin (CD3D) and <dir>/prm_tcd.bin,
 // <dir>/opt_tcd.bin, <dir>/td.bin, <dir>/u0.bin (TCD3D), and writes
-// <dir>/sol.bin, <dir>/rhs.bin, <dir>/entries.bin, <dir>/sol_tcd.bin and the
+// <dir>/sol.bin, <dir>/rhs.bin, <dir>/entries.bin, <dir>/sol_tcd.bin, the
 // entries / rhs of Laplace.h's callbacks through TAssembleMat3D and of the
-// built-in functor (<dir>/laplace_{cb,fn}_{entries,rhs}.bin).
+// built-in functor (<dir>/laplace_{cb,fn}_{entries,rhs}.bin), and the
+// solution with the caller's MG level matrices (reads
+// <dir>/mg_entries_{0,1,2}.bin and <dir>/mg_rhs.bin, writes <dir>/mg_sol.bin).
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -100,6 +102,61 @@ int main(int argc, char **argv) {
       save(dir + "/laplace_cb_rhs.bin", rhs_cb.data(), N);
       save(dir + "/laplace_fn_rhs.bin", rhs_fn.data(), N);
       std::printf("laplace callbacks assembled\n");
+    }
+    // the caller's matrices as the MG levels (TMGLevel3D + TMultiGrid3D::
+    // AddLevel) and Iterate on the caller's finest matrix: 3 levels
+    // 2^1..2^3 Q1 whose system was assembled with another form; the
+    // entries of every level and the rhs come from <dir>/mg_entries_<l>.bin,
+    // <dir>/mg_rhs.bin
+    {
+      pmx::TFESpace3D fine(3, 1);
+      const int N = fine.GetN_DegreesOfFreedom();
+      std::vector<double> sol(N, 0.0), rhs0(N, 0.0);
+      pmx::TSystemCD3D system(ctx, 3, 1, 1, sol.data(), rhs0.data());
+      std::vector<double> prm(PMX_PRM_LEN, 0.0);
+      prm[PMX_PRM_EXAMPLE] = PMX_EX_LAPLACE;
+      prm[PMX_PRM_EPS] = 1.0;
+      prm[PMX_PRM_FORM] = PMX_FORM_GALERKIN;
+      prm[PMX_PRM_POLYDEG] = 1;
+      system.Init(prm.data());
+      system.Assemble();
+      const std::vector<double> opt = load(dir + "/opt.bin", 24);
+      std::vector<double> mgp = {opt[8], opt[8]};
+      pmx::TMultiGrid3D mg(system.handle(), 1, 2, mgp.data());
+      std::vector<pmx::TFESpace3D *> spaces;
+      std::vector<pmx::TSquareStructure3D *> structures;
+      std::vector<pmx::TSquareMatrix3D *> matrices;
+      std::vector<pmx::TMGLevel3D *> levels;
+      for (int l = 0; l < 3; l++) {
+        spaces.push_back(new pmx::TFESpace3D(1 + l, 1));
+        structures.push_back(new pmx::TSquareStructure3D(*spaces.back()));
+        matrices.push_back(new pmx::TSquareMatrix3D(*structures.back()));
+        const std::vector<double> e = load(dir + "/mg_entries_" + std::to_string(l) + ".bin",
+                                           (size_t)matrices.back()->GetN_Entries());
+        std::copy(e.begin(), e.end(), matrices.back()->GetEntries());
+        levels.push_back(new pmx::TMGLevel3D(l, matrices.back(), nullptr, nullptr, 2, nullptr));
+        if (mg.AddLevel(levels.back()) != l + 1) throw pmx::Error(PMX_ERR_STATE, "AddLevel count");
+      }
+      std::vector<double> rhs = load(dir + "/mg_rhs.bin", N);
+      for (int i = fine.GetActiveBound(); i < N; i++) sol[i] = rhs[i];
+      pmx::TItMethod itm(mg, opt.data());
+      pmx::TSquareMatrix3D *sq[1] = {matrices.back()};
+      const int its = itm.Iterate(sq, nullptr, sol.data(), rhs.data());
+      save(dir + "/mg_sol.bin", sol.data(), N);
+      std::printf("caller mg levels: %d iterations\n", its);
+      // a matrix of another pattern is rejected
+      bool rejected = false;
+      try {
+        pmx::TMGLevel3D wrong(2, matrices[1], nullptr, nullptr, 2, nullptr);
+        mg.AddLevel(&wrong);
+      } catch (const pmx::Error &e) {
+        rejected = e.code() == PMX_ERR_ARG;
+      }
+      if (!rejected) throw pmx::Error(PMX_ERR_STATE, "a matrix of another level's pattern was accepted");
+      for (auto *p : levels) delete p;
+      for (auto *p : matrices) delete p;
+      for (auto *p : structures) delete p;
+      for (auto *p : spaces) delete p;
     }
     // TCD3D: 4 levels 2^1..2^4 Q1, Crank-Nicolson
     {

@@ -62,6 +62,12 @@ def test_shim_solve_and_time_steps():
     opt_t.tofile(os.path.join(d, "opt_tcd.bin"))
     np.array([tau, steps, .5, .5, .5, .5]).tofile(os.path.join(d, "td.bin"))
     u0.tofile(os.path.join(d, "u0.bin"))
+    # the caller's MG level matrices: the oracle's SUPG matrices of 2^1..2^3 Q1
+    G = orc.System(1, 3, 1)
+    G.assemble(prm)
+    for lv in range(3):
+        G.entries(lv).tofile(os.path.join(d, f"mg_entries_{lv}.bin"))
+    G.rhs().tofile(os.path.join(d, "mg_rhs.bin"))
     out = subprocess.run([build("shim_solve"), d], capture_output=True, text=True, timeout=300)
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0 and out.stdout.strip().endswith("OK")
@@ -93,6 +99,16 @@ def test_shim_solve_and_time_steps():
     scale2 = np.maximum.reduceat(np.abs(e_fn), rp2[:-1])
     assert np.max(np.abs(e_cb - e_fn) / scale2[rows2]) <= 1e-14
     assert np.max(np.abs(r_cb - r_fn)) <= 1e-14 * np.max(np.abs(r_fn))
+    # TMGLevel3D + TMultiGrid3D::AddLevel with the caller's matrices, Iterate on
+    # the caller's finest matrix: the solution of that system (the device's own
+    # operators were assembled with another form)
+    gf = G.levels[-1]
+    g0 = np.zeros(gf.N)
+    g0[gf.active:] = G.rhs()[gf.active:]
+    gref, _ = G.solve(P.solver_opt(solver=16, restart=30, maxit=300, res_norm_min=0.0, red_factor=1e-13,
+                                   smoother=3, omega=1.0), G.rhs(), g0)
+    mg_sol = np.fromfile(os.path.join(d, "mg_sol.bin"))
+    assert np.max(np.abs(mg_sol - gref)) < 1e-10 * np.max(np.abs(gref))
     # TCD3D against the oracle's time loop
     o_sol, _ = O.tcd3d(prm_t, [tau, 1e9, .5, .5, .5, .5], opt_t, u0, max_steps=steps)
     s_t = np.fromfile(os.path.join(d, "sol_tcd.bin"))

@@ -157,3 +157,44 @@ def test_tcd_phases(ctx):
         assert np.array_equal(A.solution(), B.solution()), m
     A.close()
     B.close()
+
+
+def test_caller_level_matrices(ctx):
+    """pmx_level_set_entries: the operators of every MG level replaced by the
+    caller's matrices (TMGLevel3D(level, A, ...) + TMultiGrid3D::AddLevel):
+    a system assembled with the Galerkin Laplace form, then given the
+    oracle's SUPG matrices of its levels, solves like the system assembled
+    with SUPG on the device (FP64 and FP32 forms); a matrix of another
+    pattern and a non-identity Dirichlet row are rejected."""
+    print("[case] caller level matrices", flush=True)
+    prm = P.config1b()
+    O = orc.System(1, 3, 1)
+    O.assemble(prm)
+    S = pa.System(ctx, 1, 3, 1)
+    S.assemble(P.base(P.EX_LAPLACE, eps=1.0))
+    for lv in range(3):
+        S.set_entries(O.entries(lv), lv)
+    R = pa.System(ctx, 1, 3, 1)
+    R.assemble(prm)
+    rhs = O.rhs()
+    fine = O.levels[-1]
+    s0 = np.zeros_like(rhs)
+    s0[fine.active:] = rhs[fine.active:]
+    for prec in (0, 1):
+        opt = P.solver_opt(solver=16, restart=20, maxit=200, res_norm_min=0.0, red_factor=0.0, smoother=3,
+                           omega=1.0, precision=prec, rel_target=1e-11)
+        a, ia, _ = S.solve(opt, rhs, s0)
+        b, ib, _ = R.solve(opt, rhs, s0)
+        assert int(ia[0]) == int(ib[0]) and np.max(np.abs(a - b)) <= 1e-10 * np.max(np.abs(b)), (prec, ia, ib)
+    # the exported operator is the caller's matrix
+    assert np.max(np.abs(S.entries(level=1) - O.entries(1))) == 0.0
+    rp1, kc1 = S.structure(1)
+    with pytest.raises(pa.PmxError):
+        S.set_entries(O.entries(1), level=2, rowptr=rp1, kcol=kc1)
+    bad = O.entries(2).copy()
+    rp, _ = S.structure(2)
+    bad[rp[fine.active]] = 2.0
+    with pytest.raises(pa.PmxError):
+        S.set_entries(bad, level=2)
+    S.close()
+    R.close()
