@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libpmx.so")
 ARCH = os.environ.get("PMX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pmx_mesh.cpp", "pmx_ops.hip", "pmx_capi.hip", "pmx_learned.hip"]
+SOURCES = ["pmx_mesh.cpp", "pmx_ops.hip", "pmx_capi.hip", "pmx_learned.hip", "pmx_setup.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC",

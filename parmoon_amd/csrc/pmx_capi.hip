@@ -183,8 +183,35 @@ struct System {
 static void setup_level(System &Y, Level &lv, int nref) {
   hipStream_t st = Y.ctx->st;
   build_layout(lv.L, nref, Y.order, kRowsPerBlock);
-  build_numbering(lv.Nb, lv.L);
   const Layout &L = lv.L;
+  {
+    // the DOF numbering on the device, with host copies for the host-side
+    // users (exports, CSR on demand, permutations of caller data)
+    Numbering &Nb = lv.Nb;
+    const int k = L.order + 1;
+    Nb.nb = k * k * k;
+    Nb.ncells = (int64_t)L.n * L.n * L.n;
+    Nb.N = L.N;
+    Nb.active = L.nactive;
+    Nb.begin_index.resize(Nb.ncells + 1);
+    for (int64_t i = 0; i <= Nb.ncells; i++) Nb.begin_index[i] = (int)(i * Nb.nb);
+    lv.lat2dof.alloc(L.N);
+    lv.dof2lat.alloc(L.N);
+    lv.corner.alloc(Nb.ncells);
+    DevBuf<int> gn;
+    gn.alloc(Nb.ncells * Nb.nb);
+    device_numbering(L, lv.lat2dof.p, lv.dof2lat.p, gn.p, lv.corner.p, st);
+    Nb.global_numbers.resize(Nb.ncells * Nb.nb);
+    Nb.lat2dof.resize(L.N);
+    Nb.dof2lat.resize(L.N);
+    Nb.corner.resize(Nb.ncells);
+    check(hipMemcpyAsync(Nb.global_numbers.data(), gn.p, sizeof(int) * gn.n, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipMemcpyAsync(Nb.lat2dof.data(), lv.lat2dof.p, sizeof(int) * L.N, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipMemcpyAsync(Nb.dof2lat.data(), lv.dof2lat.p, sizeof(int) * L.N, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipMemcpyAsync(Nb.corner.data(), lv.corner.p, Nb.ncells, hipMemcpyDeviceToHost, st), "d2h");
+    check(hipStreamSynchronize(st), "numbering copies");
+    Nb.have_csr = false;
+  }
   DevLayout &D = lv.D;
   D.order = L.order; D.n = L.n; D.M = L.M; D.S = L.S;
   D.N = L.N; D.nvals = L.nvals; D.nactive = L.nactive;
@@ -250,8 +277,6 @@ static void setup_level(System &Y, Level &lv, int nref) {
     D.nsegs = (int)L.segs.size();
   }
   lv.boundary.upload(L.boundary.data(), L.boundary.size(), st);
-  lv.lat2dof.upload(lv.Nb.lat2dof.data(), lv.Nb.lat2dof.size(), st);
-  lv.dof2lat.upload(lv.Nb.dof2lat.data(), lv.Nb.dof2lat.size(), st);
   D.boundary = lv.boundary.p; D.nboundary = (int)L.boundary.size();
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
