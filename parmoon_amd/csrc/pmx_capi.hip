@@ -47,6 +47,14 @@ struct DevBuf {
     check(hipMalloc(&p, sizeof(T) * m), "hipMalloc");
     n = m;
   }
+  // alloc, and a fresh allocation starts zeroed (stream-ordered): the
+  // padding rows of the last value slice of a class are never written by
+  // the assembly and stay zero
+  void alloc_zeroed(size_t m, hipStream_t st) {
+    if (m == n && p) return;
+    alloc(m);
+    if (p) check(hipMemsetAsync(p, 0, sizeof(T) * m, st), "clear");
+  }
   void upload(const T *h, size_t m, hipStream_t st) {
     alloc(m);
     check(hipMemcpyAsync(p, h, sizeof(T) * m, hipMemcpyHostToDevice, st), "upload");
@@ -284,7 +292,7 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.colour_off_dev = lv.colour_off.p;
 
   lv.rhs.alloc(L.N);
-  lv.v64.val.alloc(L.nvals);
+  lv.v64.val.alloc_zeroed(L.nvals, st);
   for (auto *b : {&lv.v64.x, &lv.v64.f, &lv.v64.d, &lv.v64.aux}) b->alloc(L.N);
 }
 
@@ -1569,7 +1577,7 @@ static void system_assemble(System &Y, const double *prm, const HostCallbacks *c
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, nullptr, nullptr, nullptr,
                lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off, cell_corners(Y, lv));
     } else {
-      if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
+      if (Y.two_matrices) lv.val2.alloc_zeroed(lv.L.nvals, st);
       assemble(lv.D, Y.prm.p, Y.example, form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y), lv.v64.val.p,
                Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, cb != nullptr, Y.qpbuf.p + Y.qp_tab_off,
                cell_corners(Y, lv));
@@ -1646,9 +1654,9 @@ static void assemble_into(System &Y, const double *prm, bool mass) {
   const int nq = Y.nq;
   for (auto &lp : Y.lev) {
     Level &lv = *lp;
-    if (mass) lv.mass.alloc(lv.L.nvals);
-    else lv.stiff.alloc(lv.L.nvals);
-    if (Y.two_matrices) lv.val2.alloc(lv.L.nvals);
+    if (mass) lv.mass.alloc_zeroed(lv.L.nvals, st);
+    else lv.stiff.alloc_zeroed(lv.L.nvals, st);
+    if (Y.two_matrices) lv.val2.alloc_zeroed(lv.L.nvals, st);
     assemble(lv.D, Y.prm.p, Y.example, Y.form, Y.qref.p, nq, Y.tab.p, Y.qpbuf.p, cell_buffer(Y),
              mass ? lv.mass.p : lv.stiff.p, Y.two_matrices ? lv.val2.p : nullptr, lv.rhs.p, st, false,
              Y.qpbuf.p + Y.qp_tab_off, cell_corners(Y, lv));

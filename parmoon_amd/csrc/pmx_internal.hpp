@@ -20,8 +20,6 @@
 // column indices are stored.  Dirichlet rows (lattice boundary) are identity
 // and not stored.
 #pragma once
-#include <hip/hip_runtime.h>
-
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -99,11 +97,6 @@ struct Numbering {
 void build_layout(Layout &L, int nref, int order, int block);
 void build_numbering(Numbering &Nb, const Layout &L);
 void build_csr(Numbering &Nb, const Layout &L);
-
-// the numbering of build_numbering on the device (pmx_setup.hip): lat2dof,
-// dof2lat (N each), GlobalNumbers (cells x local DOFs, optional) and the
-// per-cell frame codes (n^3), all device arrays; synchronises the stream
-void device_numbering(const Layout &L, int *lat2dof, int *dof2lat, int *gn, uint8_t *corner, hipStream_t st);
 
 // legacy-VTK file of nvar scalar functions (reference numbering) on the
 // level's hexahedra, in the layout of TOutput3D::WriteVtk

@@ -670,11 +670,25 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   double *v2 = TWO ? val2 + vbase(s, idx) : nullptr;
   constexpr int cnt = kSlice;
   double frhs = 0;
+  // Q1 rows of the one-matrix forms (27 slots) are accumulated in registers
+  // over the 8 cells (loops unrolled, every slot a compile-time index) and
+  // stored once; Q2 rows and the two-matrix form (whose unrolled loops need
+  // too many registers) add each cell's contribution into the cleared values
+  constexpr bool REG = ORDER == 1 && !TWO;
+  constexpr int UNR = REG ? 2 : 1;
+  double racc[REG ? 27 : 1];
+  if constexpr (REG) {
+#pragma unroll
+    for (int q = 0; q < 27; q++) racc[q] = 0.0;
+  }
   // incident cells per direction: lattice coordinate x lies in cells
   // floor((x-1)/ORDER) .. floor(x/ORDER) (two cells at a vertex plane)
   const int cx0 = (px - 1) / ORDER, cy0 = (py - 1) / ORDER, cz0 = (pz - 1) / ORDER;
+#pragma unroll UNR
   for (int iz = 0; iz < NCZ; iz++)
+#pragma unroll UNR
     for (int iy = 0; iy < NCY; iy++)
+#pragma unroll UNR
       for (int ix = 0; ix < NCX; ix++) {
         const int cx = cx0 + ix, cy = cy0 + iy, cz = cz0 + iz;
         const int ax = px - ORDER * cx, ay = py - ORDER * cy, az = pz - ORDER * cz;
@@ -734,10 +748,18 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
           const int bx = b % K, by = (b / K) % K, bz = b / (K * K);
           const int dx = ORDER * cx + bx - px, dy = ORDER * cy + by - py, dz = ORDER * cz + bz - pz;
           const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
-          v[slot * cnt] += acc[b];
-          if (TWO) v2[slot * cnt] += acc2[b];
+          if constexpr (REG) {
+            racc[slot] += acc[b];
+          } else {
+            v[slot * cnt] += acc[b];
+            if (TWO) v2[slot * cnt] += acc2[b];
+          }
         }
       }
+  if constexpr (REG && !RONLY) {
+#pragma unroll
+    for (int q = 0; q < 27; q++) v[q * cnt] = racc[q];
+  }
   rhs[p] = frhs;
 }
 
@@ -1529,23 +1551,38 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int f
 }
 // Row gather from the cell buffer: the thread of active node p adds row a of
 // each incident cell's local matrix into its CSS row and sums the rhs column.
+// The row's stencil slots are accumulated in registers over its incident
+// cells (in the order (iz, iy, ix)) and every slot is stored once, after the last cell that touches it (a slot (dx, dy, dz)
+// gets its last contribution from the cell on the positive side of each
+// offset that is >= 0), so the values are written without being read and
+// without a prior clear.  The local node of the row in cell (ix, iy, iz) and
+// the slot of every cell entry are compile-time constants: along an axis
+// with extent 2 (even lattice coordinate, a vertex plane) the node is local
+// coordinate 2 - 2 i of cell i, along an axis with extent 1 local coordinate 1.
 template <int EX, int EY, int EZ>
 __device__ __forceinline__ void assemble_row_kc(const DevSegment &s, int idx, int M, int n,
                                                 const double *__restrict__ kc, double *__restrict__ val,
                                                 double *__restrict__ rhs) {
   constexpr int NCX = EX == 1 ? 1 : 2, NCY = EY == 1 ? 1 : 2, NCZ = EZ == 1 ? 1 : 2;
+  constexpr int WX = 2 * EX + 1, WY = 2 * EY + 1, WZ = 2 * EZ + 1, NS = WX * WY * WZ;
   const int i = idx % s.cx, t = idx / s.cx;
   const int j = t % s.cy, k = t / s.cy;
   const int px = s.x0 + s.stride * i, py = s.y0 + s.stride * j, pz = s.z0 + s.stride * k;
   double *v = val + vbase(s, idx);
   double frhs = 0;
+  double acc[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) acc[q] = 0.0;
   const int cx0 = (px - 1) / 2, cy0 = (py - 1) / 2, cz0 = (pz - 1) / 2;
+#pragma unroll
   for (int iz = 0; iz < NCZ; iz++)
+#pragma unroll
     for (int iy = 0; iy < NCY; iy++)
+#pragma unroll
       for (int ix = 0; ix < NCX; ix++) {
-        const int cx = cx0 + ix, cy = cy0 + iy, cz = cz0 + iz;
-        const int a = (px - 2 * cx) + 3 * ((py - 2 * cy) + 3 * (pz - 2 * cz));
-        const int64_t cell = cx + (int64_t)n * (cy + (int64_t)n * cz);
+        const int ax = EX == 1 ? 1 : 2 - 2 * ix, ay = EY == 1 ? 1 : 2 - 2 * iy, az = EZ == 1 ? 1 : 2 - 2 * iz;
+        const int a = ax + 3 * (ay + 3 * az);
+        const int64_t cell = (cx0 + ix) + (int64_t)n * ((cy0 + iy) + (int64_t)n * (cz0 + iz));
         const double2 *Kr = reinterpret_cast<const double2 *>(kc + (cell * 27 + a) * kKcRow);
         double row[kKcRow];
 #pragma unroll
@@ -1556,12 +1593,17 @@ __device__ __forceinline__ void assemble_row_kc(const DevSegment &s, int idx, in
         }
 #pragma unroll
         for (int b = 0; b < 27; b++) {
-          const int bx = b % 3, by = (b / 3) % 3, bz = b / 9;
-          const int dx = 2 * cx + bx - px, dy = 2 * cy + by - py, dz = 2 * cz + bz - pz;
-          const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
-          v[slot * kSlice] += row[b];
+          const int dx = b % 3 - ax, dy = (b / 3) % 3 - ay, dz = b / 9 - az;
+          acc[(dx + EX) + WX * ((dy + EY) + WY * (dz + EZ))] += row[b];
         }
         frhs += row[27];
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          const int dx = q % WX - EX, dy = (q / WX) % WY - EY, dz = q / (WX * WY) - EZ;
+          const int lx = (NCX == 2 && dx >= 0) ? 1 : 0, ly = (NCY == 2 && dy >= 0) ? 1 : 0,
+                    lz = (NCZ == 2 && dz >= 0) ? 1 : 0;
+          if (lx == ix && ly == iy && lz == iz) v[q * kSlice] = acc[q];
+        }
       }
   rhs[px + M * (py + M * pz)] = frhs;
 }
@@ -1631,8 +1673,6 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
                        D.nboundary, D.M, prm_dev, rhs);
     return;
   }
-  check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
-  if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   if (kc && D.order == 2 && !val2) {
     // local matrices on MFMA, then the row gather from the cell buffer
     const unsigned grid = (unsigned)std::min<int64_t>((ncells + kCellWaves - 1) / kCellWaves, 4096);
@@ -1654,6 +1694,12 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
     hipLaunchKernelGGL(k_boundary_rhs, dim3((D.nboundary + kBlock - 1) / kBlock), dim3(kBlock), 0, st, D.boundary,
                        D.nboundary, D.M, prm_dev, rhs);
     return;
+  }
+  // the per-row evaluation of Q2 rows and of the two-matrix form adds every
+  // cell's contribution into the values (the other Q1 rows are stored once)
+  if (D.order == 2 || val2) {
+    check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
+    if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   }
 #define PMX_ASM(O, TW, IN)                                                                                   \
   hipLaunchKernelGGL((k_assemble<O, TW, IN, false>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,             \

@@ -31,6 +31,11 @@ struct DevLayout {
 
 void check(hipError_t e, const char *what);
 
+// the numbering of build_numbering on the device (pmx_setup.hip): lat2dof,
+// dof2lat (N each), GlobalNumbers (cells x local DOFs, optional) and the
+// per-cell frame codes (n^3), all device arrays; synchronises the stream
+void device_numbering(const Layout &L, int *lat2dof, int *dof2lat, int *gn, uint8_t *corner, hipStream_t st);
+
 // event pairs of timed dispatches (see set_dispatch_timer)
 struct DispatchTimer {
   std::vector<hipEvent_t> ev;
