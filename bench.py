@@ -414,6 +414,11 @@ def main():
         b_spmv = nvals * sv + N * sv + nact * sv
         b_col = nvals * sv / ncol + N * sv + 2 * nact * sv / ncol
         roof[prec] = {"spmv_ms": ms_spmv, "spmv_bytes": b_spmv, "col_ms": ms_col, "col_bytes": b_col}
+    # calibration of the colour launch on the same level and with the same
+    # dispatch events: a dispatch that does nothing (its workgroup counts) and
+    # a plain streaming read of the colour's FP32 values alone
+    cal = {"empty_dispatch_ms": S.time_kernel(12, -1, 1, reps=4),
+           "stream_read_values_ms": S.time_kernel(13, -1, 1, reps=4)}
     kern["ssor_sweep_fp32_ms"] = S.time_kernel(3, -1, 1, reps=10)
     kern["ssor_sweep_fp64_ms"] = S.time_kernel(3, -1, 0, reps=10)
     for name, k in (("vcycle_fp32_ms", 7), ("restrict_fp32_ms", 6), ("prolong_fp32_ms", 5)):
@@ -526,7 +531,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_rows<float,3,1> (one SSOR colour launch, finest level)",
                      "achieved": round(col_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(col_gbs / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_SSOR_LAUNCH,
-                     "algorithmic_bytes": int(r32["col_bytes"]), "dispatch_ms": round(r32["col_ms"], 5)},
+                     "algorithmic_bytes": int(r32["col_bytes"]), "dispatch_ms": round(r32["col_ms"], 5),
+                     "calibration": {"empty_dispatch_ms": round(cal["empty_dispatch_ms"], 5),
+                                     "stream_read_of_the_values_ms": round(cal["stream_read_values_ms"], 5),
+                                     "launch_over_stream_read": round(r32["col_ms"] / cal["stream_read_values_ms"],
+                                                                      3)}},
         "roofline_spmv": {"bound": "hbm", "kernel": "k_rows<float,0,1> (stencil SpMV, finest level)",
                           "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_SPMV,

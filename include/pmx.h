@@ -278,8 +278,11 @@ int pmx_tcd_restore_massmat(pmx_system *sys);
  * 10 and 11 give the average duration of a single dispatch, from events
  * recorded by the dispatch itself (hipExtLaunchKernelGGL), without the gaps
  * between dispatches: 10 one colour launch of an SSOR sweep (launch per
- * colour), 11 one SpMV.  precision 0 FP64, 1 FP32, 3 FP32 with the SOR/SSOR
- * colour launches on row-scaled FP16 values (opt[15] = 3).  out: ms. */
+ * colour), 11 one SpMV; calibration of 10 on the same level, also per
+ * dispatch: 12 a dispatch that does nothing with the workgroup count of each
+ * colour launch, 13 a plain streaming read of each colour's stored values in
+ * the precision's storage.  precision 0 FP64, 1 FP32, 3 FP32 with the
+ * SOR/SSOR colour launches on row-scaled FP16 values (opt[15] = 3).  out: ms. */
 int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int reps, double *ms);
 
 /* ---- learned post-smoother (no counterpart in the reference; it replaces

@@ -2147,6 +2147,44 @@ int pmx_time_kernel(pmx_system *sys, int kernel, int level, int precision, int r
       (void)hipEventDestroy(e1);
       return PMX_OK;
     }
+    if (kernel == 12 || kernel == 13) {
+      // calibration of kind 10 on the same level: 12 a dispatch that does
+      // nothing with the workgroup count of each colour launch; 13 a plain
+      // streaming read of each colour's stored values (FP64, FP32 or the FP16
+      // smoother values), one dispatch per colour; average per dispatch
+      std::vector<int64_t> lo(lv.L.ncolours, -1), hi(lv.L.ncolours, 0);
+      for (const Segment &g : lv.L.segs) {
+        const int64_t len = (g.count + kSlice - 1) / kSlice * kSlice * g.ns;
+        if (lo[g.colour] < 0 || g.voff < lo[g.colour]) lo[g.colour] = g.voff;
+        hi[g.colour] = std::max(hi[g.colour], g.voff + len);
+      }
+      const size_t es = Y.half_smoother ? 2 : (precision ? 4 : 8);
+      const char *base = Y.half_smoother ? (const char *)lv.val16.p
+                                         : (precision ? (const char *)lv.v32.val.p : (const char *)lv.v64.val.p);
+      float *sink = lv.v32.aux.p ? lv.v32.aux.p : (float *)lv.v64.aux.p;
+      auto run = [&]() {
+        for (int c = 0; c < lv.L.ncolours; c++) {
+          if (lo[c] < 0) continue;
+          if (kernel == 12) empty_dispatch(lv.D.colour_off[c + 1] - lv.D.colour_off[c], st);
+          else stream_read(base + lo[c] * es, (hi[c] - lo[c]) * (int64_t)es, sink, st);
+        }
+      };
+      run();
+      DispatchTimer tm;
+      set_dispatch_timer(&tm);
+      try {
+        for (int r = 0; r < reps; r++) run();
+      } catch (...) {
+        set_dispatch_timer(nullptr);
+        throw;
+      }
+      set_dispatch_timer(nullptr);
+      check(hipStreamSynchronize(st), "sync");
+      *ms = tm.used ? tm.total_ms() / tm.used : 0.0;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return PMX_OK;
+    }
     if (kernel >= 8) {
       if (!precision) throw std::invalid_argument("the learned smoother runs in FP32");
       if (!Y.nets[li].ready) throw std::logic_error("learned smoother: train or load parameters first");

@@ -50,6 +50,13 @@ struct DispatchTimer {
 // dispatch between two events of the timer; nullptr uninstalls
 void set_dispatch_timer(DispatchTimer *t);
 
+// calibration of the dispatch measurements (both honour the dispatch timer):
+// a dispatch of `blocks` workgroups that does nothing, and a plain streaming
+// read of `bytes` bytes at p (16-byte aligned; sink: one float, written never
+// in practice)
+void empty_dispatch(int blocks, hipStream_t st);
+void stream_read(const void *p, int64_t bytes, float *sink, hipStream_t st);
+
 template <typename T> void fill(T *x, T v, int64_t n, hipStream_t st);
 template <typename T> void copy(T *y, const T *x, int64_t n, hipStream_t st);
 template <typename T>

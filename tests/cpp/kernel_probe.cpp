@@ -4,7 +4,9 @@
 // the curl-Fourier convection of the benchmark).  Five levels 4^3 .. 64^3,
 // then on the finest level: 6 FP32 stencil SpMV launches, 3 FP32 SSOR sweeps
 // (53 colour launches each; the first SpMV and sweep are untimed), 3 SSOR
-// sweeps on the row-scaled FP16 values (k_sor_scaled, the first untimed) and
+// sweeps on the row-scaled FP16 values (k_sor_scaled, the first untimed), the
+// calibration dispatches of the colour launches (empty dispatches with their
+// workgroup counts, streaming reads of each colour's FP32 and FP16 values) and
 // 5 device copies of 1 GiB (hipMemcpy device to device, the calibration kernel
 // of known size).  Prints the average dispatch durations.
 //
@@ -54,6 +56,22 @@ int main(int argc, char **argv) {
     return 1;
   }
   std::printf("fp32 ssor colour launch, FP16 values %.4f ms per dispatch\n", ms);
+  // calibration of the colour launches: an empty dispatch with each colour
+  // launch's workgroup count, and a plain streaming read of each colour's
+  // stored values (FP32, then the FP16 smoother values)
+  const struct {
+    int kernel, precision;
+    const char *what;
+  } cal[] = {{12, 1, "empty dispatch, colour-launch grids"},
+             {13, 1, "stream read of a colour's FP32 values"},
+             {13, 3, "stream read of a colour's FP16 values"}};
+  for (const auto &c : cal) {
+    if (pmx_time_kernel(sys, c.kernel, -1, c.precision, 2, &ms)) {
+      std::printf("time_kernel failed: %s\n", pmx_last_error());
+      return 1;
+    }
+    std::printf("%s %.4f ms per dispatch\n", c.what, ms);
+  }
   const size_t bytes = size_t(1) << 30;
   void *a = nullptr, *b = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) return 1;

@@ -198,3 +198,23 @@ def test_caller_level_matrices(ctx):
         S.set_entries(bad, level=2)
     S.close()
     R.close()
+
+
+def test_dispatch_timings(ctx):
+    """pmx_time_kernel's per-dispatch timings (10 colour launch, 11 SpMV) and
+    the calibration dispatches of the colour launch (12 empty, 13 streaming
+    read of each colour's values) in every precision return positive
+    durations; timing before assembly and unknown precisions are errors."""
+    S = pa.System(ctx, 1, 3, 2)
+    with pytest.raises(pa.PmxError):
+        S.time_kernel(10, -1, 1, reps=1)
+    S.assemble(P.config1b())
+    for k in (10, 11, 12, 13):
+        for prec in (0, 1, 3):
+            if k == 11 and prec == 3:
+                continue
+            ms = S.time_kernel(k, -1, prec, reps=2)
+            assert 0.0 < ms < 50.0, (k, prec, ms)
+    with pytest.raises(pa.PmxError):
+        S.time_kernel(12, -1, 2, reps=1)
+    S.close()
