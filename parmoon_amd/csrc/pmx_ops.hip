@@ -507,9 +507,12 @@ __global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T
 // ============================================================ assembly
 // Coefficients at quadrature point q of cell (cx, cy, cz), in the canonical
 // (axis-aligned) frame of the lattice: {w|detJ|, eps, b0, b1, b2, c, f, delta}.
+// with_delta = false: the record without its stabilisation parameter
+// (o[7] = 0), for callers that read it from the per-point buffer of
+// k_qp_delta
 __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const double *__restrict__ qref, int n,
                                           int cx, int cy, int cz, int q, int form, double *o,
-                                          const uint8_t *__restrict__ corner = nullptr) {
+                                          const uint8_t *__restrict__ corner = nullptr, bool with_delta = true) {
   const double h = 1.0 / n;
   const double xi = qref[4 * q], eta = qref[4 * q + 1], zeta = qref[4 * q + 2], w = qref[4 * q + 3];
   // THexaAffin::GetOrigFromRef: X = xc0 + xc1 xi + ... with xc0 the cell centre
@@ -517,6 +520,11 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
   const QpCoeffs c = example_coeffs(prm, X, Y, Z);
+  if (!with_delta) {
+    o[0] = w * (h * h * h / 8.0);
+    o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = 0.0;
+    return;
+  }
   if (corner && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
     // the coefficients at the reference's first point of the cell, (a, a, a)
     // with a = qref[0] in the cell's own frame: centre + (h/2) a (e1 + e2 + e3)
@@ -542,6 +550,28 @@ __global__ __launch_bounds__(kBlock) void k_qp_data(const double *__restrict__ p
   const int64_t cell = id / nq;
   const int cx = cell % n, cy = (cell / n) % n, cz = cell / ((int64_t)n * n);
   qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], out + id * 8, corner);
+}
+// The stabilisation parameter alone (record entry 7) of every quadrature
+// point of every cell, once: the per-row evaluation of the inline
+// coefficient functions would otherwise compute it (a square root, a
+// hyperbolic tangent and divisions) once for each of the up to 8 rows that
+// share a cell.  Point-major, dl[q * n^3 + cell], so that the rows of a
+// wavefront (cells two apart along x) read nearby entries.  Grid: x over the
+// cells of one z-plane, y = q * n + cz.
+__global__ __launch_bounds__(kBlock) void k_qp_delta(const double *__restrict__ prm, const double *__restrict__ qref,
+                                                     int n, double *__restrict__ dl) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * n) return;
+  const int cx = t % n, cy = t / n;
+  const int cz = blockIdx.y % n, q = blockIdx.y / n;
+  double o[8];
+  qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], o);
+  dl[(int64_t)q * n * n * n + t + (int64_t)n * n * cz] = o[7];
+}
+// forms whose records carry a stabilisation parameter
+__host__ __device__ inline bool form_has_delta(int form) {
+  return form == PMX_FORM_SDFEM || form == PMX_FORM_MASS_SUPG || form == PMX_FORM_RHS_SUPG ||
+         form == PMX_FORM_TCD_SUPG_AK;
 }
 // The curl-Fourier convection (fourier_b) without a cosine per mode and
 // point.  The quadrature rules are tensor products (point q = ix + np (iy +
@@ -698,10 +728,17 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
         double acc[NB], acc2[NB];
 #pragma unroll
         for (int b = 0; b < NB; b++) acc[b] = acc2[b] = 0;
+        // inline coefficients: qp holds the per-point stabilisation
+        // parameters of k_qp_delta (nullptr for the forms without one), read
+        // one point ahead
+        const int64_t ncell = (int64_t)n * n * n;
+        double dl_next = (INL && qp) ? qp[cell] : 0.0;
         for (int q = 0; q < nq; q++) {
           double Qq[8];
           if (INL) {
-            qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq);
+            qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq, nullptr, false);
+            Qq[7] = dl_next;
+            if (qp) dl_next = qp[(q + 1 < nq ? q + 1 : q) * ncell + cell];
           } else {
 #pragma unroll
             for (int e = 0; e < 8; e++) Qq[e] = Q[8 * q + e];
@@ -1697,11 +1734,23 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
                          qref, nq, D.n, qpbuf, corner);
     }
   }
+  // the per-row evaluation with inline coefficients reads the stabilisation
+  // parameters from a per-point buffer (the start of qpbuf), filled once
+  const double *qsrc = qpbuf;
+  if (inl) {
+    qsrc = nullptr;
+    if (form_has_delta(form) && !(kc && D.order == 2 && !val2 && form != PMX_FORM_RHS_SUPG)) {
+      if ((int64_t)D.n * nq > 65535) throw std::invalid_argument("assemble: level too large for k_qp_delta");
+      hipLaunchKernelGGL(k_qp_delta, dim3((unsigned)((D.n * D.n + kBlock - 1) / kBlock), (unsigned)(D.n * nq)),
+                         dim3(kBlock), 0, st, prm_dev, qref, D.n, qpbuf);
+      qsrc = qpbuf;
+    }
+  }
   if (form == PMX_FORM_RHS || form == PMX_FORM_RHS_SUPG) {
     // right-hand-side forms (no local matrices): the matrices are left as they are
 #define PMX_RHS(O, IN)                                                                                     \
   hipLaunchKernelGGL((k_assemble<O, false, IN, true>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,        \
-                     D.blocks_all, D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, nullptr, nullptr, rhs)
+                     D.blocks_all, D.segs, D.M, D.n, nq, prm_dev, qref, qsrc, tab, nullptr, nullptr, rhs)
     if (D.order == 1) {
       if (inl) PMX_RHS(1, true);
       else PMX_RHS(1, false);
@@ -1745,7 +1794,7 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
 #define PMX_ASM(O, TW, IN)                                                                                   \
   hipLaunchKernelGGL((k_assemble<O, TW, IN, false>), dim3(D.nblocks_all), dim3(kRowsPerBlock), 0, st,             \
                      D.blocks_all,                                                                                \
-                     D.segs, D.M, D.n, nq, prm_dev, qref, qpbuf, tab, val, val2, rhs)
+                     D.segs, D.M, D.n, nq, prm_dev, qref, qsrc, tab, val, val2, rhs)
 #define PMX_ASM2(O, TW) \
   do {                   \
     if (inl) PMX_ASM(O, TW, true); \
