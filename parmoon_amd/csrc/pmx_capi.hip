@@ -536,13 +536,17 @@ static void learned_post(System &Y, Level &lv, LearnedNet &net, bool fresh = tru
     // below (dinv = 0 there, A c = 0 there) and are finite (zero or f - x)
     css_apply<float>(lv.D, 1, v.val.p, v.x.p, v.f.p, v.d.p, 0.0, -1, st);
   }
+  // the network's first layer builds its input from the features, u = D^-1 d
+  // and the partials of ||u||^2
+  LearnedInput in{lv.feat.p, lv.lc.p, nullptr, learned_input_parts(N)};
   if (fresh) {
-    learned_scaled_input(net, lv.dinv.p, d0, lv.lc.p, N, part, sc, sq, lv.feat.p, st);
+    learned_input_sq(lv.dinv.p, d0, lv.lc.p, N, part, st);
+    in.part = part;
   } else {
     // d, u = D^-1 d and the partials of ||u||^2 were left by the previous step
-    learned_carried_input(net, lv.lc.p, N, part + 2 * kRedBlocks, sc, sq, lv.feat.p, st);
+    in.part = part + 2 * kRedBlocks;
   }
-  net_forward_correction(net, lv.L.M, sc, sq, lv.lc.p, st);
+  net_forward_correction(net, lv.L.M, sc, sq, lv.lc.p, st, &in);
   css_apply<float>(lv.D, 0, v.val.p, lv.lc.p, nullptr, lv.lac.p, 0.0, -1, st);  // Dirichlet entries stay 0
   if (carry)
     learned_update_carry(d0, v.d.p, lv.lac.p, lv.lc.p, v.x.p, lv.dinv.p, lv.lc.p, N, part, part + 2 * kRedBlocks,

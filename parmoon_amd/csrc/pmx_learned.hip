@@ -79,11 +79,20 @@ template <int CIN> __host__ __device__ constexpr int tap_lds(int t) {
   return ((t % 3 - 1) + kHX * ((t / 3) % 3 - 1 + kHY * (t / 9 - 1))) * CIN;
 }
 
-template <int CIN, int COUT, int MODE>
+// IN0 (first layer of a learned step, CIN = kNetIn): the input is built while
+// it is staged: `in` holds the operator features and channel 0 becomes
+// bf16(uin * sq / ||u||), ||u|| reduced by every workgroup from the npin
+// partials pin in a fixed order (a strided sum per thread, then block_sum),
+// so every workgroup gets the same value; workgroup 0 stores it in nrmout
+// for the output layer's scaling.
+template <int CIN, int COUT, int MODE, bool IN0 = false>
 __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict__ in, const uint16_t *__restrict__ w,
                                               const float *__restrict__ bias, float *__restrict__ pre,
                                               uint16_t *__restrict__ out_bf, float *__restrict__ out_f,
-                                              const double *__restrict__ nrm = nullptr, double sq = 1.0) {
+                                              const double *__restrict__ nrm = nullptr, double sq = 1.0,
+                                              const float *__restrict__ uin = nullptr,
+                                              const double *__restrict__ pin = nullptr, int npin = 0,
+                                              double *__restrict__ nrmout = nullptr) {
   // K chunks of 32: CIN 1 -> taps 0..26 in one chunk; CIN 8 -> 4 taps per
   // chunk (7 chunks); CIN 16 -> 2 taps per chunk (14 chunks)
   constexpr int TPC = 32 / CIN;
@@ -92,6 +101,21 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
   const int nbx = (M + kBX - 1) / kBX, nby = (M + kBY - 1) / kBY;
   const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
   const int X0 = bx * kBX, Y0 = by * kBY, Z0 = bz * kBZ;
+  float sc_in = 1.f;
+  if constexpr (IN0) {
+    static_assert(CIN == kNetIn, "IN0: the first layer");
+    __shared__ double sh_norm;
+    double s = 0;
+    for (int i = threadIdx.x; i < npin; i += 256) s += pin[i];
+    s = block_sum<double>(s);
+    if (threadIdx.x == 0) {
+      sh_norm = sqrt(s);
+      if (blockIdx.x == 0) nrmout[0] = sh_norm;
+    }
+    __syncthreads();
+    const double nv = sh_norm;
+    sc_in = nv > 0 ? (float)(sq / nv) : 0.f;
+  }
   // stage the halo brick (zero outside the lattice)
   if (CIN == 1) {
     for (int q = threadIdx.x; q < kHalo; q += 256) {
@@ -107,8 +131,13 @@ __global__ __launch_bounds__(256) void k_conv(int M, const uint16_t *__restrict_
       const int hx = n % kHX, hy = (n / kHX) % kHY, hz = n / (kHX * kHY);
       const int x = X0 - 1 + hx, y = Y0 - 1 + hy, z = Z0 - 1 + hz;
       u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M)
-        v = *reinterpret_cast<const u16x8 *>(in + (x + (int64_t)M * (y + (int64_t)M * z)) * CIN + 8 * piece);
+      if (x >= 0 && y >= 0 && z >= 0 && x < M && y < M && z < M) {
+        const int64_t node = x + (int64_t)M * (y + (int64_t)M * z);
+        v = *reinterpret_cast<const u16x8 *>(in + node * CIN + 8 * piece);
+        if constexpr (IN0) {
+          if (piece == 0) v[0] = f2bf(uin[node] * sc_in);
+        }
+      }
       *reinterpret_cast<u16x8 *>(sh + n * CIN + 8 * piece) = v;
     }
   }
@@ -572,11 +601,19 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
                      nullptr, nullptr, net.out);
 }
 
-void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st) {
+void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st,
+                            const LearnedInput *fused) {
   const unsigned g = conv_grid(M);
   auto P = [&](int l) { return net.params + net_param_offset(l) + net_wsize(l); };  // bias
-  hipLaunchKernelGGL((k_conv<kNetIn, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride, P(0),
-                     nullptr, net.act[1], nullptr, nullptr, 1.0);
+  if (fused) {
+    // the first layer builds its input (features + scaled defect) itself
+    hipLaunchKernelGGL((k_conv<kNetIn, 16, 0, true>), dim3(g), dim3(256), 0, st, M, fused->feat,
+                       net.wbf + 0 * kWStride, P(0), nullptr, net.act[1], nullptr, nullptr, sqrt_count, fused->u,
+                       fused->part, fused->np, const_cast<double *>(nrm));
+  } else {
+    hipLaunchKernelGGL((k_conv<kNetIn, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride,
+                       P(0), nullptr, net.act[1], nullptr, nullptr, 1.0);
+  }
   hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[1], net.wbf + 1 * kWStride, P(1),
                      nullptr, net.act[2], nullptr, nullptr, 1.0);
   hipLaunchKernelGGL((k_conv<16, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[2], net.wbf + 2 * kWStride, P(2),
@@ -761,34 +798,6 @@ __global__ void k_interior_scale(const float *__restrict__ in, int M, const doub
     c[i] = interior ? in[i] * sc : 0.f;
   }
 }
-// act0 = feat with channel 0 = bf16(u sq / ||u||), ||u|| reduced by every
-// workgroup from the partials of k_learned_input_sq in the fixed order of
-// k_reduce_final (the same value everywhere); workgroup 0 stores it for the
-// output scaling
-__global__ __launch_bounds__(kBlock) void k_build_input_parts(const float *__restrict__ u,
-                                                              const double *__restrict__ part, int np, double sq,
-                                                              const uint16_t *__restrict__ feat, int64_t n,
-                                                              uint16_t *__restrict__ act0, double *__restrict__ nrm) {
-  __shared__ double sh_norm;
-  double s = 0;
-  for (int i = threadIdx.x; i < np; i += kBlock) s += part[i];
-  s = block_sum<double>(s);
-  if (threadIdx.x == 0) {
-    sh_norm = sqrt(s);
-    if (blockIdx.x == 0) nrm[0] = sh_norm;
-  }
-  __syncthreads();
-  const double nv = sh_norm;
-  const float sc = nv > 0 ? (float)(sq / nv) : 0.f;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-#pragma unroll
-    for (int piece = 0; piece < kNetIn / 8; piece++) {
-      u16x8 v = *reinterpret_cast<const u16x8 *>(feat + i * kNetIn + 8 * piece);
-      if (piece == 0) v[0] = f2bf(u[i] * sc);
-      *reinterpret_cast<u16x8 *>(act0 + i * kNetIn + 8 * piece) = v;
-    }
-  }
-}
 // x += alpha c, alpha = max(0, (d, Ac) / (Ac, Ac)) reduced by every workgroup
 // from the partials of k_dot2_partial in the fixed order of k_reduce_final
 __global__ __launch_bounds__(kBlock) void k_learned_step_parts(float *__restrict__ x, const float *__restrict__ c,
@@ -867,18 +876,10 @@ void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipSt
 static unsigned red_grid(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, kRedBlocks));
 }
-void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
-                          double *nrm, double sq, const uint16_t *feat, hipStream_t st) {
-  const unsigned g = red_grid(n);
-  hipLaunchKernelGGL(k_learned_input_sq, dim3(g), dim3(kBlock), 0, st, dinv, d, u, part, n);
-  hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, part, (int)g, sq, feat, n,
-                     net.act[0], nrm);
+void learned_input_sq(const float *dinv, const float *d, float *u, int64_t n, double *part, hipStream_t st) {
+  hipLaunchKernelGGL(k_learned_input_sq, dim3(red_grid(n)), dim3(kBlock), 0, st, dinv, d, u, part, n);
 }
-void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const double *pu, double *nrm, double sq,
-                           const uint16_t *feat, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_input_parts, dim3(grid_for(n)), dim3(kBlock), 0, st, u, pu, (int)red_grid(n), sq, feat,
-                     n, net.act[0], nrm);
-}
+int learned_input_parts(int64_t n) { return (int)red_grid(n); }
 void learned_update_carry(const float *d_in, float *d, const float *ac, const float *c, float *x, const float *dinv,
                           float *u, int64_t n, double *part, double *pu, hipStream_t st) {
   const unsigned g = red_grid(n);

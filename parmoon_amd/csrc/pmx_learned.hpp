@@ -71,7 +71,22 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st);
 // inference from net.act[0] straight into the correction: c = out * (*nrm) /
 // sqrt_count on interior nodes, 0 on the boundary (the output layer applies
 // the scaling and the boundary mask to its FP32 result)
-void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st);
+// the first layer's input built inside that layer (k_conv IN0): feat (the
+// level's operator features, [N][kNetIn] BF16), u (FP32 scaled defect
+// D^-1 d) and the np partial sums of ||u||^2 in part
+struct LearnedInput {
+  const uint16_t *feat;
+  const float *u;
+  const double *part;
+  int np;
+};
+// the network's correction c on an M^3 lattice; nrm: ||u|| (read, or with
+// `fused` written by the first layer); the input is net.act[0] unless fused
+void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st,
+                            const LearnedInput *fused = nullptr);
+// u = dinv d and the partials of ||u||^2 (learned_input_parts(n) of them)
+void learned_input_sq(const float *dinv, const float *d, float *u, int64_t n, double *part, hipStream_t st);
+int learned_input_parts(int64_t n);
 // backward for the loss mean((out - t)^2) over interior nodes; accumulates into net.grads
 // and returns (asynchronously) the loss into *loss_dev
 void net_backward(LearnedNet &net, int M, const float *t, float *loss_dev, hipStream_t st);
@@ -87,18 +102,10 @@ struct DevLayout;
 void learned_diag_inv(const DevLayout &D, const float *val, float *dinv, hipStream_t st);
 // u = dinv * d (lattice), the unscaled network input
 void learned_input(const float *dinv, const float *d, float *u, int64_t n, hipStream_t st);
-// u = dinv d, *nrm = ||u|| and net.act[0] = feat with channel 0 =
-// bf16(u sqrt_count / ||u||), in two launches (part: kRedBlocks partials)
-void learned_scaled_input(LearnedNet &net, const float *dinv, const float *d, float *u, int64_t n, double *part,
-                          double *nrm, double sqrt_count, const uint16_t *feat, hipStream_t st);
 // x += alpha c with alpha = max(0, (d, ac) / (ac, ac)), in two launches
 // (part: 2 kRedBlocks partials)
 void learned_update(const float *d, const float *ac, const float *c, float *x, int64_t n, double *part,
                     hipStream_t st);
-// net.act[0] from u and the partials pu of ||u||^2 that learned_update_carry
-// left (*nrm = ||u||)
-void learned_carried_input(LearnedNet &net, const float *u, int64_t n, const double *pu, double *nrm,
-                           double sqrt_count, const uint16_t *feat, hipStream_t st);
 // learned_update with the defect d_in, then from the updated defect
 // d = d_in - alpha ac (d_in may be d) the next step's u = dinv d (u may alias
 // c) and the partials of ||u||^2 into pu (kRedBlocks doubles apart from
