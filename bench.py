@@ -291,7 +291,7 @@ def main():
     # learned-form defaults: the steady Q2 workload smooths the levels below
     # the finest with three network steps in place of pre- and post-smoothing;
     # the transient Q1 workload (8-colour SSOR, cheap on coarse levels) uses
-    # one network step as the finest level's post-smoother
+    # two network steps as the finest level's post-smoother
     ap.add_argument("--learned-levels", type=int, default=None,
                     help="finest levels with a network (0: all but the coarsest)")
     ap.add_argument("--learned-skip", type=int, default=None, help="the finest of those levels keep SSOR")
@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--eval-seeds", type=int, default=4)
     args = ap.parse_args()
     if args.transient > 0:
-        dflt = dict(learned_levels=1, learned_skip=0, learned_steps=3, learned_pre=0, train_steps=1200)
+        dflt = dict(learned_levels=1, learned_skip=0, learned_steps=2, learned_pre=0, train_steps=1200)
     elif args.train_set > 0:
         dflt = dict(learned_levels=0, learned_skip=1, learned_steps=2, learned_pre=1, train_steps=16)
     else:
