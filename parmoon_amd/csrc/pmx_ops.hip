@@ -633,9 +633,15 @@ __device__ __forceinline__ QpCoeffs fourier_coeffs_tab(const double *__restrict_
   return k;
 }
 // Stage 1 for the curl-Fourier example: the records of k_qp_data with the
-// coefficients from the table above.
-// Grid: x over the points of one line of cells (cx, q), y = cy + n cz; the
-// rule size is a template parameter so that no index needs a runtime division.
+// coefficients from the table above.  A thread takes the np points of one
+// x-line (iy, iz) of a cell: per mode it reads the y and z factors once and
+// the np x factors, so the table reads per point (the bound of this kernel:
+// 24 modes x 3 complex factors per point) drop from 3 to 1 + 2/np complex
+// values.  Per point and mode the products and sums are those of
+// fourier_coeffs_tab, in the same order.
+// Grid: x over the (cx, iy, iz) lines of one row of cells, y = cy + n cz;
+// the rule size is a template parameter so that no index needs a runtime
+// division.
 template <int NQ>
 __global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__restrict__ prm,
                                                             const double *__restrict__ qref, int n,
@@ -643,23 +649,67 @@ __global__ __launch_bounds__(kBlock) void k_qp_data_fourier(const double *__rest
                                                             const uint8_t *__restrict__ corner) {
   constexpr int np = NQ == 8 ? 2 : (NQ == 27 ? 3 : 4);
   static_assert(np * np * np == NQ, "tensor rule");
+  constexpr int NL = np * np;  // x-lines per cell
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n * NQ) return;
-  const int cx = t / NQ, q = t % NQ;
+  if (t >= n * NL) return;
+  const int cx = t / NL, r = t % NL, iy = r % np, iz = r / np;
   const int cy = blockIdx.y % n, cz = blockIdx.y / n;
-  const int64_t id = ((int64_t)blockIdx.y * n + cx) * NQ + q;
-  const int ix = q % np, iy = (q / np) % np, iz = q / (np * np);
-  const QpCoeffs k = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, ix, iy, iz);
-  if (corner && (int)prm[PMX_PRM_CELL_MEASURE] == 4) {
+  const double twopi = 6.283185307179586476925286766559;
+  const int64_t L = (int64_t)n * np;  // entries per (mode, axis)
+  double dA[np][3][3];
+#pragma unroll
+  for (int i = 0; i < np; i++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) dA[i][c][0] = dA[i][c][1] = dA[i][c][2] = 0.0;
+  // modes 8 c .. 8 c + 7 belong to component c of the potential (the
+  // component loop is unrolled so that every accumulator index is a constant)
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll 2
+    for (int mm = 0; mm < kFourierModes / 3; mm++) {
+      const int m = (kFourierModes / 3) * c + mm;
+      const double *tm = prm + PMX_PRM_FOURIER + m * 5;
+      const double *fx = tab + 2 * ((m * 3 + 0) * L + cx * np);
+      const double *fy = tab + 2 * ((m * 3 + 1) * L + cy * np + iy);
+      const double *fz = tab + 2 * ((m * 3 + 2) * L + cz * np + iz);
+      const double fy0 = fy[0], fy1 = fy[1], fz0 = fz[0], fz1 = fz[1];
+      const double t0 = tm[0], t1 = tm[1], t2 = tm[2], t3 = tm[3];
+#pragma unroll
+      for (int ix = 0; ix < np; ix++) {
+        const double fx0 = fx[2 * ix], fx1 = fx[2 * ix + 1];
+        const double r1 = fx0 * fy0 - fx1 * fy1, i1 = fx0 * fy1 + fx1 * fy0;
+        const double cs = t0 * (r1 * fz0 - i1 * fz1) * twopi;
+        dA[ix][c][0] += cs * t1;
+        dA[ix][c][1] += cs * t2;
+        dA[ix][c][2] += cs * t3;
+      }
+    }
+  }
+  const double sc = prm[PMX_PRM_BSCALE];
+  const bool cm4 = corner && (int)prm[PMX_PRM_CELL_MEASURE] == 4;
+  QpCoeffs k0;
+  if (cm4) {
     // the coefficients at the reference's first point of the cell, (a, a, a)
     // in its own frame: 1D point 0 (a) along an axis where the frame points
     // forward, np - 1 (-a; the rules are symmetric) where it points back
     const int code = corner[cx + (int64_t)n * (cy + (int64_t)n * cz)];
-    const QpCoeffs k0 = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, (code & 1) ? 0 : np - 1,
-                                           (code & 2) ? 0 : np - 1, (code & 4) ? 0 : np - 1);
-    qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8, &k0);
-  } else {
-    qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, out + id * 8);
+    k0 = fourier_coeffs_tab(prm, tab, n, np, cx, cy, cz, (code & 1) ? 0 : np - 1, (code & 2) ? 0 : np - 1,
+                            (code & 4) ? 0 : np - 1);
+  }
+  const int64_t cell = cx + (int64_t)n * (blockIdx.y);
+#pragma unroll
+  for (int ix = 0; ix < np; ix++) {
+    QpCoeffs k;
+    k.eps = prm[PMX_PRM_EPS];
+    k.b0 = sc * (dA[ix][2][1] - dA[ix][1][2]);
+    k.b1 = sc * (dA[ix][0][2] - dA[ix][2][0]);
+    k.b2 = sc * (dA[ix][1][0] - dA[ix][0][1]);
+    k.c = 0;
+    k.f = 1.0;
+    const int q = ix + np * r;
+    double *o = out + (cell * NQ + q) * 8;
+    if (cm4) qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, o, &k0);
+    else qp_record(prm, (int)prm[PMX_PRM_FORM], 1.0 / n, qref[4 * q + 3], k, o);
   }
 }
 // examples whose coefficients are cheaper to evaluate again than to read back
@@ -1721,7 +1771,8 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
       const int64_t nt = fourier_tab_len(D.n, nq) / 2;  // complex entries
       hipLaunchKernelGGL(k_fourier_tables, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
                          prm_dev, qref, nq, D.n, ftab);
-      const dim3 g((unsigned)((D.n * nq + kBlock - 1) / kBlock), (unsigned)(D.n * D.n));
+      const int nl = nq == 8 ? 4 : (nq == 27 ? 9 : 16);  // x-lines of points per cell
+      const dim3 g((unsigned)((D.n * nl + kBlock - 1) / kBlock), (unsigned)(D.n * D.n));
       if (nq == 8)
         hipLaunchKernelGGL(k_qp_data_fourier<8>, g, dim3(kBlock), 0, st, prm_dev, qref, D.n, ftab, qpbuf, corner);
       else if (nq == 27)
