@@ -1588,7 +1588,15 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 // 8 wavefronts share one copy of the tables (64 KB for the 64-point rule)
 constexpr int kCellWaves = 8, kCellThreads = 64 * kCellWaves;
 constexpr int kKcRow = 28;  // doubles per local row in the cell buffer: 27 entries + rhs
-template <bool INL, bool MASS>
+#ifndef PMX_CELLS_PER_PASS
+#define PMX_CELLS_PER_PASS 2
+#endif
+constexpr int kCellsPerPass = PMX_CELLS_PER_PASS;  // cells per wavefront pass of k_cell_mfma
+// CPC cells per wavefront pass: every quadrature point's table values are
+// read from LDS once for the CPC cells (10 LDS reads per lane and point
+// serve CPC x 4 MFMAs), with CPC independent accumulation chains per
+// wavefront.  Each cell's products and sums are those of one cell alone.
+template <bool INL, bool MASS, int CPC>
 __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int form, const double *__restrict__ prm,
                                                   const double *__restrict__ qref, const double *__restrict__ qp,
                                                   const double *__restrict__ tab, double *__restrict__ kc) {
@@ -1609,70 +1617,99 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int f
   const double s1 = comp == 1 ? 1.0 : 0.0, s2 = comp == 2 ? 1.0 : 0.0, s3 = comp == 3 ? 1.0 : 0.0;
   const bool rhs_col = col == 11;  // column 27 = 16 + 11 of the second column tile
   const int64_t ncells = (int64_t)n * n * n;
-  for (int64_t cell = (int64_t)blockIdx.x * kCellWaves + wave; cell < ncells;
-       cell += (int64_t)gridDim.x * kCellWaves) {
-    const int cx = (int)(cell % n), cy = (int)((cell / n) % n), cz = (int)(cell / ((int64_t)n * n));
-    doublex4 a00 = {0.0, 0.0, 0.0, 0.0}, a01 = a00, a10 = a00, a11 = a00;
+  const int64_t ngroups = (ncells + CPC - 1) / CPC;
+  for (int64_t grp = (int64_t)blockIdx.x * kCellWaves + wave; grp < ngroups;
+       grp += (int64_t)gridDim.x * kCellWaves) {
+    // the cells of this pass (a pass past the last cell repeats the last
+    // one and does not store it)
+    int64_t cell[CPC];
+    int cx[CPC], cy[CPC], cz[CPC];
+#pragma unroll
+    for (int k = 0; k < CPC; k++) {
+      cell[k] = grp * CPC + k < ncells ? grp * CPC + k : ncells - 1;
+      cx[k] = (int)(cell[k] % n);
+      cy[k] = (int)((cell[k] / n) % n);
+      cz[k] = (int)(cell[k] / ((int64_t)n * n));
+    }
+    doublex4 a00[CPC], a01[CPC], a10[CPC], a11[CPC];
+#pragma unroll
+    for (int k = 0; k < CPC; k++) {
+      a00[k] = doublex4{0.0, 0.0, 0.0, 0.0};
+      a01[k] = a00[k];
+      a10[k] = a00[k];
+      a11[k] = a00[k];
+    }
     // coefficients of the next quadrature point are loaded one step ahead
-    double Qn[8];
+    double Qn[CPC][8];
     if (!INL) {
 #pragma unroll
-      for (int e = 0; e < 8; e++) Qn[e] = qp[cell * nq * 8 + e];
+      for (int k = 0; k < CPC; k++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) Qn[k][e] = qp[cell[k] * nq * 8 + e];
     }
     for (int q = 0; q < nq; q++) {
-      double Qq[8];
-      if (INL) {
-        qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq);
-      } else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) Qq[e] = Qn[e];
-        const int qn = q + 1 < nq ? q + 1 : q;
-#pragma unroll
-        for (int e = 0; e < 8; e++) Qn[e] = qp[(cell * nq + qn) * 8 + e];
-      }
-      const double w = Qq[0], eps = Qq[1], cc = Qq[5], ff = Qq[6], dl = Qq[7];
-      const double b0 = Qq[2], b1 = Qq[3], b2 = Qq[4];
-      // row comp of D = s v + t e_comp with v = (c, g b): s = w for row 0 and
-      // w g delta b_comp below it, t = w g^2 eps on the diagonal of rows 1-3;
-      // the right-hand-side weight is f s
-      double d0, d1, d2, d3, rr;
-      if (MASS) {
-        d0 = r0 ? w : 0.0;
-        d1 = d2 = d3 = 0.0;
-        // right-hand-side weights w f (1, g delta b): delta = 0 for
-        // MatrixMRhsAssemble, Compute_SDFEM_delta for MatrixMRhsAssemble_SUPG
-        const double be = s1 * b0 + s2 * b1 + s3 * b2;
-        rr = ff * (r0 ? w : w * g * dl * be);
-      } else {
-        const double be = s1 * b0 + s2 * b1 + s3 * b2;  // b_comp (0 for comp 0)
-        const double sc = r0 ? w : w * g * dl * be;
-        const double t = r0 ? 0.0 : w * g * g * eps;
-        const double gb0 = g * b0, gb1 = g * b1, gb2 = g * b2;
-        d0 = sc * cc;
-        d1 = sc * gb0 + s1 * t;
-        d2 = sc * gb1 + s2 * t;
-        d3 = sc * gb2 + s3 * t;
-        rr = ff * sc;
-      }
       const double *T = lt + q * 128;
-      const double bv0 = d0 * T[col] + d1 * T[32 + col] + d2 * T[64 + col] + d3 * T[96 + col];
-      double bv1 = d0 * T[16 + col] + d1 * T[48 + col] + d2 * T[80 + col] + d3 * T[112 + col];
-      bv1 = rhs_col ? rr : bv1;
+      const double t0 = T[col], t1 = T[32 + col], t2 = T[64 + col], t3 = T[96 + col];
+      const double u0 = T[16 + col], u1 = T[48 + col], u2 = T[80 + col], u3 = T[112 + col];
       const double av0 = T[comp * 32 + col], av1 = T[comp * 32 + 16 + col];
-      a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, a00, 0, 0, 0);
-      a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, a01, 0, 0, 0);
-      a10 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, a10, 0, 0, 0);
-      a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, a11, 0, 0, 0);
-    }
-    double *K = kc + cell * (27 * kKcRow);
+      const int qn = q + 1 < nq ? q + 1 : q;
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int a0 = comp + 4 * r, a1 = 16 + comp + 4 * r, b0 = col, b1 = 16 + col;
-      K[a0 * kKcRow + b0] = a00[r];
-      if (b1 < kKcRow) K[a0 * kKcRow + b1] = a01[r];
-      if (a1 < 27) {
-        K[a1 * kKcRow + b0] = a10[r];
-        if (b1 < kKcRow) K[a1 * kKcRow + b1] = a11[r];
+      for (int k = 0; k < CPC; k++) {
+        double Qq[8];
+        if (INL) {
+          qp_coeffs(prm, qref, n, cx[k], cy[k], cz[k], q, form, Qq);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; e++) Qq[e] = Qn[k][e];
+#pragma unroll
+          for (int e = 0; e < 8; e++) Qn[k][e] = qp[(cell[k] * nq + qn) * 8 + e];
+        }
+        const double w = Qq[0], eps = Qq[1], cc = Qq[5], ff = Qq[6], dl = Qq[7];
+        const double b0 = Qq[2], b1 = Qq[3], b2 = Qq[4];
+        // row comp of D = s v + t e_comp with v = (c, g b): s = w for row 0 and
+        // w g delta b_comp below it, t = w g^2 eps on the diagonal of rows 1-3;
+        // the right-hand-side weight is f s
+        double d0, d1, d2, d3, rr;
+        if (MASS) {
+          d0 = r0 ? w : 0.0;
+          d1 = d2 = d3 = 0.0;
+          // right-hand-side weights w f (1, g delta b): delta = 0 for
+          // MatrixMRhsAssemble, Compute_SDFEM_delta for MatrixMRhsAssemble_SUPG
+          const double be = s1 * b0 + s2 * b1 + s3 * b2;
+          rr = ff * (r0 ? w : w * g * dl * be);
+        } else {
+          const double be = s1 * b0 + s2 * b1 + s3 * b2;  // b_comp (0 for comp 0)
+          const double sc = r0 ? w : w * g * dl * be;
+          const double t = r0 ? 0.0 : w * g * g * eps;
+          const double gb0 = g * b0, gb1 = g * b1, gb2 = g * b2;
+          d0 = sc * cc;
+          d1 = sc * gb0 + s1 * t;
+          d2 = sc * gb1 + s2 * t;
+          d3 = sc * gb2 + s3 * t;
+          rr = ff * sc;
+        }
+        const double bv0 = d0 * t0 + d1 * t1 + d2 * t2 + d3 * t3;
+        double bv1 = d0 * u0 + d1 * u1 + d2 * u2 + d3 * u3;
+        bv1 = rhs_col ? rr : bv1;
+        a00[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv0, a00[k], 0, 0, 0);
+        a01[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av0, bv1, a01[k], 0, 0, 0);
+        a10[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv0, a10[k], 0, 0, 0);
+        a11[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av1, bv1, a11[k], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CPC; k++) {
+      if (grp * CPC + k >= ncells) continue;
+      double *K = kc + cell[k] * (27 * kKcRow);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int a0 = comp + 4 * r, a1 = 16 + comp + 4 * r, b0 = col, b1 = 16 + col;
+        K[a0 * kKcRow + b0] = a00[k][r];
+        if (b1 < kKcRow) K[a0 * kKcRow + b1] = a01[k][r];
+        if (a1 < 27) {
+          K[a1 * kKcRow + b0] = a10[k][r];
+          if (b1 < kKcRow) K[a1 * kKcRow + b1] = a11[k][r];
+        }
       }
     }
   }
@@ -1680,10 +1717,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int f
 // Row gather from the cell buffer: the thread of active node p adds row a of
 // each incident cell's local matrix into its CSS row and sums the rhs column.
 // The row's stencil slots are accumulated in registers over its incident
-// cells (in the order (iz, iy, ix)) and every slot is stored once, after the last cell that touches it (a slot (dx, dy, dz)
-// gets its last contribution from the cell on the positive side of each
-// offset that is >= 0), so the values are written without being read and
-// without a prior clear.  The local node of the row in cell (ix, iy, iz) and
+// cells (in the order (iz, iy, ix)) and every slot is stored once, after the
+// last cell that touches it (a slot (dx, dy, dz) gets its last contribution
+// from the cell on the positive side of each offset that is >= 0), so the
+// values are written without being read and without a prior clear.  The local node of the row in cell (ix, iy, iz) and
 // the slot of every cell entry are compile-time constants: along an axis
 // with extent 2 (even lattice coordinate, a vertex plane) the node is local
 // coordinate 2 - 2 i of cell i, along an axis with extent 1 local coordinate 1.
@@ -1816,11 +1853,12 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
   }
   if (kc && D.order == 2 && !val2) {
     // local matrices on MFMA, then the row gather from the cell buffer
-    const unsigned grid = (unsigned)std::min<int64_t>((ncells + kCellWaves - 1) / kCellWaves, 4096);
+    const int64_t ngroups = (ncells + kCellsPerPass - 1) / kCellsPerPass;
+    const unsigned grid = (unsigned)std::min<int64_t>((ngroups + kCellWaves - 1) / kCellWaves, 4096);
     const size_t lds = (size_t)nq * 128 * sizeof(double);
 #define PMX_CELL(IN, MS)                                                                                   \
-  hipLaunchKernelGGL((k_cell_mfma<IN, MS>), dim3(grid), dim3(kCellThreads), lds, st, D.n, nq, form, prm_dev, qref,   \
-                     qpbuf, tab, kc)
+  hipLaunchKernelGGL((k_cell_mfma<IN, MS, kCellsPerPass>), dim3(grid), dim3(kCellThreads), lds, st, D.n, nq, form,   \
+                     prm_dev, qref, qpbuf, tab, kc)
     const bool ms = form == PMX_FORM_MASS || form == PMX_FORM_MASS_SUPG;
     if (inl) {
       if (ms) PMX_CELL(true, true);
