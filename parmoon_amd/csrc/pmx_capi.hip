@@ -84,6 +84,7 @@ struct Level {
   DevBuf<int> boundary, segidx;
   DevBuf<DevSegment> segs;
   DevBuf<BlockInfo> blocks_all, blocks_colour;
+  DevBuf<RowBlock> rows_all, rows_colour;  // the same with the class records inline
   DevBuf<int> colour_off, colour_seg;
   DevBuf<int> lat2dof, dof2lat, rowptr, kcol;
   DevBuf<double> val2;                // second matrix (SUPG K)
@@ -254,9 +255,16 @@ static void setup_level(System &Y, Level &lv, int nref) {
     std::stable_sort(v.begin(), v.end(), [&](const BlockInfo &a, const BlockInfo &b) { return key(a) < key(b); });
     return v;
   };
+  auto inline_records = [&](const std::vector<BlockInfo> &v) {
+    std::vector<RowBlock> r(v.size());
+    for (size_t i = 0; i < v.size(); i++) r[i] = RowBlock{ds[v[i].seg], v[i].base, 0};
+    return r;
+  };
   {
     const std::vector<BlockInfo> ba = spatial(L.blocks_all);
     lv.blocks_all.upload(ba.data(), ba.size(), st);
+    const std::vector<RowBlock> ra = inline_records(ba);
+    lv.rows_all.upload(ra.data(), ra.size(), st);
   }
   std::vector<BlockInfo> bc;
   D.colour_off.assign(L.ncolours + 1, 0);
@@ -267,6 +275,10 @@ static void setup_level(System &Y, Level &lv, int nref) {
   }
   D.colour_off[L.ncolours] = (int)bc.size();
   lv.blocks_colour.upload(bc.data(), bc.size(), st);
+  {
+    const std::vector<RowBlock> rc = inline_records(bc);
+    lv.rows_colour.upload(rc.data(), rc.size(), st);
+  }
   lv.colour_off.upload(D.colour_off.data(), D.colour_off.size(), st);
   {
     // segments are stored colour-major: colour c owns segments cs[c] .. cs[c+1]-1
@@ -289,6 +301,8 @@ static void setup_level(System &Y, Level &lv, int nref) {
   D.segs = lv.segs.p; D.segidx = lv.segidx.p;
   D.blocks_all = lv.blocks_all.p; D.nblocks_all = (int)L.blocks_all.size();
   D.blocks_colour = lv.blocks_colour.p;
+  D.rows_all = lv.rows_all.p;
+  D.rows_colour = lv.rows_colour.p;
   D.colour_off_dev = lv.colour_off.p;
 
   lv.rhs.alloc(L.N);

@@ -66,6 +66,13 @@ struct BlockInfo {
   int base;
 };
 
+// a block entry of the row kernels with its class record inline, so that a
+// workgroup finds its rows with one load instead of two dependent ones
+struct RowBlock {
+  DevSegment s;
+  int base, pad;
+};
+
 struct Layout {
   int nref = 0, order = 1, n = 0, M = 0, S = 2;
   int64_t N = 0;            // lattice nodes = DOFs

@@ -133,13 +133,12 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
 // R lanes per row, kRowsPerBlock rows per workgroup; with one lane per row
 // the compiler is held to 128 registers so that 4 waves share a SIMD.
 template <typename T, int MODE, int R>
-__global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(const BlockInfo *__restrict__ blocks,
-                                                 const DevSegment *__restrict__ segs, int M,
+__global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(const RowBlock *__restrict__ rows, int M,
                                                  const T *__restrict__ val, const T *x, const T *__restrict__ f,
                                                  T *y, double omega) {
-  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
-  const DevSegment s = segs[bi.seg];
-  const int idx = bi.base + threadIdx.x / R;
+  const RowBlock &rb = rows[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = rb.s;
+  const int idx = rb.base + threadIdx.x / R;
   const int part = threadIdx.x % R;
   if (idx >= s.count) return;  // the R lanes of a row leave together
 #define PMX_ROW(EX, EY, EZ) row_split<T, EX, EY, EZ, R, MODE>(s, idx, part, M, val, x, f, y, omega)
@@ -160,12 +159,11 @@ __global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_rows(cons
 // out as k_rows.
 template <int R>
 __global__ __launch_bounds__(kRowsPerBlock * R, R == 1 ? 4 : 1) void k_sor_scaled(
-    const BlockInfo *__restrict__ blocks, const DevSegment *__restrict__ segs, int M,
-    const _Float16 *__restrict__ val, const float *__restrict__ dinv, float *x, const float *__restrict__ f,
-    double omega) {
-  const BlockInfo bi = blocks[xcd_remap(blockIdx.x, gridDim.x)];
-  const DevSegment s = segs[bi.seg];
-  const int idx = bi.base + threadIdx.x / R;
+    const RowBlock *__restrict__ rows, int M, const _Float16 *__restrict__ val, const float *__restrict__ dinv,
+    float *x, const float *__restrict__ f, double omega) {
+  const RowBlock &rb = rows[xcd_remap(blockIdx.x, gridDim.x)];
+  const DevSegment s = rb.s;
+  const int idx = rb.base + threadIdx.x / R;
   const int part = threadIdx.x % R;
   if (idx >= s.count) return;
 #define PMX_ROW(EX, EY, EZ) row_split<float, EX, EY, EZ, R, 4, false, _Float16>(s, idx, part, M, val, x, f, x, omega, dinv)
@@ -1356,18 +1354,18 @@ static void timer_events(hipEvent_t *e0, hipEvent_t *e1) {
 template <typename T>
 void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *f, T *y, double omega, int colour,
                hipStream_t st) {
-  const BlockInfo *blk = colour < 0 ? D.blocks_all : D.blocks_colour + D.colour_off[colour];
+  const RowBlock *blk = colour < 0 ? D.rows_all : D.rows_colour + D.colour_off[colour];
   const int nblk = colour < 0 ? D.nblocks_all : D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
 #define PMX_ROWS(MODE, R)                                                                                      \
   if (g_timer) {                                                                                               \
     hipEvent_t e0, e1;                                                                                         \
     timer_events(&e0, &e1);                                                                                    \
-    hipExtLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk,       \
-                          D.segs, D.M, val, x, f, y, omega);                                                   \
+    hipExtLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk, D.M,  \
+                          val, x, f, y, omega);                                                                \
   } else                                                                                                       \
-    hipLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val, x, \
-                       f, y, omega)
+    hipLaunchKernelGGL((k_rows<T, MODE, R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.M, val, x, f, y,  \
+                       omega)
 #define PMX_ROWS_R(MODE)                                                                                       \
   switch (row_lanes(D)) {                                                                                      \
     case 1: PMX_ROWS(MODE, 1); break;                                                                          \
@@ -1385,18 +1383,18 @@ void css_apply(const DevLayout &D, int mode, const T *val, const T *x, const T *
 }
 void css_sor_scaled(const DevLayout &D, const _Float16 *val16, const float *dinv, float *x, const float *f,
                     double omega, int colour, hipStream_t st) {
-  const BlockInfo *blk = D.blocks_colour + D.colour_off[colour];
+  const RowBlock *blk = D.rows_colour + D.colour_off[colour];
   const int nblk = D.colour_off[colour + 1] - D.colour_off[colour];
   if (nblk == 0) return;
 #define PMX_ROWS(R)                                                                                            \
   if (g_timer) {                                                                                               \
     hipEvent_t e0, e1;                                                                                         \
     timer_events(&e0, &e1);                                                                                    \
-    hipExtLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk, D.segs, \
-                          D.M, val16, dinv, x, f, omega);                                                      \
+    hipExtLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, e0, e1, 0, blk, D.M,     \
+                          val16, dinv, x, f, omega);                                                           \
   } else                                                                                                       \
-    hipLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.segs, D.M, val16,   \
-                       dinv, x, f, omega)
+    hipLaunchKernelGGL((k_sor_scaled<R>), dim3(nblk), dim3(kRowsPerBlock * R), 0, st, blk, D.M, val16, dinv, x,  \
+                       f, omega)
 #ifdef PMX_SCALED_LANES
   const int lanes = PMX_SCALED_LANES;  // a fixed lane count (tuning builds)
 #else

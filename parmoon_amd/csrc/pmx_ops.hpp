@@ -21,6 +21,8 @@ struct DevLayout {
   BlockInfo *blocks_all = nullptr;  // all rows, spatial order (z, y of the first row)
   int nblocks_all = 0;
   BlockInfo *blocks_colour = nullptr;
+  const RowBlock *rows_all = nullptr;     // blocks_all with the class records inline (row kernels)
+  const RowBlock *rows_colour = nullptr;  // blocks_colour likewise
   std::vector<int> colour_off;  // host: block offsets per colour
   int *colour_off_dev = nullptr;  // the same on the device
   int *colour_seg_dev = nullptr;  // first segment of each colour (segments are colour-major), ncolours + 1
