@@ -317,8 +317,8 @@ def main():
         if getattr(args, k) is None:
             setattr(args, k, v)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(a z-slab decomposition over several GPUs is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.transient > 0:
         transient(args)
