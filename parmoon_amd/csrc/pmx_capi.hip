@@ -1250,12 +1250,28 @@ static Level &level_of(System &Y, int level) {
   if (level < 0 || level >= (int)Y.lev.size()) throw std::invalid_argument("level out of range");
   return *Y.lev[level];
 }
+// the level's CSR structure, built on the device (device_csr); the host copy
+// (Numbering::rowptr / kcol) only where a caller needs host arrays
 static void ensure_csr_dev(System &Y, Level &lv) {
-  if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
   if (!lv.rowptr.p) {
-    lv.rowptr.upload(lv.Nb.rowptr.data(), lv.Nb.rowptr.size(), Y.ctx->st);
-    lv.kcol.upload(lv.Nb.kcol.data(), lv.Nb.kcol.size(), Y.ctx->st);
+    lv.rowptr.alloc(lv.L.N + 1);
+    lv.kcol.alloc(csr_nnz(lv.L));
+    device_csr(lv.L, lv.lat2dof.p, lv.dof2lat.p, lv.rowptr.p, lv.kcol.p, Y.ctx->st);
   }
+}
+static void ensure_csr_host(System &Y, Level &lv) {
+  if (lv.Nb.have_csr) return;
+  ensure_csr_dev(Y, lv);
+  lv.Nb.rowptr.resize(lv.L.N + 1);
+  lv.Nb.kcol.resize(csr_nnz(lv.L));
+  check(hipMemcpyAsync(lv.Nb.rowptr.data(), lv.rowptr.p, sizeof(int) * lv.Nb.rowptr.size(), hipMemcpyDeviceToHost,
+                       Y.ctx->st),
+        "csr download");
+  check(hipMemcpyAsync(lv.Nb.kcol.data(), lv.kcol.p, sizeof(int) * lv.Nb.kcol.size(), hipMemcpyDeviceToHost,
+                       Y.ctx->st),
+        "csr download");
+  check(hipStreamSynchronize(Y.ctx->st), "csr download");
+  lv.Nb.have_csr = true;
 }
 // host (reference numbering) <-> device lattice vector
 template <typename T>
@@ -1457,8 +1473,7 @@ int pmx_system_destroy(pmx_system *sys) {
 int pmx_system_dims(pmx_system *sys, int level, long long *d) {
   PMX_TRY({
     Level &lv = level_of(sys->s, level);
-    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
-    d[0] = lv.L.N; d[1] = lv.L.nactive; d[2] = lv.Nb.ncells; d[3] = lv.Nb.rowptr[lv.L.N];
+    d[0] = lv.L.N; d[1] = lv.L.nactive; d[2] = lv.Nb.ncells; d[3] = csr_nnz(lv.L);
     d[4] = lv.Nb.nb; d[5] = lv.L.M; d[6] = lv.L.nvals; d[7] = lv.L.ncolours;
   })
 }
@@ -1472,7 +1487,7 @@ int pmx_space_numbering(pmx_system *sys, int level, int *bi, int *gn) {
 int pmx_space_structure(pmx_system *sys, int level, int *rp, int *kc) {
   PMX_TRY({
     Level &lv = level_of(sys->s, level);
-    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
+    if (rp || kc) ensure_csr_host(sys->s, lv);
     if (rp) std::memcpy(rp, lv.Nb.rowptr.data(), sizeof(int) * lv.Nb.rowptr.size());
     if (kc) std::memcpy(kc, lv.Nb.kcol.data(), sizeof(int) * lv.Nb.kcol.size());
   })
@@ -1820,7 +1835,7 @@ int pmx_matrix_entries(pmx_system *sys, int level, int which, double *entries) {
       throw std::invalid_argument("which: 0 operator, 1 K, 2 mass, 3 stiffness, 4 FP32 operator, 5 FP16 smoother");
     ensure_csr_dev(Y, lv);
     DevBuf<double> out;
-    out.alloc(lv.Nb.kcol.size());
+    out.alloc(csr_nnz(lv.L));
     if (which == 4) {
       ensure_val32(Y);
       css_to_csr<float>(lv.D, lv.rowptr.p, lv.kcol.p, lv.dof2lat.p, lv.L.nactive, lv.v32.val.p, out.p, Y.ctx->st);
@@ -1844,8 +1859,8 @@ int pmx_level_set_entries(pmx_system *sys, int level, int n, const int *rowptr, 
     if (!Y.assembled) throw std::logic_error("assemble first");
     if (!rowptr || !kcol || !entries) throw std::invalid_argument("NULL argument");
     Level &lv = level_of(Y, level);
-    if (!lv.Nb.have_csr) build_csr(lv.Nb, lv.L);
     if (n != (int)lv.L.N) throw std::invalid_argument("n differs from the level's DOFs");
+    ensure_csr_host(Y, lv);
     // the pattern must be the level's own (TSquareStructure3D of its space)
     if (std::memcmp(rowptr, lv.Nb.rowptr.data(), sizeof(int) * (n + 1)) != 0 ||
         std::memcmp(kcol, lv.Nb.kcol.data(), sizeof(int) * lv.Nb.kcol.size()) != 0)

@@ -37,6 +37,10 @@ void check(hipError_t e, const char *what);
 // dof2lat (N each), GlobalNumbers (cells x local DOFs, optional) and the
 // per-cell frame codes (n^3), all device arrays; synchronises the stream
 void device_numbering(const Layout &L, int *lat2dof, int *dof2lat, int *gn, uint8_t *corner, hipStream_t st);
+// the CSR structure of build_csr on the device (pmx_setup.hip): rowptr
+// (N + 1) and kcol (csr_nnz(L)) from the device numbering; synchronises st
+void device_csr(const Layout &L, const int *lat2dof, const int *dof2lat, int *rowptr, int *kcol, hipStream_t st);
+int64_t csr_nnz(const Layout &L);
 
 // event pairs of timed dispatches (see set_dispatch_timer)
 struct DispatchTimer {

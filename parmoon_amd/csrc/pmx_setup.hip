@@ -13,6 +13,7 @@
 //     position order (two exclusive scans), which numbers inner nodes first
 //     and Dirichlet nodes after them, each class in first-encounter order;
 //  4. GlobalNumbers and the inverse permutation by gathers.
+// and the CSR structure of a level on the device (device_csr).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -116,6 +117,94 @@ __global__ __launch_bounds__(kBlock) void k_inverse_perm(const int *__restrict__
                                                          int *__restrict__ dof2lat) {
   const int l = blockIdx.x * kBlock + threadIdx.x;
   if (l < n) dof2lat[lat2dof[l]] = l;
+}
+
+// ---- the CSR structure (TSquareStructure3D + Sort) in the reference numbering
+// (build_csr's result): row r of an interior node holds the DOFs of the
+// node's stencil box, ascending; a Dirichlet row only its diagonal.
+__device__ __forceinline__ int stencil_ext(int order, int c) { return (order == 2 && (c & 1) == 0) ? 2 : 1; }
+// row lengths into len[r + 1] (len[0] = 0): the inclusive scan gives RowPtr
+__global__ __launch_bounds__(kBlock) void k_csr_len(int M, int order, const int *__restrict__ dof2lat, int64_t nact,
+                                                    int64_t N, int *__restrict__ len) {
+  const int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (r == 0) len[0] = 0;
+  if (r >= N) return;
+  int n = 1;
+  if (r < nact) {
+    const int l = dof2lat[r];
+    const int x = l % M, y = (l / M) % M, z = l / (M * M);
+    n = (2 * stencil_ext(order, x) + 1) * (2 * stencil_ext(order, y) + 1) * (2 * stencil_ext(order, z) + 1);
+  }
+  len[r + 1] = n;
+}
+// one wavefront per row: the row's column numbers go to LDS, then every
+// entry's position is its rank among them (the numbers of a row are
+// distinct), which sorts the row without a sorting pass
+constexpr int kCsrRowsPerBlock = kBlock / 64;
+__global__ __launch_bounds__(kBlock) void k_csr_cols(int M, int order, const int *__restrict__ dof2lat,
+                                                     const int *__restrict__ lat2dof,
+                                                     const int *__restrict__ rowptr, int64_t nact, int64_t N,
+                                                     int *__restrict__ kcol) {
+  __shared__ int sh[kCsrRowsPerBlock][128];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t r = blockIdx.x * (int64_t)kCsrRowsPerBlock + w;
+  if (r >= N) return;  // the whole wavefront
+  const int b = rowptr[r];
+  if (r >= nact) {
+    if (lane == 0) kcol[b] = (int)r;
+    return;
+  }
+  const int l = dof2lat[r];
+  const int x = l % M, y = (l / M) % M, z = l / (M * M);
+  const int ex = stencil_ext(order, x), ey = stencil_ext(order, y), ez = stencil_ext(order, z);
+  const int wx = 2 * ex + 1, wy = 2 * ey + 1, n = wx * wy * (2 * ez + 1);
+  int *row = sh[w];
+  for (int t = lane; t < n; t += 64) {
+    const int dx = t % wx - ex, dy = (t / wx) % wy - ey, dz = t / (wx * wy) - ez;
+    row[t] = lat2dof[(x + dx) + M * ((y + dy) + M * (z + dz))];
+  }
+  // the row's LDS entries are written and read by this wavefront only
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int t = lane; t < n; t += 64) {
+    const int v = row[t];
+    int rank = 0;
+    for (int u = 0; u < n; u++) rank += row[u] < v ? 1 : 0;
+    kcol[b + rank] = v;
+  }
+}
+int64_t csr_nnz(const Layout &L) {
+  int64_t nnz = L.N - L.nactive;
+  for (const Segment &g : L.segs) nnz += g.count * g.ns;
+  return nnz;
+}
+void device_csr(const Layout &L, const int *lat2dof, const int *dof2lat, int *rowptr, int *kcol, hipStream_t st) {
+  const int64_t N = L.N;
+  if (csr_nnz(L) > INT_MAX) throw std::invalid_argument("device_csr: more than 2^31 - 1 entries");
+  const unsigned g = (unsigned)((N + kBlock - 1) / kBlock);
+  int *len = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+  auto release = [&]() {
+    if (len) (void)hipFree(len);
+    if (tmp) (void)hipFree(tmp);
+  };
+  try {
+    check(hipMalloc(&len, sizeof(int) * (N + 1)), "csr: lengths");
+    hipLaunchKernelGGL(k_csr_len, dim3(g), dim3(kBlock), 0, st, L.M, L.order, dof2lat, L.nactive, N, len);
+    check(hipcub::DeviceScan::InclusiveSum(nullptr, tmp_bytes, len, rowptr, (int)(N + 1), st), "csr: scan size");
+    check(hipMalloc(&tmp, tmp_bytes), "csr: scan storage");
+    check(hipcub::DeviceScan::InclusiveSum(tmp, tmp_bytes, len, rowptr, (int)(N + 1), st), "csr: scan");
+    const unsigned gr = (unsigned)((N + kCsrRowsPerBlock - 1) / kCsrRowsPerBlock);
+    hipLaunchKernelGGL(k_csr_cols, dim3(gr), dim3(kBlock), 0, st, L.M, L.order, dof2lat, lat2dof, rowptr,
+                       L.nactive, N, kcol);
+    check(hipStreamSynchronize(st), "csr");
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
 }
 
 void device_numbering(const Layout &L, int *lat2dof, int *dof2lat, int *gn, uint8_t *corner, hipStream_t st) {
