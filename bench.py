@@ -47,8 +47,8 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, TFLOP/s
 # WRITE_SIZE x 1.000 passes over tests/cpp/kernel_probe (the factors make the
 # counters of a 1 GiB device copy read 1 GiB each), in
 # profiles/k_rows_ssor_spmv_fp32_q2_64_hbm_traffic.md:
-TRAFFIC_SSOR_LAUNCH = 30_777_365  # k_rows<float,3,1>, one colour launch (average over the sweeps)
-TRAFFIC_SPMV = 555_854_976  # k_rows<float,0,1>
+TRAFFIC_SSOR_LAUNCH = 30_917_171  # k_rows<float,3,1>, one colour launch (average over the sweeps)
+TRAFFIC_SPMV = 561_289_188  # k_rows<float,0,1>
 
 
 def cpu_info():
