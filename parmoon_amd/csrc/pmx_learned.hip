@@ -601,7 +601,7 @@ void net_forward_act(LearnedNet &net, int M, bool training, hipStream_t st) {
                      nullptr, nullptr, net.out);
 }
 
-void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st,
+void net_forward_correction(LearnedNet &net, int M, double *nrm, double sqrt_count, float *c, hipStream_t st,
                             const LearnedInput *fused) {
   const unsigned g = conv_grid(M);
   auto P = [&](int l) { return net.params + net_param_offset(l) + net_wsize(l); };  // bias
@@ -609,7 +609,7 @@ void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sq
     // the first layer builds its input (features + scaled defect) itself
     hipLaunchKernelGGL((k_conv<kNetIn, 16, 0, true>), dim3(g), dim3(256), 0, st, M, fused->feat,
                        net.wbf + 0 * kWStride, P(0), nullptr, net.act[1], nullptr, nullptr, sqrt_count, fused->u,
-                       fused->part, fused->np, const_cast<double *>(nrm));
+                       fused->part, fused->np, nrm);
   } else {
     hipLaunchKernelGGL((k_conv<kNetIn, 16, 0>), dim3(g), dim3(256), 0, st, M, net.act[0], net.wbf + 0 * kWStride,
                        P(0), nullptr, net.act[1], nullptr, nullptr, 1.0);

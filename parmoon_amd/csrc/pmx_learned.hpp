@@ -82,7 +82,7 @@ struct LearnedInput {
 };
 // the network's correction c on an M^3 lattice; nrm: ||u|| (read, or with
 // `fused` written by the first layer); the input is net.act[0] unless fused
-void net_forward_correction(LearnedNet &net, int M, const double *nrm, double sqrt_count, float *c, hipStream_t st,
+void net_forward_correction(LearnedNet &net, int M, double *nrm, double sqrt_count, float *c, hipStream_t st,
                             const LearnedInput *fused = nullptr);
 // u = dinv d and the partials of ||u||^2 (learned_input_parts(n) of them)
 void learned_input_sq(const float *dinv, const float *d, float *u, int64_t n, double *part, hipStream_t st);
