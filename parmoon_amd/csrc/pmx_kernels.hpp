@@ -28,6 +28,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int G) {
 __device__ __forceinline__ int64_t vbase(const DevSegment &s, int idx) {
   return s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice) + (idx % kSlice);
 }
+// The row-scaled FP16 smoother values are stored with the slots of a row in
+// groups of eight: inside a slice of kSlice rows, group o holds, row by row,
+// the row's slots 8o .. 8o+7 (16 contiguous bytes per row, a last group of
+// ns mod 8 slots likewise), so a row's values arrive in 16-byte loads.  The
+// slice sizes are those of the FP32 values.
+__device__ __forceinline__ int64_t v16slice(const DevSegment &s, int idx) {
+  return s.voff + (int64_t)(idx / kSlice) * ((int64_t)s.ns * kSlice);
+}
+__host__ __device__ constexpr int v16off(int ns, int r, int slot) {
+  return (slot >> 3) * 8 * kSlice + r * ((slot >> 3) < (ns >> 3) ? 8 : (ns & 7)) + (slot & 7);
+}
 
 // ---------------------------------------------------------------- BLAS-1
 template <typename T>

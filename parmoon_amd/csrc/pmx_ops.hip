@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <stdexcept>
+#include <type_traits>
 #include <utility>
 
 #include "pmx_functors.hpp"
@@ -79,7 +80,10 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
   const int t = idx / s.cx;
   const int j = t % s.cy, k = t / s.cy;
   const int p = (s.x0 + s.stride * i) + M * ((s.y0 + s.stride * j) + M * (s.z0 + s.stride * k));
-  const V *v = val + vbase(s, idx);
+  constexpr int NS = WX * WY * WZ;
+  // MODE 4: the FP16 values' slice and the row within it (v16off layout)
+  const V *v = MODE == 4 ? val + v16slice(s, idx) : val + vbase(s, idx);
+  const int rs = idx % kSlice;
   A fp = 0, diag = 1, told = 0;
   if constexpr (MODE >= 1) fp = (A)f[p];
   if constexpr (MODE == 4) {
@@ -110,7 +114,10 @@ __device__ __forceinline__ void row_split(const DevSegment &s, int idx, int part
       load_run<T, WX>(q, xr);
     }
 #pragma unroll
-    for (int e = 0; e < WX; e++) vr[e] = v[(g * WX + e) * kSlice];
+    for (int e = 0; e < WX; e++) {
+      if constexpr (MODE == 4) vr[e] = v[v16off(NS, rs, g * WX + e)];
+      else vr[e] = v[(g * WX + e) * kSlice];
+    }
 #pragma unroll
     for (int e = 0; e < WX; e++) {
       const bool use = ok && !(MODE >= 2 && e == EX && g == DG);
@@ -200,9 +207,10 @@ __global__ __launch_bounds__(kRowsPerBlock) void k_scaled_values(const BlockInfo
   const float d = val[vb + (int64_t)s.dslot * kSlice];
   const float r = 1.0f / d;
   dinv[p] = r;
+  const int64_t sb = v16slice(s, idx);
   for (int q = 0; q < s.ns; q++) {
     const float w = val[vb + (int64_t)q * kSlice] * r;
-    val16[vb + (int64_t)q * kSlice] = (_Float16)fminf(fmaxf(w, -65504.0f), 65504.0f);
+    val16[sb + v16off(s.ns, idx % kSlice, q)] = (_Float16)fminf(fmaxf(w, -65504.0f), 65504.0f);
   }
 }
 
@@ -916,7 +924,10 @@ __global__ void k_css_to_csr(const int *__restrict__ rowptr, const int *__restri
     const int q = dof2lat[kcol[n]];
     const int dx = q % M - x, dy = (q / M) % M - y, dz = q / (M * M) - z;
     const int slot = (dx + s.ex) + (2 * s.ex + 1) * ((dy + s.ey) + (2 * s.ey + 1) * (dz + s.ez));
-    out[n] = (double)val[vbase(s, idx) + (int64_t)slot * kSlice];
+    if constexpr (std::is_same<T, _Float16>::value)
+      out[n] = (double)val[v16slice(s, idx) + v16off(s.ns, idx % kSlice, slot)];
+    else
+      out[n] = (double)val[vbase(s, idx) + (int64_t)slot * kSlice];
   }
 }
 
