@@ -182,6 +182,8 @@ int64_t csr_nnz(const Layout &L) {
 void device_csr(const Layout &L, const int *lat2dof, const int *dof2lat, int *rowptr, int *kcol, hipStream_t st) {
   const int64_t N = L.N;
   if (csr_nnz(L) > INT_MAX) throw std::invalid_argument("device_csr: more than 2^31 - 1 entries");
+  // a row's columns are staged in 128 LDS entries: at most 5^3 (Q2)
+  if (L.order != 1 && L.order != 2) throw std::invalid_argument("device_csr: order must be 1 or 2");
   const unsigned g = (unsigned)((N + kBlock - 1) / kBlock);
   int *len = nullptr;
   void *tmp = nullptr;
