@@ -759,7 +759,8 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
   // Q1 rows of the one-matrix forms (27 slots) are accumulated in registers
   // over the 8 cells (loops unrolled, every slot a compile-time index) and
   // stored once; Q2 rows and the two-matrix form (whose unrolled loops need
-  // too many registers) add each cell's contribution into the cleared values
+  // too many registers) add each cell's contribution into the values (Q2:
+  // cleared beforehand; Q1: the first cell of a slot stores)
   constexpr bool REG = ORDER == 1 && !TWO;
   constexpr int UNR = REG ? 2 : 1;
   double racc[REG ? 27 : 1];
@@ -843,6 +844,12 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
           const int slot = (dx + EX) + (2 * EX + 1) * ((dy + EY) + (2 * EY + 1) * (dz + EZ));
           if constexpr (REG) {
             racc[slot] += acc[b];
+          } else if (ORDER == 1 && (ix == 0 || bx == 1) && (iy == 0 || by == 1) && (iz == 0 || bz == 1)) {
+            // Q1: the first of the row's cells (in loop order) that meets
+            // the slot, (max(0, dx), max(0, dy), max(0, dz)), stores; the
+            // later ones add, so the values need no clearing beforehand
+            v[slot * cnt] = acc[b];
+            if (TWO) v2[slot * cnt] = acc2[b];
           } else {
             v[slot * cnt] += acc[b];
             if (TWO) v2[slot * cnt] += acc2[b];
@@ -1883,9 +1890,10 @@ void assemble(const DevLayout &D, const double *prm_dev, int example, int form, 
                        D.nboundary, D.M, prm_dev, rhs);
     return;
   }
-  // the per-row evaluation of Q2 rows and of the two-matrix form adds every
-  // cell's contribution into the values (the other Q1 rows are stored once)
-  if (D.order == 2 || val2) {
+  // the per-row evaluation of Q2 rows adds every cell's contribution into
+  // the cleared values (Q1 rows store each slot first; the padding rows of
+  // the slices are never written and keep the zeros of the allocation)
+  if (D.order == 2) {
     check(hipMemsetAsync(val, 0, sizeof(double) * D.nvals, st), "memset val");
     if (val2) check(hipMemsetAsync(val2, 0, sizeof(double) * D.nvals, st), "memset val2");
   }
