@@ -41,6 +41,11 @@ __host__ __device__ inline double grf_field(const double *prm, double x, double 
   return s * 0.17677669529663688110021109052621;  // sqrt(2/64)
 }
 
+// CHEAP = true: only the examples whose coefficients are evaluated inline
+// at every point (Laplace, sine, heat, rotating); the curl-Fourier and
+// lognormal branches, which kernels for the inline examples never take, are
+// left out of the code so they do not weigh on its registers
+template <bool CHEAP = false>
 __host__ __device__ inline QpCoeffs example_coeffs(const double *prm, double x, double y, double z) {
   const double pi = 3.14159265358979323846;
   QpCoeffs k;
@@ -62,22 +67,26 @@ __host__ __device__ inline QpCoeffs example_coeffs(const double *prm, double x, 
     const double cx = cos(pi * x), cy = cos(pi * y), cz = cos(pi * z);
     const double u = sx * sy * sz;
     k.f = 3 * pi * pi * eps * u + pi * (k.b0 * cx * sy * sz + k.b1 * sx * cy * sz + k.b2 * sx * sy * cz) + k.c * u;
-  } else if (ex == PMX_EX_CURL_FOURIER) {
-    double b[3];
-    fourier_b(prm, x, y, z, b);
-    k.b0 = b[0];
-    k.b1 = b[1];
-    k.b2 = b[2];
-    k.c = 0;
-    k.f = 1.0;
+  } else if (!CHEAP && ex == PMX_EX_CURL_FOURIER) {
+    if constexpr (!CHEAP) {
+      double b[3];
+      fourier_b(prm, x, y, z, b);
+      k.b0 = b[0];
+      k.b1 = b[1];
+      k.b2 = b[2];
+      k.c = 0;
+      k.f = 1.0;
+    }
   } else if (ex == PMX_EX_HEAT_AMC) {
     k.eps = 1.0;
     k.b0 = k.b1 = k.b2 = 0;
     k.c = 0;
     k.f = (3. * pi * pi - 0.1) * exp(-0.1 * t) * sin(pi * x) * cos(pi * y) * cos(pi * z);
-  } else if (ex == PMX_EX_LOGNORMAL) {
-    k.eps = eps * exp(prm[PMX_PRM_GRF_SIGMA] * grf_field(prm, x, y, z));
-    k.f = 1.0;
+  } else if (!CHEAP && ex == PMX_EX_LOGNORMAL) {
+    if constexpr (!CHEAP) {
+      k.eps = eps * exp(prm[PMX_PRM_GRF_SIGMA] * grf_field(prm, x, y, z));
+      k.f = 1.0;
+    }
   } else if (ex == PMX_EX_ROTATING) {
     k.b0 = -(y - 0.5);
     k.b1 = x - 0.5;

@@ -516,6 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(T *__restrict__ rc, const T
 // with_delta = false: the record without its stabilisation parameter
 // (o[7] = 0), for callers that read it from the per-point buffer of
 // k_qp_delta
+template <bool CHEAP = false>
 __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const double *__restrict__ qref, int n,
                                           int cx, int cy, int cz, int q, int form, double *o,
                                           const uint8_t *__restrict__ corner = nullptr, bool with_delta = true) {
@@ -525,7 +526,7 @@ __device__ __forceinline__ void qp_coeffs(const double *__restrict__ prm, const 
   const double X = (cx * h + 0.5 * h) + 0.5 * h * xi;
   const double Y = (cy * h + 0.5 * h) + 0.5 * h * eta;
   const double Z = (cz * h + 0.5 * h) + 0.5 * h * zeta;
-  const QpCoeffs c = example_coeffs(prm, X, Y, Z);
+  const QpCoeffs c = example_coeffs<CHEAP>(prm, X, Y, Z);
   if (!with_delta) {
     o[0] = w * (h * h * h / 8.0);
     o[1] = c.eps; o[2] = c.b0; o[3] = c.b1; o[4] = c.b2; o[5] = c.c; o[6] = c.f; o[7] = 0.0;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_qp_delta(const double *__restrict__ 
   const int cx = t % n, cy = t / n;
   const int cz = blockIdx.y % n, q = blockIdx.y / n;
   double o[8];
-  qp_coeffs(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], o);
+  qp_coeffs<true>(prm, qref, n, cx, cy, cz, q, (int)prm[PMX_PRM_FORM], o);
   dl[(int64_t)q * n * n * n + t + (int64_t)n * n * cz] = o[7];
 }
 // forms whose records carry a stabilisation parameter
@@ -793,7 +794,7 @@ __device__ __forceinline__ void assemble_row(const DevSegment &s, int idx, int M
         for (int q = 0; q < nq; q++) {
           double Qq[8];
           if (INL) {
-            qp_coeffs(prm, qref, n, cx, cy, cz, q, form, Qq, nullptr, false);
+            qp_coeffs<true>(prm, qref, n, cx, cy, cz, q, form, Qq, nullptr, false);
             Qq[7] = dl_next;
             if (qp) dl_next = qp[(q + 1 < nq ? q + 1 : q) * ncell + cell];
           } else {
@@ -1673,7 +1674,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cell_mfma(int n, int nq, int f
       for (int k = 0; k < CPC; k++) {
         double Qq[8];
         if (INL) {
-          qp_coeffs(prm, qref, n, cx[k], cy[k], cz[k], q, form, Qq);
+          qp_coeffs<true>(prm, qref, n, cx[k], cy[k], cz[k], q, form, Qq);
         } else {
 #pragma unroll
           for (int e = 0; e < 8; e++) Qq[e] = Qn[k][e];
