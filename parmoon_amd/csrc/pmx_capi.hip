@@ -1766,8 +1766,10 @@ static void tcd_systmat(System &Y) {
           tau * th2, Y.b.p, st);
   for (auto &lp : Y.lev) {
     Level &l = *lp;
-    system_matrix(l.v64.val.p, l.mass.p, supg ? l.val2.p : nullptr, l.stiff.p, tau * th1, l.L.nvals, st);
-    l.have_val32 = false;
+    // the FP32 copy (once the FP32 forms have allocated it) in the same pass
+    system_matrix(l.v64.val.p, l.v32.val.p, l.mass.p, supg ? l.val2.p : nullptr, l.stiff.p, tau * th1, l.L.nvals,
+                  st);
+    l.have_val32 = l.v32.val.p != nullptr;
     l.have_val16 = false;
     l.have_dinv = false;
   }

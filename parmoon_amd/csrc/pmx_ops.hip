@@ -2027,14 +2027,17 @@ void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream
   hipLaunchKernelGGL(k_interpolate, dim3(grid_for(D.N)), dim3(kBlock), 0, st, prm_dev, D.M, x);
 }
 // val = mass (+ k) + c a (MatAdd of Blas2.C:128-150 on the stored stencil values)
-__global__ void k_system_matrix(double *__restrict__ val, const double *__restrict__ mass,
+__global__ void k_system_matrix(double *__restrict__ val, float *__restrict__ val32, const double *__restrict__ mass,
                                 const double *__restrict__ k, const double *__restrict__ a, double c, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    val[i] = (k ? mass[i] + k[i] : mass[i]) + c * a[i];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = (k ? mass[i] + k[i] : mass[i]) + c * a[i];
+    val[i] = v;
+    if (val32) val32[i] = (float)v;
+  }
 }
-void system_matrix(double *val, const double *mass, const double *k, const double *a, double c, int64_t n,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_system_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st, val, mass, k, a, c, n);
+void system_matrix(double *val, float *val32, const double *mass, const double *k, const double *a, double c,
+                   int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_system_matrix, dim3(grid_for(n)), dim3(kBlock), 0, st, val, val32, mass, k, a, c, n);
 }
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
              const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st) {

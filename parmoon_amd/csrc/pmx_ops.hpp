@@ -191,8 +191,9 @@ constexpr int64_t kCellMatrixDoubles = 27 * 28;  // cell buffer entries per Q2 c
 // / theta right-hand side (k, k_old: the SUPG time-derivative matrix K and
 // K oldsol, nullptr for the Galerkin forms)
 void interpolate(const DevLayout &D, const double *prm_dev, double *x, hipStream_t st);
-void system_matrix(double *val, const double *mass, const double *k, const double *a, double c, int64_t n,
-                   hipStream_t st);
+// val = mass (+ k) + c a; val32 (if not null) receives its FP32 rounding
+void system_matrix(double *val, float *val32, const double *mass, const double *k, const double *a, double c,
+                   int64_t n, hipStream_t st);
 void tcd_rhs(const DevLayout &D, const double *oldrhs, const double *rhs, const double *m_old, const double *a_old,
              const double *k_old, double t3, double t4, double t2, double *B, hipStream_t st);
 void givens(double *H, double *cs, double *sn, double *s, int i, int ld, double *res, hipStream_t st);
